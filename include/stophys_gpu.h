@@ -1,0 +1,222 @@
+/*
+ * stophys_gpu.h -- C ABI of the MI355X (gfx950) ensemble SDE engine for the
+ * StochasticPhysics particle simulator.
+ *
+ * This is the drop-in boundary that a GFX950 backend of the reference's
+ * SimulationManager binds.  The reference has no C ABI of its own: its entry
+ * point is the C++ template SimulationManager<prec, InstructionSet>
+ * (Particle_Simulation/Simulator/SimulationManager.h:545-615), which builds one
+ * SingleParticleSimulator per particle (SingleParticleSimulator.h:129-159) on a
+ * host thread pool and merges the per-particle trajectories into a moment
+ * weighted mean (Results/MeanSimulationResult.h:87-119).  Every entry point
+ * below names the reference interface it replaces.
+ *
+ * Conventions: no exceptions cross this boundary; every function returns an
+ * sp_status and sp_last_error() gives a thread-local message.  Host buffers
+ * belong to the caller; device memory belongs to the library.
+ */
+#ifndef STOPHYS_GPU_H
+#define STOPHYS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SP_ABI_VERSION 1
+
+/* ---- status codes (the reference throws std::runtime_error instead,
+ *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
+typedef enum sp_status {
+    SP_OK = 0,
+    SP_ERR_INVALID_ARG = 1,
+    SP_ERR_UNSUPPORTED = 2,      /* problem/solver/anisotropy combination not available */
+    SP_ERR_OUT_OF_MEMORY = 3,
+    SP_ERR_DEVICE = 4,
+    SP_ERR_NOT_CONVERGED = 5,    /* ImplicitMidpoint Newton hit Max_Iterations */
+    SP_ERR_CANCELLED = 6,
+    SP_ERR_TIMEOUT = 7,
+    SP_ERR_NOT_FOUND = 8,
+    SP_ERR_NOT_READY = 9
+} sp_status;
+
+/* ---- selector enums: numeric values equal the reference's enums ---- */
+/* Settings::IProblem (Settings/ProblemSettings.h:55-70) */
+enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_NEEL = 3 };
+/* Settings::ISolver (Settings/SolverSettings.h:34-47) */
+enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
+       SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7 };
+/* Properties::IField (Fields/FieldList.h:48-59) */
+enum { SP_FIELD_ZERO = 1, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4 };
+/* Properties::IAnisotropy (Problems/Anisotropy/AnisotropyList.h:45-54) */
+enum { SP_ANISO_UNIAXIAL = 1, SP_ANISO_CUBIC = 2 };
+
+/* precision of the bulk arithmetic (no reference counterpart: PREC double,
+ * General/GeneralDefines.h:3) */
+enum { SP_PREC_FP64 = 0, SP_PREC_MIXED = 1 };
+/* noise source: in-register Philox4x32-10 (replaces SDEFramework/NoiseField.inl:24-44),
+ * or replay of caller-supplied standard normals (parity testing) */
+enum { SP_NOISE_PHILOX = 0, SP_NOISE_REPLAY = 1 };
+
+/* request flags */
+enum {
+    SP_FLAG_BN_SIMPLE_MODEL = 1u << 0,   /* BrownAndNeel Use_simple_model (ProblemSettings.h:126-130) */
+    SP_FLAG_COORD_TRANSFORM = 1u << 1,   /* NeelSpherical Use_coordinate_transformation (ProblemSettings.h:192-205) */
+    SP_FLAG_KEEP_FINAL_STATE = 1u << 2,  /* copy per-particle final state back */
+    SP_FLAG_NO_FALLBACK = 1u << 3,       /* mixed precision: report the check error but never re-run in FP64 */
+    SP_FLAG_SKIP_CHECK = 1u << 4         /* mixed precision: no FP64 shadow check */
+};
+
+/*
+ * One simulation request: the content of one reference parameter file
+ * (Settings/SimulationManagerSettings.h:34-150) for one ensemble whose
+ * particles all share problem, solver, field and anisotropy type.
+ */
+typedef struct sp_request {
+    int32_t problem;          /* SP_PROBLEM_* */
+    int32_t solver;           /* SP_SOLVER_* */
+    int32_t field;            /* SP_FIELD_* */
+    int32_t anisotropy;       /* SP_ANISO_* */
+    int32_t precision;        /* SP_PREC_* */
+    int32_t noise;            /* SP_NOISE_* */
+    uint32_t flags;           /* SP_FLAG_* */
+    uint32_t newton_max_iter; /* Implicit_Midpoint Max_Iterations (SolverSettings.h:121) */
+    uint64_t n_particles;     /* Number_of_simulations (SimulationSettings.h:67-79) */
+    uint64_t n_steps;         /* Number_of_steps; (n_steps/oversampling - 1)*oversampling steps run */
+    uint64_t oversampling;    /* Oversampling */
+    uint64_t seed;            /* Philox key */
+    double dt;                /* Timestep [s] */
+    double newton_tol;        /* Implicit_Midpoint Accuracy_Goal (SolverSettings.h:122) */
+    double min_angle;         /* NeelSpherical Min_angle_transformation [rad] */
+    double check_tol;         /* mixed precision: max |<m>_mixed - <m>_fp64| on the shadow subset */
+    /* field parameters (Properties/Fields/Sinusoidal.hpp:15-35, Lissajous.hpp:15-37):
+     * sinusoidal: H = amplitude*sin(2 pi freq[0] t + phase[0]) + offset
+     * lissajous:  H_i = offset_i + amplitude_i*sin(2 pi freq_i t + phase_i) */
+    double field_offset[3];
+    double field_amplitude[3];
+    double field_freq[3];
+    double field_phase[3];
+    /* SP_NOISE_REPLAY only: standard normals z[k*n_particles + p], k = step*dim_W + j */
+    const double* replay_normals;
+    uint64_t replay_count;    /* number of k values available */
+} sp_request;
+
+/*
+ * Per-particle properties, structure of arrays, n_particles entries each
+ * (Properties/MagneticProperties.h:100-267, HydrodynamicProperties.h:31-83,
+ * ParticleProperties.h:32-127).  The library evaluates the reference's
+ * parameter calculators (Problems/Helpers/ParameterCalculator*.h) from them.
+ */
+typedef struct sp_particles {
+    const double* magnetic_radius;           /* [n] m */
+    const double* saturation_magnetisation;  /* [n] A/m */
+    const double* damping;                   /* [n] */
+    const double* gyromagnetic_ratio;        /* [n] rad/(s T) */
+    const double* temperature;               /* [n] K */
+    const double* anisotropy_constant;       /* [n] K_uniaxial or K_cubic, J/m^3 */
+    const double* hydrodynamic_radius;       /* [n] m (BrownAndNeel only, else NULL) */
+    const double* viscosity;                 /* [n] Pa s (BrownAndNeel only, else NULL) */
+    const double* axes;                      /* [n][9] body axes x,y,z (Neel/NeelSpherical; NULL = identity) */
+    const double* initial_state;             /* [n][dim_y]: m (Neel), (theta,phi) (NeelSpherical), (n,m) (BrownAndNeel) */
+} sp_particles;
+
+/*
+ * Output of one request; the caller allocates the arrays (sizes from
+ * sp_request_shape).  Mirrors Mean_Results {Time, Field, Results, Weight}
+ * (Results/MeanSimulationResult.h:131-141).
+ */
+typedef struct sp_result {
+    double* time;         /* [n_out] window centre times (SingleParticleSimulator.h:236-273) */
+    double* field;        /* [n_out*3] window mean field */
+    double* mean;         /* [n_out*dim_out] weighted ensemble mean */
+    double* weight;       /* [dim_out] sum of weights */
+    double* final_state;  /* [n_particles*dim_y] or NULL */
+    /* filled by the library */
+    uint64_t n_out;
+    uint32_t dim_out;
+    uint32_t dim_y;
+    double device_ms;          /* device time spent on this request's kernels (batch share) */
+    double latency_ms;         /* submit -> result ready */
+    double particle_steps;     /* particles * steps executed */
+    double check_error;        /* mixed: max abs error of the shadow-subset mean vs FP64 (-1 if not checked) */
+    uint32_t used_fallback;    /* 1 if the mixed result failed the check and was recomputed in FP64 */
+    uint32_t precision_used;   /* SP_PREC_* of the returned result */
+    uint64_t newton_failures;  /* ImplicitMidpoint steps that hit Max_Iterations */
+} sp_result;
+
+/* execution knobs chosen by the autotuner (or by hand) */
+typedef struct sp_tuning {
+    uint32_t block_size;       /* work-group size: 64..1024, multiple of 64 */
+    uint32_t particles_per_lane; /* 1 or 2 */
+    uint32_t chunk_steps;      /* steps advanced per request per scheduler iteration / launch */
+    uint32_t use_graph;        /* capture chunk launches in a hipGraph (static batches) */
+    uint32_t persistent_waves_per_simd; /* persistent grid = CUs * 4 * this */
+} sp_tuning;
+
+typedef struct sp_ctx sp_ctx;
+
+/* ---- errors / info ---- */
+const char* sp_last_error(void);
+int sp_abi_version(void);
+
+/* ---- selector strings (reference maps: SolverSettings.cpp:10-20, ProblemSettings.h:63-70,
+ *      FieldList.h:51-59, AnisotropyList.h:48-54); CPU only ---- */
+int sp_parse_enum(const char* kind, const char* name, int32_t* value);
+const char* sp_enum_name(const char* kind, int32_t value);
+
+/* ---- shapes; CPU only ---- */
+int sp_request_shape(const sp_request* req, uint64_t* n_out, uint32_t* dim_y,
+                     uint32_t* dim_w, uint32_t* dim_out);
+
+/* Parameter calculators of the reference (ParameterCalculatorNeel.h:44-92,
+ * ParameterCalculatorBrown.h:39-58, ParameterCalculatorBrownAndNeel.h:67-76,
+ * UniaxialAnisotropy.h:71-85, CubicAnisotropy.h:92-94).  Writes the SoA
+ * parameter block the kernels load, param-major: out[k*n + p].  CPU only. */
+int sp_compute_particle_params(const sp_request* req, const sp_particles* parts,
+                               double* out_params, uint32_t* n_params);
+
+/* ---- context: one per GPU (replaces the ThreadManager of SimulationManager.h:109,546) ---- */
+int sp_ctx_create(int device, sp_ctx** out);
+void sp_ctx_destroy(sp_ctx* ctx);
+int sp_ctx_set_tuning(sp_ctx* ctx, int problem, int solver, int precision, const sp_tuning* t);
+int sp_ctx_get_tuning(sp_ctx* ctx, int problem, int solver, int precision, sp_tuning* t);
+/* device memory budget for admitted requests (state + params + outputs) */
+int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
+
+/* ---- plain path: one request, synchronous.  Replaces SimulationManager::
+ *      StartSimulationManager + waitUntilFinsihed for one ensemble
+ *      (SimulationManager.h:549-584) ---- */
+int sp_run(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_result* res);
+
+/* ---- batched path: many uneven requests in one persistent launch per chunk ---- */
+int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs,
+                 const sp_particles* parts, sp_result* res);
+
+/* ---- serving: continuous batching request loop.  sp_submit copies the inputs;
+ *      sp_wait blocks; sp_result copies the outputs and releases the request ---- */
+int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uint64_t* id);
+int sp_wait(sp_ctx* ctx, uint64_t id, int timeout_ms);
+int sp_result_get(sp_ctx* ctx, uint64_t id, sp_result* res);
+int sp_cancel(sp_ctx* ctx, uint64_t id);   /* replaces SimulationManager::abort (SimulationManager.h:595-599) */
+int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending, iterations,
+                                                 busy_ms, wall_ms, particle_steps, fallbacks */);
+
+/* ---- autotuner: times candidate tunings on a short synthetic run ---- */
+int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts,
+                sp_tuning* best, double* best_rate);
+
+/* ---- device-side test hooks (parity of the noise stream) ---- */
+/* raw Philox4x32-10 blocks: out[4*i..] = philox(ctr = (ctr0 + i, ctr1, ctr2, ctr3), key) */
+int sp_dev_philox(sp_ctx* ctx, const uint32_t ctr[4], const uint32_t key[2], uint64_t count, uint32_t* out);
+/* standard normals z_k, k in [first, first+count), of particle p's stream under seed */
+int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first,
+                   uint64_t count, int precision, double* out);
+/* host copy of the same generator (the library's own host path) */
+int sp_host_normals(uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STOPHYS_GPU_H */

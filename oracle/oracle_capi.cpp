@@ -1,0 +1,214 @@
+// oracle_capi.cpp -- extern "C" surface of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+//
+// Loaded with ctypes by tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg.  The product library never links this file.
+#include "sp_oracle_sim.hpp"
+#include <chrono>
+
+using namespace oracle;
+
+namespace {
+MagneticProperties magFrom(const double* pr, int aniso) {
+    // pr = [radius, Ms, alpha, gamma, T, K, r_h, viscosity]
+    MagneticProperties mp{pr[0], pr[1], pr[2], pr[3], pr[5], aniso};
+    return mp;
+}
+Axes axesFrom(const double* a) {
+    Axes ax;
+    if (!a) { ax.x = v3(1.0, 0.0, 0.0); ax.y = v3(0.0, 1.0, 0.0); ax.z = v3(0.0, 0.0, 1.0); return ax; }
+    ax.x = v3(a[0], a[1], a[2]); ax.y = v3(a[3], a[4], a[5]); ax.z = v3(a[6], a[7], a[8]);
+    return ax;
+}
+}  // namespace
+
+extern "C" {
+
+void or_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+}
+
+void or_normals(uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int libm, double* out) {
+    double z[4]; uint64_t have = ~0ull;
+    for (uint64_t i = 0; i < count; ++i) {
+        const uint64_t k = first + i;
+        if ((k >> 2) != have) { normals4(seed, particle, k >> 2, 0u, z, libm != 0); have = k >> 2; }
+        out[i] = z[k & 3];
+    }
+}
+
+double or_bm_log(double u) { return bm_log(u); }
+void or_bm_sincos2pi(double u, double* s, double* c) { bm_sincos2pi(u, s, c); }
+
+void or_neel_params(double r, double Ms, double alpha, double gamma, double T, double* out) {
+    MagneticProperties mp{r, Ms, alpha, gamma, 0.0, 1};
+    const NeelParams p = calcNeelParams(mp, T);
+    out[0] = p.NeelFactor1; out[1] = p.NeelFactor2; out[2] = p.DriftPrefactor; out[3] = p.NoisePrefactor; out[4] = p.Damping;
+}
+void or_brown_params(double rh, double visc, double T, double* out) {
+    const BrownParams p = calcBrownRotationParams(rh, visc, T);
+    out[0] = p.BrownPrefactor; out[1] = p.BrownDiffusion; out[2] = p.Brown_F_Noise;
+}
+
+// anisotropy field (and Jacobian) at m: aniso 1 uniaxial (easy axis = axes z), 2 cubic
+void or_aniso_field(int aniso, const double* props, const double* m, const double* axes, double* out, double* jac) {
+    const MagneticProperties mp = magFrom(props, aniso);
+    const Axes ax = axesFrom(axes);
+    const Vec<double, 3> mv = v3(m[0], m[1], m[2]);
+    Vec<double, 3> H;
+    Mat<double, 3, 3> J;
+    if (aniso == 1) { Uniaxial u(mp); H = u.field(mv, ax); J = u.jacobian(ax); }
+    else { Cubic c(mp); H = c.field(mv, ax); J = c.jacobian(mv, ax); }
+    for (int i = 0; i < 3; ++i) out[i] = H[i];
+    if (jac) for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) jac[i * 3 + j] = J(i, j);
+}
+
+void or_euler_axes(double phi, double theta, double psi, double* out) {
+    const Axes a = axesFromEuler(phi, theta, psi);
+    for (int i = 0; i < 3; ++i) { out[i] = a.x[i]; out[3 + i] = a.y[i]; out[6 + i] = a.z[i]; }
+}
+
+// Evaluate one problem term at state y with field h (golden-vector checks).
+// what: 0 det, 1 drift, 2 sto (row-major), 3 Neel reference JacDet (dt), 4 Neel reference
+// JacSto (h = dW), 5 NeelSpherical prepare+finish round trip, 6 NeelSpherical isRotated
+// flags: bit0 BN simple model, bit1 coordinate transformation
+int or_eval(int problem, int aniso, const double* props, const double* axes, int flags, double minAngle,
+            const double* y, const double* h, double dt, int what, double* out) {
+    const MagneticProperties mp = magFrom(props, aniso);
+    const double T = props[4];
+    const Axes ax = axesFrom(axes);
+    const Vec<double, 3> hv = v3(h ? h[0] : 0.0, h ? h[1] : 0.0, h ? h[2] : 0.0);
+    auto neel = [&](auto an) -> int {
+        using A = decltype(an);
+        NeelProblem<A> P(mp, T, ax);
+        const Vec<double, 3> yv = v3(y[0], y[1], y[2]);
+        if (what == 0) { auto r = P.det(yv, hv); for (int i = 0; i < 3; ++i) out[i] = r[i]; return 0; }
+        if (what == 1) { auto r = P.drift(yv); for (int i = 0; i < 3; ++i) out[i] = r[i]; return 0; }
+        if (what == 2) { auto M = P.sto(yv); for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[i * 3 + j] = M(i, j); return 0; }
+        if constexpr (std::is_same<A, Uniaxial>::value) {
+            if (what == 3) { auto M = P.jacDetReference(yv, hv, dt); for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[i * 3 + j] = M(i, j); return 0; }
+            if (what == 4) { auto M = P.jacStoReference(hv); for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[i * 3 + j] = M(i, j); return 0; }
+        }
+        return 1;
+    };
+    auto sph = [&](auto an) -> int {
+        using A = decltype(an);
+        NeelSphericalProblem<A> P(mp, T, ax, (flags & 2) != 0, minAngle);
+        Vec<double, 2> yv; yv[0] = y[0]; yv[1] = y[1];
+        SphericalCache<A, double> c;
+        P.prepare(yv, c);
+        if (what == 0) { auto r = P.det(c, hv); out[0] = r[0]; out[1] = r[1]; return 0; }
+        if (what == 1) { auto r = P.drift(c); out[0] = r[0]; out[1] = r[1]; return 0; }
+        if (what == 2) { auto M = P.sto(c); for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) out[i * 3 + j] = M(i, j); return 0; }
+        if (what == 5) { P.finish(yv, c); out[0] = yv[0]; out[1] = yv[1]; return 0; }
+        if (what == 6) { out[0] = c.isRotated ? 1.0 : 0.0; return 0; }
+        return 1;
+    };
+    auto bn = [&](auto an, auto simple) -> int {
+        using A = decltype(an);
+        BrownAndNeelProblem<A, decltype(simple)::value> P(mp, T, props[6], props[7]);
+        Vec<double, 6> yv; for (int i = 0; i < 6; ++i) yv[i] = y[i];
+        if (what == 0) { auto r = P.det(yv, hv); for (int i = 0; i < 6; ++i) out[i] = r[i]; return 0; }
+        if (what == 1) { auto r = P.drift(yv); for (int i = 0; i < 6; ++i) out[i] = r[i]; return 0; }
+        if (what == 2) { auto M = P.sto(yv); for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) out[i * 6 + j] = M(i, j); return 0; }
+        return 1;
+    };
+    if (problem == PROB_NEEL) return aniso == 1 ? neel(Uniaxial(mp)) : neel(Cubic(mp));
+    if (problem == PROB_SPHERICAL) return aniso == 1 ? sph(Uniaxial(mp)) : sph(Cubic(mp));
+    if (problem == PROB_BN) {
+        const bool simple = flags & 1;
+        if (aniso == 1) return simple ? bn(Uniaxial(mp), std::true_type{}) : bn(Uniaxial(mp), std::false_type{});
+        return simple ? bn(Cubic(mp), std::true_type{}) : bn(Cubic(mp), std::false_type{});
+    }
+    return 2;
+}
+
+// Newton on the reference's Implicit_Solver_Test functions (Test_Function_1..3.h)
+int or_newton_testfn(int fn, const double* x0, double tol, int maxit, double* xout) {
+    if (fn == 1 || fn == 2) {
+        double y[2] = {x0[0], x0[1]};
+        auto fdf = [&](const double* x, double* F, double (*J)[2]) {
+            if (fn == 1) {
+                F[0] = x[0] + 2.0 * x[1] - 9.0; F[1] = 3.0 * x[0] + x[1] + 6 * x[0] * x[1] - 64;
+                J[0][0] = 1; J[0][1] = 2; J[1][0] = 3 + 6 * x[1]; J[1][1] = 1 + 6 * x[0];
+            } else {
+                F[0] = 16.0 * x[0] + 2.0 * std::pow(x[1], 3) - 9.0; F[1] = 3 * x[0] * x[0] + std::pow(x[1], 4) + 6 * x[0] * x[1] - 64;
+                J[0][0] = 16.0; J[0][1] = 6.0 * x[1] * x[1]; J[1][0] = 6.0 * x[0] + 6.0 * x[1]; J[1][1] = 6.0 * x[0] + 4.0 * std::pow(x[1], 3);
+            }
+        };
+        const int it = newton<2>(fdf, y, tol, maxit);
+        xout[0] = y[0]; xout[1] = y[1];
+        return it;
+    }
+    double y[3] = {x0[0], x0[1], x0[2]};
+    auto fdf = [&](const double* x, double* F, double (*J)[3]) {
+        F[0] = x[0] * x[0] - 3.0 * x[2] * x[2] - 9; F[1] = 16.0 * x[1] + x[1] * x[1] * x[2] + 64; F[2] = x[0] * x[1] * x[1] - x[2] * x[2] + 16;
+        J[0][0] = 2.0 * x[0]; J[0][1] = 0.0; J[0][2] = -6.0 * x[2];
+        J[1][0] = 0.0; J[1][1] = 16.0 + 2.0 * x[1] * x[2]; J[1][2] = x[1] * x[1];
+        J[2][0] = x[1] * x[1]; J[2][1] = 2.0 * x[0] * x[1]; J[2][2] = -2.0 * x[2];
+    };
+    const int it = newton<3>(fdf, y, tol, maxit);
+    xout[0] = y[0]; xout[1] = y[1]; xout[2] = y[2];
+    return it;
+}
+
+// Full ensemble run.  ispec = [problem, solver, aniso, flags, newton_max_iter, nthreads];
+// uspec = [n_particles, n_steps, oversampling, seed, replay_count];
+// dspec = [dt, min_angle, newton_tol, field_type, offset(3), amp(3), freq(3), phase(3)]
+// props [N][8], axes [N][9] (or NULL), y0 [N][dim_y].
+// Outputs: mean [n_out*dout], weight [dout], times [n_out], fields [n_out*3],
+// final_state [N*dim_y] (or NULL), per_particle [N*n_out*dout] weighted outputs (or NULL).
+// Returns wall seconds in stats[0], newton failures in stats[1].
+int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec,
+                const double* props, const double* axes, const double* y0, const double* replay,
+                double* mean, double* weight, double* times, double* fields, double* final_state,
+                double* per_particle, double* stats) {
+    RunSpec rs{};
+    rs.problem = (int)ispec[0]; rs.solver = (int)ispec[1]; rs.aniso = (int)ispec[2];
+    rs.bnSimple = (ispec[3] & 1) != 0; rs.coordTransform = (ispec[3] & 2) != 0;
+    rs.newtonMaxIter = (int)ispec[4];
+    const int nThreads = (int)ispec[5];
+    rs.nParticles = uspec[0]; rs.nSteps = uspec[1]; rs.oversampling = uspec[2]; rs.seed = uspec[3];
+    rs.replayCount = uspec[4]; rs.replay = replay;
+    rs.dt = dspec[0]; rs.minAngle = dspec[1]; rs.newtonTol = dspec[2];
+    rs.field.type = (int)dspec[3];
+    for (int i = 0; i < 3; ++i) { rs.field.offset[i] = dspec[4 + i]; rs.field.amp[i] = dspec[7 + i]; rs.field.freq[i] = dspec[10 + i]; rs.field.phase[i] = dspec[13 + i]; }
+    const uint64_t N = rs.nParticles;
+    const int dy = rs.problem == PROB_NEEL ? 3 : (rs.problem == PROB_SPHERICAL ? 2 : 6);
+    std::vector<ParticleIn> parts(N);
+    for (uint64_t p = 0; p < N; ++p) {
+        parts[p].mp = magFrom(props + p * 8, rs.aniso);
+        parts[p].T = props[p * 8 + 4]; parts[p].r_h = props[p * 8 + 6]; parts[p].visc = props[p * 8 + 7];
+        parts[p].ax = axesFrom(axes ? axes + p * 9 : nullptr);
+        for (int i = 0; i < dy; ++i) parts[p].y0[i] = y0[p * dy + i];
+    }
+    RunResult rr;
+    std::vector<double> pp;
+    const auto t0 = std::chrono::steady_clock::now();
+    const bool keep = final_state != nullptr;
+    std::vector<double>* ppo = per_particle ? &pp : nullptr;
+    if (rs.problem == PROB_NEEL) {
+        if (rs.aniso == 1) runEnsemble<NeelProblem<Uniaxial>, CartesianStepper<NeelProblem<Uniaxial>>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelProblem<Uniaxial>(q.mp, q.T, q.ax); }, rr, nThreads, keep, ppo);
+        else runEnsemble<NeelProblem<Cubic>, CartesianStepper<NeelProblem<Cubic>>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelProblem<Cubic>(q.mp, q.T, q.ax); }, rr, nThreads, keep, ppo);
+    } else if (rs.problem == PROB_SPHERICAL) {
+        if (rs.aniso == 1) runEnsemble<NeelSphericalProblem<Uniaxial>, SphericalStepper<Uniaxial>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelSphericalProblem<Uniaxial>(q.mp, q.T, q.ax, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo);
+        else runEnsemble<NeelSphericalProblem<Cubic>, SphericalStepper<Cubic>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelSphericalProblem<Cubic>(q.mp, q.T, q.ax, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo);
+    } else if (rs.problem == PROB_BN) {
+#define BN_RUN(A, S) runEnsemble<BrownAndNeelProblem<A, S>, CartesianStepper<BrownAndNeelProblem<A, S>>>(rs, parts.data(), [&](const ParticleIn& q) { return BrownAndNeelProblem<A, S>(q.mp, q.T, q.r_h, q.visc); }, rr, nThreads, keep, ppo)
+        if (rs.aniso == 1) { if (rs.bnSimple) BN_RUN(Uniaxial, true); else BN_RUN(Uniaxial, false); }
+        else { if (rs.bnSimple) BN_RUN(Cubic, true); else BN_RUN(Cubic, false); }
+#undef BN_RUN
+    } else return 2;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::copy(rr.mean.begin(), rr.mean.end(), mean);
+    std::copy(rr.weight.begin(), rr.weight.end(), weight);
+    if (times) std::copy(rr.times.begin(), rr.times.end(), times);
+    if (fields) std::copy(rr.fields.begin(), rr.fields.end(), fields);
+    if (keep) std::copy(rr.finalState.begin(), rr.finalState.end(), final_state);
+    if (per_particle) std::copy(pp.begin(), pp.end(), per_particle);
+    stats[0] = secs; stats[1] = (double)rr.newtonFail;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,542 @@
+// sp_oracle.hpp -- CPU oracle for the StochasticPhysics particle-simulation hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a scalar, Eigen-free restatement of the
+// reference's algorithm, used by tests/, by __graft_entry__.smoke() and by the
+// cpu_baseline leg of bench.py as the checker.  The product (the HIP library in
+// stochasticphysics_amd/) never includes, links or calls it.
+//
+// Every function follows the reference file:line named beside it, in the
+// reference's order of operations where that is visible (Eigen expression
+// order is approximated by left-to-right evaluation).  Parity pins, one per
+// function, live in tests/test_oracle_*.py:
+//   - golden vectors of the reference's own gtests (tests/golden/reference_vectors.json):
+//     Neel det/S/drift/JacDet/JacSto, NeelSpherical det/S/drift/finish,
+//     BrownAndNeel det/S(full, simple)/drift(full, simple), uniaxial Jacobian,
+//     Neel parameter closed forms, Newton roots of Test_Function_1..3;
+//   - closed forms / invariants where the reference has no vector: cubic field
+//     = -(1/Ms) grad of the cubic energy (finite differences), Box-Muller log
+//     and sin/cos against libm, Philox against the published KAT vectors,
+//     Dual-number Jacobians against finite differences, solver one-step
+//     formulas against a direct evaluation, equilibrium statistics.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#include <type_traits>
+
+namespace oracle {
+
+// ------------------------------------------------------------------ small algebra
+template <class S, int N> struct Vec {
+    S v[N];
+    S& operator[](int i) { return v[i]; }
+    const S& operator[](int i) const { return v[i]; }
+    static Vec zero() { Vec r; for (int i = 0; i < N; ++i) r.v[i] = S(0.0); return r; }
+};
+template <class S, int R, int C> struct Mat {
+    S m[R][C];
+    S& operator()(int i, int j) { return m[i][j]; }
+    const S& operator()(int i, int j) const { return m[i][j]; }
+    static Mat zero() { Mat r; for (int i = 0; i < R; ++i) for (int j = 0; j < C; ++j) r.m[i][j] = S(0.0); return r; }
+    static Mat identity() { Mat r = zero(); for (int i = 0; i < R && i < C; ++i) r.m[i][i] = S(1.0); return r; }
+};
+template <class S, int N> Vec<S, N> operator+(const Vec<S, N>& a, const Vec<S, N>& b) { Vec<S, N> r; for (int i = 0; i < N; ++i) r[i] = a[i] + b[i]; return r; }
+template <class S, int N> Vec<S, N> operator-(const Vec<S, N>& a, const Vec<S, N>& b) { Vec<S, N> r; for (int i = 0; i < N; ++i) r[i] = a[i] - b[i]; return r; }
+template <class S, class T, int N> Vec<S, N> operator*(const T& s, const Vec<S, N>& a) { Vec<S, N> r; for (int i = 0; i < N; ++i) r[i] = s * a[i]; return r; }
+template <class S, int N> Vec<S, N>& operator+=(Vec<S, N>& a, const Vec<S, N>& b) { for (int i = 0; i < N; ++i) a[i] = a[i] + b[i]; return a; }
+template <class S, int N> S dot(const Vec<S, N>& a, const Vec<S, N>& b) { S r = a[0] * b[0]; for (int i = 1; i < N; ++i) r = r + a[i] * b[i]; return r; }
+template <class S> Vec<S, 3> cross(const Vec<S, 3>& a, const Vec<S, 3>& b) {
+    Vec<S, 3> r;
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+    return r;
+}
+template <class S, int R, int C, int K> Mat<S, R, C> matmul(const Mat<S, R, K>& a, const Mat<S, K, C>& b) {
+    Mat<S, R, C> r;
+    for (int i = 0; i < R; ++i) for (int j = 0; j < C; ++j) { S s = a(i, 0) * b(0, j); for (int k = 1; k < K; ++k) s = s + a(i, k) * b(k, j); r(i, j) = s; }
+    return r;
+}
+template <class S, class T, int R, int C> Vec<S, R> matvec(const Mat<S, R, C>& a, const Vec<T, C>& x) {
+    Vec<S, R> r;
+    for (int i = 0; i < R; ++i) { S s = a(i, 0) * x[0]; for (int j = 1; j < C; ++j) s = s + a(i, j) * x[j]; r[i] = s; }
+    return r;
+}
+template <class S> Vec<S, 3> v3(S a, S b, S c) { Vec<S, 3> r; r[0] = a; r[1] = b; r[2] = c; return r; }
+template <class S, class T> Vec<S, 3> lift3(const Vec<T, 3>& a) { return v3<S>(S(a[0]), S(a[1]), S(a[2])); }
+
+// ------------------------------------------------------------------ forward-mode duals
+// Used only to form exact Newton Jacobians for the ImplicitMidpoint oracle.
+template <int N> struct Dual {
+    double v; double d[N];
+    Dual() : v(0.0) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+    Dual(double x) : v(x) { for (int i = 0; i < N; ++i) d[i] = 0.0; }
+    static Dual var(double x, int k) { Dual r(x); r.d[k] = 1.0; return r; }
+};
+template <int N> Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v + b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v - b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> Dual<N> operator-(const Dual<N>& a) { Dual<N> r; r.v = -a.v; for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v * b.v; for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v / b.v; for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] * b.v - a.v * b.d[i]) / (b.v * b.v); return r; }
+template <int N> Dual<N> operator*(double a, const Dual<N>& b) { return Dual<N>(a) * b; }
+template <int N> Dual<N> operator*(const Dual<N>& a, double b) { return a * Dual<N>(b); }
+template <int N> Dual<N> operator+(const Dual<N>& a, double b) { return a + Dual<N>(b); }
+template <int N> Dual<N> operator-(const Dual<N>& a, double b) { return a - Dual<N>(b); }
+template <int N> Dual<N> operator-(double a, const Dual<N>& b) { return Dual<N>(a) - b; }
+template <int N> Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+template <int N> Dual<N> chain(const Dual<N>& a, double f, double df) { Dual<N> r; r.v = f; for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i]; return r; }
+template <int N> Dual<N> sin(const Dual<N>& a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+template <int N> Dual<N> cos(const Dual<N>& a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+template <int N> Dual<N> sqrt(const Dual<N>& a) { const double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
+template <int N> Dual<N> acos(const Dual<N>& a) { return chain(a, std::acos(a.v), -1.0 / std::sqrt(1.0 - a.v * a.v)); }
+template <int N> Dual<N> atan2(const Dual<N>& y, const Dual<N>& x) {
+    Dual<N> r; r.v = std::atan2(y.v, x.v);
+    const double den = x.v * x.v + y.v * y.v;
+    for (int i = 0; i < N; ++i) r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) / den;
+    return r;
+}
+template <int N> Dual<N> fmod(const Dual<N>& a, double m) { Dual<N> r = a; r.v = std::fmod(a.v, m); return r; }
+inline double val(double x) { return x; }
+template <int N> double val(const Dual<N>& x) { return x.v; }
+using std::sin; using std::cos; using std::sqrt; using std::acos; using std::atan2; using std::fmod;
+
+// ------------------------------------------------------------------ constants
+constexpr double kB = 1.3806485279E-23;      // ParameterCalculatorNeel.h:43
+constexpr double PI = 3.14159265358979323846;
+constexpr double TWO_PI = 6.28318530717958647692;
+
+// MyCEL sphere volume, 4/3 pi r^3; pinned by the golden vectors (V enters mu)
+inline double sphere_volume(double r) { return 4.0 / 3.0 * PI * r * r * r; }
+
+struct MagneticProperties {          // Properties/MagneticProperties.h:100-267
+    double radius, Ms, alpha, gamma, K;   // K: K_uniaxial or K_cubic
+    int aniso;                            // 1 uniaxial, 2 cubic
+    double volume() const { return sphere_volume(radius); }           // :159
+    double moment() const { return Ms * volume(); }                   // :166
+};
+
+struct NeelParams { double NeelFactor1, NeelFactor2, DriftPrefactor, NoisePrefactor, Damping; };
+
+// ParameterCalculatorNeel.h:44-92
+inline NeelParams calcNeelParams(const MagneticProperties& mp, double T) {
+    NeelParams p{};
+    const double alpha = mp.alpha;
+    double diffsquare = 2.0 * alpha * kB * T / (mp.gamma * mp.moment());
+    if (std::isinf(diffsquare) || std::isnan(diffsquare)) diffsquare = 0.0;
+    if (std::isinf(alpha)) { p.NeelFactor1 = 0.0; p.NeelFactor2 = 0.0; }
+    else {
+        p.NeelFactor1 = -mp.gamma / (1.0 + std::pow(alpha, 2));
+        p.NeelFactor2 = std::abs(p.NeelFactor1) * alpha;
+    }
+    if (diffsquare == 0.0) p.DriftPrefactor = 0.0;
+    else p.DriftPrefactor = -1.0 * std::pow(p.NeelFactor1, 2) * (1.0 + std::pow(alpha, 2)) * diffsquare;
+    p.NoisePrefactor = std::sqrt(diffsquare);
+    p.Damping = alpha;
+    return p;
+}
+
+struct BrownParams { double BrownPrefactor, BrownDiffusion, Brown_F_Noise; };
+
+// ParameterCalculatorBrown.h:39-58 (hydrodynamic volume 4/3 pi r_h^3, HydrodynamicProperties.h:46)
+inline BrownParams calcBrownRotationParams(double r_h, double viscosity, double T) {
+    BrownParams p{};
+    const double Vh = sphere_volume(r_h);
+    p.BrownPrefactor = 1.0 / (6.0 * viscosity * Vh);
+    p.BrownDiffusion = std::sqrt(2.0 * kB * T * (6.0 * viscosity * Vh));
+    if (std::isinf(p.BrownDiffusion) || std::isnan(p.BrownDiffusion)) p.BrownDiffusion = 0.0;
+    if (std::isnan(p.BrownPrefactor)) p.BrownPrefactor = 0.0;
+    p.Brown_F_Noise = p.BrownPrefactor * p.BrownDiffusion;
+    return p;
+}
+
+// ------------------------------------------------------------------ anisotropies
+struct Axes { Vec<double, 3> x, y, z; };
+
+// UniaxialAnisotropy.h:68-150
+struct Uniaxial {
+    double prefactorField, prefactorTorque;   // -2K/Ms, -2K V
+    explicit Uniaxial(const MagneticProperties& mp)
+        : prefactorField(-2.0 * mp.K / mp.Ms), prefactorTorque(-2.0 * mp.K * mp.volume()) {}
+    template <class S> void prepareField(const Vec<S, 3>&, const Axes&) const {}
+    template <class S> Vec<S, 3> field(const Vec<S, 3>& m, const Axes& ax) const { return field_axis(m, lift3<S>(ax.z)); }
+    template <class S> Vec<S, 3> field_axis(const Vec<S, 3>& m, const Vec<S, 3>& n) const {   // :141-144
+        return (prefactorField * dot(m, n)) * n;
+    }
+    template <class S> Vec<S, 3> torque_axis(const Vec<S, 3>& m, const Vec<S, 3>& n) const {  // :146-150
+        return cross(n, (prefactorTorque * dot(m, n)) * m);
+    }
+    Mat<double, 3, 3> jacobian(const Axes& ax) const {                                       // :130-137
+        Mat<double, 3, 3> J;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) J(i, j) = (prefactorField * ax.z[i]) * ax.z[j];
+        return J;
+    }
+};
+
+// CubicAnisotropy.h:92-236; the field uses the cache prepareField fills (:109-121)
+struct Cubic {
+    double K1_MS, K1VM;
+    explicit Cubic(const MagneticProperties& mp) : K1_MS(-2.0 * mp.K / mp.Ms), K1VM(-2.0 * mp.K * mp.volume()) {}
+    template <class S> Vec<S, 3> field(const Vec<S, 3>& m, const Axes& ax) const {
+        const Vec<S, 3> xa = lift3<S>(ax.x), ya = lift3<S>(ax.y), za = lift3<S>(ax.z);
+        const S c1m = dot(m, xa), c2m = dot(m, ya), c3m = dot(m, za);
+        const S c1m_2 = c1m * c1m, c2m_2 = c2m * c2m, c3m_2 = c3m * c3m;
+        const Vec<S, 3> t = ((c2m_2 + c3m_2) * c1m) * xa + ((c1m_2 + c3m_2) * c2m) * ya + ((c1m_2 + c2m_2) * c3m) * za;
+        return K1_MS * t;                                                                     // :139
+    }
+    // Jacobian of the field (the reference returns Identity here, :228-236 "TODO");
+    // analytic form d/dm of the expression above, used for documentation tests.
+    Mat<double, 3, 3> jacobian(const Vec<double, 3>& m, const Axes& ax) const {
+        const Vec<double, 3> a[3] = {ax.x, ax.y, ax.z};
+        double c[3]; for (int i = 0; i < 3; ++i) c[i] = dot(m, a[i]);
+        Mat<double, 3, 3> J = Mat<double, 3, 3>::zero();
+        for (int i = 0; i < 3; ++i) {
+            const int j = (i + 1) % 3, k = (i + 2) % 3;
+            // term_i = c_i (c_j^2 + c_k^2) a_i ; grad term_i = (c_j^2+c_k^2) a_i a_i^T + 2 c_i c_j a_i a_j^T + 2 c_i c_k a_i a_k^T
+            for (int r = 0; r < 3; ++r) for (int s = 0; s < 3; ++s)
+                J(r, s) = J(r, s) + K1_MS * (a[i][r] * ((c[j] * c[j] + c[k] * c[k]) * a[i][s] + 2.0 * c[i] * c[j] * a[j][s] + 2.0 * c[i] * c[k] * a[k][s]));
+        }
+        return J;
+    }
+};
+
+// Euler angles -> body axes (NeelRelaxation.h:223-261)
+inline Axes axesFromEuler(double phi, double theta, double psi) {
+    const double cphi = std::cos(phi), ctheta = std::cos(theta), cpsi = std::cos(psi);
+    const double sphi = std::sin(phi), stheta = std::sin(theta), spsi = std::sin(psi);
+    const double cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
+    Axes a;
+    a.x = v3(cphicpsi - ctheta * sphispsi, -sphicpsi - ctheta * cphispsi, stheta * spsi);
+    a.y = v3(ctheta * sphicpsi + cphispsi, ctheta * cphicpsi - sphispsi, -stheta * cpsi);
+    a.z = v3(stheta * sphi, stheta * cphi, ctheta);
+    return a;
+}
+
+template <class Aniso, class S> Vec<S, 3> aniso_field(const Aniso& an, const Vec<S, 3>& m, const Axes& ax) { return an.field(m, ax); }
+
+// ------------------------------------------------------------------ Neel problem
+// Problems/NeelRelaxation.h:27-309.  Stratonovich (NeelRelaxation_Definitions.h:105).
+template <class Aniso> struct NeelProblem {
+    static constexpr int DY = 3, DW = 3, DOUT = 3;
+    NeelParams mParams; Aniso mAnisotropy; Axes ax; double mu;
+    NeelProblem(const MagneticProperties& mp, double T, const Axes& axes)
+        : mParams(calcNeelParams(mp, T)), mAnisotropy(mp), ax(axes), mu(mp.moment()) {}
+
+    template <class S> void prepare(Vec<S, 3>&) {}                         // :195-196
+    template <class S> void finish(Vec<S, 3>&) const {}                     // :201-202
+    template <class S> Mat<S, 3, 3> sto(const Vec<S, 3>& yi) const {        // :91-109
+        const double pre2 = mParams.NeelFactor2 * mParams.NoisePrefactor;
+        const double pre1 = mParams.NeelFactor1 * mParams.NoisePrefactor;
+        Mat<S, 3, 3> M;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) M(i, j) = S(i == j ? pre2 : 0.0) - (pre2 * yi[i]) * yi[j];
+        const Vec<S, 3> yi2 = pre1 * yi;
+        M(0, 1) = M(0, 1) - yi2[2]; M(0, 2) = M(0, 2) + yi2[1];
+        M(1, 0) = M(1, 0) + yi2[2]; M(1, 2) = M(1, 2) - yi2[0];
+        M(2, 0) = M(2, 0) - yi2[1]; M(2, 1) = M(2, 1) + yi2[0];
+        return M;
+    }
+    template <class S> Vec<S, 3> drift(const Vec<S, 3>& yi) const { return mParams.DriftPrefactor * yi; }   // :111-115
+    template <class S> Vec<S, 3> det(const Vec<S, 3>& yi, const Vec<double, 3>& xi) const {                 // :117-128
+        const Vec<S, 3> Heff = aniso_field(mAnisotropy, yi, ax) + lift3<S>(xi);
+        return mParams.NeelFactor1 * cross(yi, Heff) - mParams.NeelFactor2 * cross(yi, cross(yi, Heff));
+    }
+    // reference Jacobians (:134-190), kept for the golden-vector pin only
+    Mat<double, 3, 3> jacDetReference(const Vec<double, 3>& m, const Vec<double, 3>& xi, double dt) const {
+        const Mat<double, 3, 3> HJ = mAnisotropy.jacobian(ax);
+        Mat<double, 3, 3> mp = Mat<double, 3, 3>::zero();
+        mp(0, 1) = -m[2]; mp(0, 2) = m[1]; mp(1, 0) = m[2]; mp(1, 2) = -m[0]; mp(2, 0) = -m[1]; mp(2, 1) = m[0];
+        const Vec<double, 3> H = aniso_field(mAnisotropy, m, ax) + xi;
+        Mat<double, 3, 3> J = matmul(mp, HJ);
+        J(0, 1) += H[2]; J(0, 2) -= H[1]; J(1, 0) -= H[2]; J(1, 2) += H[0]; J(2, 0) += H[1]; J(2, 1) -= H[0];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) J(i, j) = mParams.NeelFactor1 * J(i, j) - 2.0 * mParams.Damping / dt * mp(i, j);
+        return J;
+    }
+    Mat<double, 3, 3> jacStoReference(const Vec<double, 3>& dW) const {
+        Mat<double, 3, 3> J = Mat<double, 3, 3>::zero();
+        const double pre1 = mParams.NeelFactor1 * mParams.NoisePrefactor;
+        const Vec<double, 3> dw2 = (-pre1) * dW;
+        J(0, 1) -= dw2[2]; J(0, 2) += dw2[1]; J(1, 0) += dw2[2]; J(1, 2) -= dw2[0]; J(2, 0) -= dw2[1]; J(2, 1) += dw2[0];
+        return J;
+    }
+    Vec<double, 3> output(const Vec<double, 3>& y) const { return y; }          // :210-214
+    Vec<double, 3> weight() const { return v3(mu, mu, mu); }                   // :287-291
+};
+
+// ------------------------------------------------------------------ NeelSpherical problem
+// Problems/NeelRelaxationSpherical.h:32-647; state (theta, phi), noise R^3.
+template <class Aniso, class S> struct SphericalCache {
+    bool isRotated = false;
+    S one_div_sin_t, Mx, My, Mz;
+    Vec<S, 3> MagDir, e_theta, e_phi;
+    Mat<S, 2, 3> P;
+};
+
+template <class Aniso> struct NeelSphericalProblem {
+    static constexpr int DY = 2, DW = 3, DOUT = 3;
+    NeelParams mParams; Aniso mAnisotropy; Axes ax; double mu;
+    bool rotate; double minAngle, maxAngle;
+    NeelSphericalProblem(const MagneticProperties& mp, double T, const Axes& axes, bool useRot, double minA)
+        : mParams(calcNeelParams(mp, T)), mAnisotropy(mp), ax(axes), mu(mp.moment()),
+          rotate(useRot), minAngle(minA), maxAngle(PI - minA) {}   // :109-116
+
+    template <class S> bool needsRotation(const Vec<S, 2>& y) const {            // :546-559
+        if (rotate) { const double th = val(y[0]); if (th < minAngle || th > maxAngle) return true; }
+        return false;
+    }
+    template <class S> static Vec<S, 2> rotateFwd(const Vec<S, 2>& y) {             // :568-580
+        const S sin_t = sin(y[0]);
+        Vec<S, 2> r; r[0] = acos(-cos(y[1]) * sin_t); r[1] = atan2(sin(y[1]) * sin_t, cos(y[0])); return r;
+    }
+    template <class S> static Vec<S, 2> rotateBack(const Vec<S, 2>& y) {            // :589-602
+        const S sin_t = sin(y[0]);
+        Vec<S, 2> r; r[0] = acos(cos(y[1]) * sin_t); r[1] = atan2(sin(y[1]) * sin_t, -cos(y[0])); return r;
+    }
+    // prepareCalculations (:124-237): may rotate y in place and fills the chart cache
+    template <class S> void prepare(Vec<S, 2>& y, SphericalCache<Aniso, S>& c) const {
+        if (needsRotation(y)) { y = rotateFwd(y); c.isRotated = true; } else c.isRotated = false;
+        const S cos_t = cos(y[0]), cos_p = cos(y[1]), sin_t = sin(y[0]), sin_p = sin(y[1]);
+        S ods = S(1.0) / sin_t;
+        if (std::isinf(val(ods))) ods = S(0.0);
+        c.one_div_sin_t = ods;
+        const double a = mParams.NeelFactor1, b = mParams.NeelFactor2;
+        if (!c.isRotated) {
+            c.MagDir = v3(sin_t * cos_p, sin_t * sin_p, cos_t);
+            c.e_theta = v3(cos_t * cos_p, cos_t * sin_p, -sin_t);
+            c.e_phi = v3(-sin_p, cos_p, S(0.0));
+        } else {
+            c.MagDir = v3(-cos_t, sin_t * sin_p, sin_t * cos_p);
+            c.e_theta = v3(sin_t, cos_t * sin_p, cos_t * cos_p);
+            c.e_phi = v3(S(0.0), cos_p, -sin_p);
+        }
+        for (int j = 0; j < 3; ++j) {
+            c.P(0, j) = -a * c.e_phi[j] + b * c.e_theta[j];
+            c.P(1, j) = ods * (a * c.e_theta[j] + b * c.e_phi[j]);
+        }
+        if (val(ods) == 0.0) { if (!c.isRotated) c.P(1, 2) = S(a); else c.P(1, 0) = S(-a); }
+    }
+    template <class S> Vec<S, 2> det(const SphericalCache<Aniso, S>& c, const Vec<double, 3>& xi) const {   // :239-265
+        const Vec<S, 3> Heff = aniso_field(mAnisotropy, c.MagDir, ax) + lift3<S>(xi);
+        Vec<S, 2> r;
+        for (int i = 0; i < 2; ++i) r[i] = c.P(i, 0) * Heff[0] + c.P(i, 1) * Heff[1] + c.P(i, 2) * Heff[2];
+        return r;
+    }
+    template <class S> Mat<S, 2, 3> sto(const SphericalCache<Aniso, S>& c) const {                            // :267-293
+        Mat<S, 2, 3> r;
+        for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) r(i, j) = mParams.NoisePrefactor * c.P(i, j);
+        return r;
+    }
+    template <class S> Vec<S, 2> drift(const SphericalCache<Aniso, S>& c) const {                             // :295-309
+        const S cos_t = c.isRotated ? -c.MagDir[0] : c.MagDir[2];
+        Vec<S, 2> r; r[0] = -0.5 * mParams.DriftPrefactor * cos_t * c.one_div_sin_t; r[1] = S(0.0);
+        return r;
+    }
+    template <class S> void finish(Vec<S, 2>& y, const SphericalCache<Aniso, S>& c) const {                  // :440-471
+        if (c.isRotated) y = rotateBack(y);
+        else { y[0] = fmod(y[0], TWO_PI); y[1] = fmod(y[1], TWO_PI); }
+    }
+    Vec<double, 3> output(const Vec<double, 2>& y) const {                                                   // :499-516
+        const double st = std::sin(y[0]), ct = std::cos(y[0]), sp = std::sin(y[1]), cp = std::cos(y[1]);
+        return v3(st * cp, st * sp, ct);
+    }
+    Vec<double, 3> weight() const { return v3(mu, mu, mu); }                                                 // :527-531
+};
+
+// ------------------------------------------------------------------ BrownAndNeel problem
+// Problems/BrownAndNeelRelaxation.h:71-193, .inl:16-324; y = (n, m), noise R^6.
+template <class Aniso, bool Simple> struct BrownAndNeelProblem {
+    static constexpr int DY = 6, DW = 6, DOUT = 6;
+    NeelParams mNeel; BrownParams mBrown; Aniso mAnisotropy; double MagneticMoment; bool isCubic;
+    BrownAndNeelProblem(const MagneticProperties& mp, double T, double r_h, double visc)
+        : mNeel(calcNeelParams(mp, T)), mBrown(calcBrownRotationParams(r_h, visc, T)), mAnisotropy(mp),
+          MagneticMoment(mp.moment()), isCubic(mp.aniso == 2) {}
+
+    template <class S> void prepare(Vec<S, 6>&) {}
+    template <class S> void finish(Vec<S, 6>&) const {}
+    template <class S> static Vec<S, 3> head(const Vec<S, 6>& y) { return v3(y[0], y[1], y[2]); }
+    template <class S> static Vec<S, 3> tail(const Vec<S, 6>& y) { return v3(y[3], y[4], y[5]); }
+    template <class S> static Vec<S, 6> join(const Vec<S, 3>& a, const Vec<S, 3>& b) { Vec<S, 6> r; for (int i = 0; i < 3; ++i) { r[i] = a[i]; r[i + 3] = b[i]; } return r; }
+
+    // getStochasticMatrixFull (.inl:39-111) / Simplified (:114-170)
+    Mat<double, 6, 6> sto(const Vec<double, 6>& yi) const {
+        const Vec<double, 3> ni = head(yi), ei = tail(yi);
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, c = mBrown.BrownPrefactor, d = c * MagneticMoment;
+        const double F = mBrown.Brown_F_Noise;
+        Mat<double, 6, 6> M = Mat<double, 6, 6>::zero();
+        auto skewT = [&](const Vec<double, 3>& v, int r0, int c0) {   // Brown_F / Neel_F pattern
+            M(r0 + 1, c0 + 0) = -v[2]; M(r0 + 2, c0 + 0) = v[1]; M(r0 + 0, c0 + 1) = v[2];
+            M(r0 + 2, c0 + 1) = -v[0]; M(r0 + 0, c0 + 2) = -v[1]; M(r0 + 1, c0 + 2) = v[0];
+        };
+        if (!Simple) {
+            const Vec<double, 3> dni = (d * mNeel.NoisePrefactor) * ni;
+            const double s = dot(dni, ei);
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) M(i, 3 + j) = (i == j ? s : 0.0) - ei[i] * dni[j];
+            skewT(F * ei, 3, 0);
+        }
+        skewT(F * ni, 0, 0);
+        const double pre = (Simple ? b : (b + d)) * mNeel.NoisePrefactor;
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) M(3 + i, 3 + j) = pre * ((i == j ? 1.0 : 0.0) - ei[i] * ei[j]);
+        const Vec<double, 3> aei = (a * mNeel.NoisePrefactor) * ei;
+        M(4, 3) += aei[2]; M(5, 3) -= aei[1]; M(3, 4) -= aei[2]; M(5, 4) += aei[0]; M(3, 5) += aei[1]; M(4, 5) -= aei[0];
+        return M;
+    }
+    // getStratonovichtoItoFull (.inl:198-219) / Simplified (:179-196)
+    Vec<double, 6> drift(const Vec<double, 6>& yi) const {
+        const Vec<double, 3> ni = head(yi), ei = tail(yi);
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, c = mBrown.BrownPrefactor, d = c * MagneticMoment;
+        const double cF_2 = mBrown.Brown_F_Noise * mBrown.Brown_F_Noise;
+        const double PreH_2 = mNeel.NoisePrefactor * mNeel.NoisePrefactor;
+        if (Simple) return join((-cF_2) * ni, (-((a * a + b * b) * PreH_2)) * ei);
+        const double bpd = b + d;
+        const Vec<double, 3> nie = dot(ei, ei) * ni;          // (n e^T) e
+        const Vec<double, 3> h = (-cF_2) * ni + PreH_2 * ((d * a) * cross(ni, ei) - (0.5 * d * d) * (nie - ni));
+        return join(h, (-(cF_2 + (a * a + bpd * bpd) * PreH_2)) * ei);
+    }
+    // getDeterministicVector (.inl:222-271); returns zero for cubic (:225-229)
+    template <class S> Vec<S, 6> det(const Vec<S, 6>& yi, const Vec<double, 3>& xi) const {
+        if constexpr (!std::is_same<Aniso, Uniaxial>::value) { return Vec<S, 6>::zero(); } else {
+        const Vec<S, 3> ni = head(yi), ei = tail(yi);
+        const Vec<S, 3> Heff = mAnisotropy.template field_axis<S>(ei, ni) + lift3<S>(xi);
+        const Vec<S, 3> Teff = mAnisotropy.template torque_axis<S>(ei, ni);
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, c = mBrown.BrownPrefactor, d = c * MagneticMoment;
+        const Vec<S, 3> mxHeff = cross(ei, Heff);
+        const Vec<S, 3> omegabrown = c * Teff + d * mxHeff;
+        const Vec<S, 3> omeganeel = (-a) * Heff + (b + d) * mxHeff + c * Teff;
+        return join(cross(omegabrown, ni), cross(omeganeel, ei));
+        }
+    }
+    Vec<double, 6> output(const Vec<double, 6>& y) const { return y; }
+    Vec<double, 6> weight() const { Vec<double, 6> w; for (int i = 0; i < 3; ++i) { w[i] = 1.0; w[i + 3] = MagneticMoment; } return w; }  // .h:152-159
+};
+
+// ------------------------------------------------------------------ fields
+// Fields/ZeroField.h:39, SinusoidalField.h:38-51, LissajousField.h:50-101
+struct FieldSpec {
+    int type = 1;                              // 1 zero, 3 sinusoidal, 4 lissajous
+    double offset[3] = {0, 0, 0}, amp[3] = {0, 0, 0}, freq[3] = {0, 0, 0}, phase[3] = {0, 0, 0};
+    Vec<double, 3> at(double t) const {
+        if (type == 3) {
+            const double w = TWO_PI * freq[0];
+            const double s = std::sin(w * t + phase[0]);
+            return v3(amp[0] * s + offset[0], amp[1] * s + offset[1], amp[2] * s + offset[2]);
+        }
+        if (type == 4) {
+            Vec<double, 3> r;
+            for (int i = 0; i < 3; ++i) r[i] = offset[i] + amp[i] * std::sin(TWO_PI * freq[i] * t + phase[i]);
+            return r;
+        }
+        return Vec<double, 3>::zero();
+    }
+};
+
+// ------------------------------------------------------------------ noise
+// Philox4x32-10 (Salmon et al. SC'11) and the engine's Box-Muller stream; an
+// independent copy of the generator the product uses (same definition).
+inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+        c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+}
+inline double u01(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
+// log(u) by 2 atanh((m-1)/(m+1)) with coefficients 1/(2k+1); same operation sequence as the product
+inline double bm_log(double u) {
+    int e; double m = std::frexp(u, &e);
+    if (m < 0.7071067811865476) { m = m * 2.0; e -= 1; }
+    const double f = m - 1.0, s = f / (2.0 + f), t = s * s;
+    static const double A[11] = {0.043478260869565216, 0.047619047619047616, 0.05263157894736842, 0.058823529411764705,
+                                 0.06666666666666667, 0.07692307692307693, 0.09090909090909091, 0.1111111111111111,
+                                 0.14285714285714285, 0.2, 0.3333333333333333};
+    double P = A[0];
+    for (int i = 1; i < 11; ++i) P = std::fma(P, t, A[i]);
+    P = P * t;
+    const double lm = std::fma(-s, std::fma(-2.0, P, f), f);
+    const double de = (double)e;
+    return std::fma(de, 0.6931471805599453, std::fma(de, 2.3190468138462996e-17, lm));
+}
+inline void bm_sincos2pi(double u, double* s, double* c) {
+    const double x = 4.0 * u, q = std::rint(x), r = x - q, r2 = r * r;
+    static const double SC[10] = {-4.377065467313742e-14, 6.0669357311061955e-12, -6.688035109811468e-10, 5.692172921967927e-08,
+                                  -3.598843235212085e-06, 0.00016044118478735983, -0.004681754135318688, 0.07969262624616705,
+                                  -0.6459640975062463, 1.5707963267948966};
+    static const double CC[11] = {3.437739179098607e-15, -5.294400200734623e-13, 6.565963114979473e-11, -6.386603083791852e-09,
+                                  4.710874778818172e-07, -2.5202042373060607e-05, 0.0009192602748394266, -0.02086348076335296,
+                                  0.25366950790104803, -1.2337005501361697, 1.0};
+    double S = SC[0]; for (int i = 1; i < 10; ++i) S = std::fma(S, r2, SC[i]);
+    S = S * r;
+    double C = CC[0]; for (int i = 1; i < 11; ++i) C = std::fma(C, r2, CC[i]);
+    const int iq = ((int)q) & 3;
+    if (iq == 0) { *s = S; *c = C; } else if (iq == 1) { *s = C; *c = -S; }
+    else if (iq == 2) { *s = -S; *c = -C; } else { *s = -C; *c = S; }
+}
+inline void normals4(uint64_t seed, uint64_t particle, uint64_t call, uint32_t tag, double z[4], bool libm = false) {
+    uint32_t c[4] = {(uint32_t)call, (uint32_t)(call >> 32), (uint32_t)particle, tag};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    if (libm) {   // textbook Box-Muller with libm, for the tolerance check of bm_log / bm_sincos2pi
+        const double r0 = std::sqrt(-2.0 * std::log(u01(c[0]))), r1 = std::sqrt(-2.0 * std::log(u01(c[2])));
+        z[0] = r0 * std::cos(TWO_PI * u01(c[1])); z[1] = r0 * std::sin(TWO_PI * u01(c[1]));
+        z[2] = r1 * std::cos(TWO_PI * u01(c[3])); z[3] = r1 * std::sin(TWO_PI * u01(c[3]));
+        return;
+    }
+    const double r0 = std::sqrt(-2.0 * bm_log(u01(c[0]))), r1 = std::sqrt(-2.0 * bm_log(u01(c[2])));
+    double s0, c0, s1, c1;
+    bm_sincos2pi(u01(c[1]), &s0, &c0); bm_sincos2pi(u01(c[3]), &s1, &c1);
+    z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+}
+
+// per-particle standard-normal source: Philox stream or replayed buffer
+struct NoiseSource {
+    uint64_t seed = 0, particle = 0, n_particles = 1;
+    const double* replay = nullptr; uint64_t replay_count = 0;
+    uint64_t k = 0;                  // next normal index
+    double buf[4]; uint64_t buf_call = ~0ull;
+    double next() {
+        double z;
+        if (replay) z = (k < replay_count) ? replay[k * n_particles + particle] : 0.0;
+        else {
+            const uint64_t call = k >> 2;
+            if (call != buf_call) { normals4(seed, particle, call, 0u, buf); buf_call = call; }
+            z = buf[k & 3];
+        }
+        ++k;
+        return z;
+    }
+};
+
+// ------------------------------------------------------------------ small linear solve
+template <int N> bool solveLinear(double A[N][N], double b[N], double x[N]) {   // Gaussian elimination, partial pivoting
+    for (int k = 0; k < N; ++k) {
+        int p = k; double best = std::abs(A[k][k]);
+        for (int i = k + 1; i < N; ++i) if (std::abs(A[i][k]) > best) { best = std::abs(A[i][k]); p = i; }
+        if (best == 0.0) return false;
+        if (p != k) { for (int j = 0; j < N; ++j) std::swap(A[k][j], A[p][j]); std::swap(b[k], b[p]); }
+        for (int i = k + 1; i < N; ++i) {
+            const double f = A[i][k] / A[k][k];
+            for (int j = k; j < N; ++j) A[i][j] -= f * A[k][j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int i = N - 1; i >= 0; --i) { double s = b[i]; for (int j = i + 1; j < N; ++j) s -= A[i][j] * x[j]; x[i] = s / A[i][i]; }
+    return true;
+}
+
+// Newton iteration with the stopping rule of Implicit_Midpoint.inl:107-180
+// (dx = J^-1 (-F); y += dx; stop when |dx| <= tol) and at most maxIter steps.
+// fdf(y, F, J) evaluates residual and Jacobian.  Returns iterations used, or -1.
+template <int N, class FDF> int newton(FDF&& fdf, double y[N], double tol, int maxIter) {
+    for (int it = 1; it <= maxIter; ++it) {
+        double F[N], J[N][N], dx[N];
+        fdf(y, F, J);
+        for (int i = 0; i < N; ++i) F[i] = -F[i];
+        if (!solveLinear<N>(J, F, dx)) return -1;
+        double nrm = 0.0;
+        for (int i = 0; i < N; ++i) { y[i] += dx[i]; nrm += dx[i] * dx[i]; }
+        if (std::sqrt(nrm) <= tol) return it;
+    }
+    return -1;
+}
+
+}  // namespace oracle
