@@ -185,10 +185,22 @@ int sp_ctx_get_tuning(sp_ctx* ctx, int problem, int solver, int precision, sp_tu
 /* device memory budget for admitted requests (state + params + outputs) */
 int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
 
+/* statistics of the last sp_run on this context: [0] kernel launches, [1] kernel ms,
+ * [2] reduction + copy ms, [3] wall ms, [4] particle-steps, [5] VGPRs of the kernel,
+ * [6] grid size (work-groups), [7] work-group size */
+int sp_ctx_last_run_stats(sp_ctx* ctx, double* out /* [8] */);
+
 /* ---- plain path: one request, synchronous.  Replaces SimulationManager::
  *      StartSimulationManager + waitUntilFinsihed for one ensemble
  *      (SimulationManager.h:549-584) ---- */
 int sp_run(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_result* res);
+
+/* ---- plain path with the inputs resident in HBM: sp_prepare uploads parameters and
+ *      initial state once; every sp_execute restarts from that state on the device ---- */
+typedef struct sp_prepared sp_prepared;
+int sp_prepare(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_prepared** out);
+int sp_execute(sp_prepared* h, sp_result* res);
+void sp_release(sp_prepared* h);
 
 /* ---- batched path: many uneven requests in one persistent launch per chunk ---- */
 int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs,

@@ -1,0 +1,787 @@
+// sp_device.h -- gfx950 device code of the ensemble SDE engine.
+//
+// One particle per lane.  A wave owns a 64-particle tile of one request and
+// keeps the particle state, its parameters and the window accumulators in
+// VGPRs for a whole step range; noise is Philox4x32-10 + Box-Muller in
+// registers; the field is evaluated once per step per wave (each lane computes
+// one of the next 64 steps, v_readlane broadcasts it); at the end of every
+// output window the moment-weighted window means are reduced across the wave
+// and lane 0 writes DOUT doubles.  Nothing else touches HBM inside the loop.
+//
+// Physics follows the reference problem classes; each function names the
+// file:line it restates.  R is the arithmetic type of drift, diffusion and
+// solver (double, or float in the mixed-precision mode); state accumulation,
+// ensemble accumulation, chart rotation and Newton stay in double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sp_rng.h"
+#include "sp_types.h"
+
+namespace sp {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;
+constexpr double INV_TWO_PI = 0.15915494309189533576888376337251;
+
+template <class R> struct V3 { R x, y, z; };
+template <class R> __device__ __forceinline__ V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> __device__ __forceinline__ V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> __device__ __forceinline__ V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> __device__ __forceinline__ R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> __device__ __forceinline__ V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------- field
+// Fields/ZeroField.h:39, SinusoidalField.h:47-51 (A sin(w t + phi) + offset),
+// LissajousField.h:98-101 (offset + A o sin(w o t + phi)).
+__device__ __forceinline__ V3<double> field_at(const DevRequest& rq, double t) {
+    if (rq.field_type == 3) {
+        const double s = sin(rq.f_w[0] * t + rq.f_ph[0]);
+        return {rq.f_amp[0] * s + rq.f_off[0], rq.f_amp[1] * s + rq.f_off[1], rq.f_amp[2] * s + rq.f_off[2]};
+    }
+    if (rq.field_type == 4) {
+        return {rq.f_off[0] + rq.f_amp[0] * sin(rq.f_w[0] * t + rq.f_ph[0]),
+                rq.f_off[1] + rq.f_amp[1] * sin(rq.f_w[1] * t + rq.f_ph[1]),
+                rq.f_off[2] + rq.f_amp[2] * sin(rq.f_w[2] * t + rq.f_ph[2])};
+    }
+    return {0.0, 0.0, 0.0};
+}
+
+// Field values for 64 consecutive steps, one step per lane; step n (0-based) is
+// evaluated at t = dt (n+1) (+ dt/2 for ImplicitMidpoint), the reference's
+// time = _timestep*(++counter) (SingleParticleSimulator.h:145) and
+// Implicit_Midpoint.inl:59.
+struct FieldCache {
+    uint64_t base;
+    double hx, hy, hz;
+    bool active;
+    __device__ __forceinline__ void init(const DevRequest& rq) { base = ~0ull; hx = hy = hz = 0.0; active = rq.field_type != 1; }
+    template <bool MIDPOINT>
+    __device__ __forceinline__ V3<double> get(const DevRequest& rq, uint64_t n) {
+        if (!active) return {0.0, 0.0, 0.0};
+        if (n - base >= 64 || base == ~0ull) {
+            base = n;
+            const int lane = threadIdx.x & 63;
+            double t = rq.dt * (double)(n + lane + 1);
+            if (MIDPOINT) t = t + 0.5 * rq.dt;
+            const V3<double> h = field_at(rq, t);
+            hx = h.x; hy = h.y; hz = h.z;
+        }
+        const int k = (int)(n - base);
+        return {readlane_d(hx, k), readlane_d(hy, k), readlane_d(hz, k)};
+    }
+};
+
+// ---------------------------------------------------------------- noise
+// 12 standard normals = 3 Philox blocks, starting at block `call`.
+template <class R> struct NormalsT;
+template <> struct NormalsT<double> {
+    __device__ __forceinline__ static void philox(uint64_t seed, uint64_t p, uint64_t call, double z[12]) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) box_muller4_f64(noise_block(seed, p, call + b, 0u), z + 4 * b);
+    }
+};
+template <> struct NormalsT<float> {
+    __device__ __forceinline__ static void philox(uint64_t seed, uint64_t p, uint64_t call, float z[12]) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) box_muller4_f32(noise_block(seed, p, call + b, 0u), z + 4 * b);
+    }
+};
+template <class R>
+__device__ __forceinline__ void normals12(const DevRequest& rq, uint64_t p, uint64_t k0, R z[12]) {
+    if (rq.flags & RQ_REPLAY) {
+        const uint64_t N = rq.n_particles;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+            const uint64_t k = k0 + j;
+            z[j] = (R)(k < rq.replay_count ? rq.replay[k * N + p] : 0.0);
+        }
+    } else {
+        NormalsT<R>::philox(rq.seed, p, k0 >> 2, z);
+    }
+}
+
+__device__ __forceinline__ double ld(const DevRequest& rq, int k, uint64_t p) { return rq.params[(uint64_t)k * rq.n_particles + p]; }
+
+// ---------------------------------------------------------------- anisotropy
+template <class R> struct AnisoLane { R k1; V3<R> ex, ey, ez; };
+template <int ANISO, class R>
+__device__ __forceinline__ void load_aniso(const DevRequest& rq, uint64_t p, AnisoLane<R>& A) {
+    A.k1 = (R)ld(rq, P_K1, p);
+    A.ez = {(R)ld(rq, P_AX + 6, p), (R)ld(rq, P_AX + 7, p), (R)ld(rq, P_AX + 8, p)};
+    if (ANISO == 2) {
+        A.ex = {(R)ld(rq, P_AX + 0, p), (R)ld(rq, P_AX + 1, p), (R)ld(rq, P_AX + 2, p)};
+        A.ey = {(R)ld(rq, P_AX + 3, p), (R)ld(rq, P_AX + 4, p), (R)ld(rq, P_AX + 5, p)};
+    } else {
+        A.ex = A.ey = {(R)0, (R)0, (R)0};
+    }
+}
+// UniaxialAnisotropy.h:141-144: H = (-2K/Ms)(m.z) z
+// CubicAnisotropy.h:109-146:   H = K1_MS sum_i c_i (c_j^2 + c_k^2) axis_i, c_i = m.axis_i
+template <int ANISO, class R>
+__device__ __forceinline__ V3<R> aniso_field(const AnisoLane<R>& A, V3<R> m) {
+    if (ANISO == 1) return (A.k1 * dot(m, A.ez)) * A.ez;
+    const R c1 = dot(m, A.ex), c2 = dot(m, A.ey), c3 = dot(m, A.ez);
+    const R c1s = c1 * c1, c2s = c2 * c2, c3s = c3 * c3;
+    return A.k1 * (((c2s + c3s) * c1) * A.ex + ((c1s + c3s) * c2) * A.ey + ((c1s + c2s) * c3) * A.ez);
+}
+
+enum { S_EM = 1, S_IM = 3, S_MIL = 6, S_HEUN = 7 };
+
+// ================================================================ Neel (Cartesian)
+// Problems/NeelRelaxation.h: det = a m x H - b m x (m x H) (:117-128);
+// S dW = b sigma (dW - m (m.dW)) + a sigma (m x dW) (:91-109); drift D m (:111-115).
+template <int ANISO, class R> struct NeelLane {
+    R a, b, d, as, bs;   // a dt, b dt, D dt, a sigma sqrt(dt), b sigma sqrt(dt)
+    AnisoLane<R> A;
+    double w;
+};
+template <int ANISO, class R>
+__device__ __forceinline__ void neel_load(const DevRequest& rq, uint64_t p, NeelLane<ANISO, R>& L) {
+    const double a = ld(rq, P_A, p), b = ld(rq, P_B, p), s = ld(rq, P_SIG, p), D = ld(rq, P_D, p);
+    L.a = (R)(a * rq.dt); L.b = (R)(b * rq.dt); L.d = (R)(D * rq.dt);
+    L.as = (R)(a * s * rq.sqdt); L.bs = (R)(b * s * rq.sqdt);
+    load_aniso<ANISO, R>(rq, p, L.A);
+    L.w = ld(rq, P_MU, p);
+}
+// (det + drift) dt
+template <int ANISO, class R>
+__device__ __forceinline__ V3<R> neel_f(const NeelLane<ANISO, R>& L, V3<R> m, V3<R> h) {
+    const V3<R> H = aniso_field<ANISO>(L.A, m) + h;
+    const V3<R> mxH = cross(m, H);
+    const V3<R> mxmxH = cross(m, mxH);
+    return (L.a * mxH - L.b * mxmxH) + L.d * m;
+}
+template <int ANISO, class R>
+__device__ __forceinline__ V3<R> neel_det(const NeelLane<ANISO, R>& L, V3<R> m, V3<R> h) {
+    const V3<R> H = aniso_field<ANISO>(L.A, m) + h;
+    const V3<R> mxH = cross(m, H);
+    return L.a * mxH - L.b * cross(m, mxH);
+}
+template <int ANISO, class R>
+__device__ __forceinline__ V3<R> neel_s(const NeelLane<ANISO, R>& L, V3<R> m, V3<R> z) {
+    return L.bs * (z - dot(m, z) * m) + L.as * cross(m, z);
+}
+
+template <int ANISO, int SOLVER, class R> struct NeelCfg {
+    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4;
+    using Lane = NeelLane<ANISO, R>;
+    struct Cache {};
+    __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { neel_load<ANISO, R>(rq, p, L); }
+    __device__ static void init_cache(const Lane&, const double*, Cache&) {}
+    __device__ static void weights(const Lane& L, double w[3]) { w[0] = w[1] = w[2] = L.w; }
+    __device__ static void output(const Lane&, const double* y, const Cache&, double o[3]) { o[0] = y[0]; o[1] = y[1]; o[2] = y[2]; }
+    __device__ __forceinline__ static void step(const DevRequest&, const Lane& L, double* y, Cache&, const R* z, V3<double> hd) {
+        const V3<R> m = {(R)y[0], (R)y[1], (R)y[2]};
+        const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
+        const V3<R> zz = {z[0], z[1], z[2]};
+        V3<R> inc;
+        if (SOLVER == S_EM) {                 // EulerMaruyama.inl:41-72
+            inc = neel_f<ANISO>(L, m, h) + neel_s<ANISO>(L, m, zz);
+        } else if (SOLVER == S_HEUN) {        // Heun_Strong.inl:36-47
+            const V3<R> f0 = neel_f<ANISO>(L, m, h);
+            const V3<R> s0 = neel_s<ANISO>(L, m, zz);
+            const V3<R> sup = m + f0 + s0;
+            const V3<R> f1 = neel_f<ANISO>(L, sup, h);
+            inc = (R)0.5 * (f1 + f0) + s0;
+        } else {                              // Millstein.inl:34-44: y + a dt + S dW + drift (dW.dW)
+            const R zz2 = dot(zz, zz);
+            inc = neel_det<ANISO>(L, m, h) + neel_s<ANISO>(L, m, zz) + (L.d * zz2) * m;
+        }
+        y[0] += (double)inc.x; y[1] += (double)inc.y; y[2] += (double)inc.z;
+    }
+};
+
+// ================================================================ BrownAndNeel
+// Problems/BrownAndNeelRelaxation.inl: det (:222-271), S full/simple (:39-170),
+// Stratonovich->Ito drift full/simple (:179-219).  y = (n, m), dW = (dW_F, dW_H).
+template <int ANISO, class R> struct BNLane {
+    R pf, pt;                 // -2K/Ms, -2K V
+    R a, bpd, c, d;           // a dt, (b+d) dt, c dt, d dt
+    R cFs, dsig, bpds, as;    // cF sqrt(dt), d sigma sqrt(dt), (b+d|b) sigma sqrt(dt), a sigma sqrt(dt)
+    R kn1, kn2, kn3, km;      // drift coefficients times dt
+    double w;
+    bool simple;
+};
+template <int ANISO, class R>
+__device__ __forceinline__ void bn_load(const DevRequest& rq, uint64_t p, BNLane<ANISO, R>& L) {
+    const double a = ld(rq, P_A, p), b = ld(rq, P_B, p), s = ld(rq, P_SIG, p);
+    const double c = ld(rq, P_C, p), cF = ld(rq, P_CF, p), d = ld(rq, P_DD, p);
+    const bool simple = (rq.flags & RQ_BN_SIMPLE) != 0;
+    const double dt = rq.dt, sq = rq.sqdt, s2 = s * s, cF2 = cF * cF, bpd = b + d;
+    L.simple = simple;
+    L.pf = (R)ld(rq, P_K1, p); L.pt = (R)ld(rq, P_PT, p);
+    L.a = (R)(a * dt); L.bpd = (R)(bpd * dt); L.c = (R)(c * dt); L.d = (R)(d * dt);
+    L.cFs = (R)(cF * sq); L.dsig = (R)(d * s * sq); L.as = (R)(a * s * sq);
+    L.bpds = (R)((simple ? b : bpd) * s * sq);
+    L.kn1 = (R)(-cF2 * dt);
+    L.kn2 = (R)(simple ? 0.0 : s2 * d * a * dt);
+    L.kn3 = (R)(simple ? 0.0 : -0.5 * d * d * s2 * dt);
+    L.km = (R)(simple ? -((a * a + b * b) * s2) * dt : -(cF2 + (a * a + bpd * bpd) * s2) * dt);
+    L.w = ld(rq, P_MU, p);
+}
+// (det + drift) dt for both blocks
+template <int ANISO, class R>
+__device__ __forceinline__ void bn_f(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m, V3<R> h, V3<R>& fn, V3<R>& fm, bool with_drift) {
+    if (ANISO == 1) {
+        const R mn = dot(m, n);
+        const V3<R> H = (L.pf * mn) * n + h;
+        const V3<R> Teff = (L.pt * mn) * cross(n, m);      // n x (pt (m.n) m), UniaxialAnisotropy.h:146-150
+        const V3<R> mxH = cross(m, H);
+        const V3<R> wB = L.c * Teff + L.d * mxH;
+        const V3<R> wN = (L.bpd * mxH + L.c * Teff) - L.a * H;
+        fn = cross(wB, n);
+        fm = cross(wN, m);
+    } else {                                               // cubic: zero deterministic vector (.inl:225-229)
+        fn = {(R)0, (R)0, (R)0};
+        fm = {(R)0, (R)0, (R)0};
+    }
+    if (with_drift) {
+        fn = fn + L.kn1 * n;
+        if (!L.simple) fn = fn + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
+        fm = fm + L.km * m;
+    }
+}
+template <int ANISO, class R>
+__device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m, V3<R> zF, V3<R> zH, V3<R>& sn, V3<R>& sm) {
+    sn = L.cFs * cross(zF, n);
+    sm = L.bpds * (zH - dot(m, zH) * m) + L.as * cross(m, zH);
+    if (!L.simple) {
+        sn = sn + L.dsig * (dot(n, m) * zH - dot(n, zH) * m);
+        sm = sm + L.cFs * cross(zF, m);
+    }
+}
+
+template <int ANISO, int SOLVER, class R> struct BNCfg {
+    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2;
+    using Lane = BNLane<ANISO, R>;
+    struct Cache {};
+    __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { bn_load<ANISO, R>(rq, p, L); }
+    __device__ static void init_cache(const Lane&, const double*, Cache&) {}
+    __device__ static void weights(const Lane& L, double w[6]) { w[0] = w[1] = w[2] = 1.0; w[3] = w[4] = w[5] = L.w; }
+    __device__ static void output(const Lane&, const double* y, const Cache&, double o[6]) {
+        for (int i = 0; i < 6; ++i) o[i] = y[i];
+    }
+    __device__ __forceinline__ static void step(const DevRequest&, const Lane& L, double* y, Cache&, const R* z, V3<double> hd) {
+        const V3<R> n = {(R)y[0], (R)y[1], (R)y[2]}, m = {(R)y[3], (R)y[4], (R)y[5]};
+        const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
+        const V3<R> zF = {z[0], z[1], z[2]}, zH = {z[3], z[4], z[5]};
+        V3<R> in, im;
+        if (SOLVER == S_EM) {
+            V3<R> fn, fm, sn, sm;
+            bn_f<ANISO>(L, n, m, h, fn, fm, true);
+            bn_s<ANISO>(L, n, m, zF, zH, sn, sm);
+            in = fn + sn; im = fm + sm;
+        } else if (SOLVER == S_HEUN) {
+            V3<R> fn0, fm0, sn, sm, fn1, fm1;
+            bn_f<ANISO>(L, n, m, h, fn0, fm0, true);
+            bn_s<ANISO>(L, n, m, zF, zH, sn, sm);
+            bn_f<ANISO>(L, n + fn0 + sn, m + fm0 + sm, h, fn1, fm1, true);
+            in = (R)0.5 * (fn1 + fn0) + sn;
+            im = (R)0.5 * (fm1 + fm0) + sm;
+        } else {   // Millstein: y + det dt + S dW + drift (dW.dW)
+            V3<R> fn, fm, sn, sm, dn, dm;
+            bn_f<ANISO>(L, n, m, h, fn, fm, false);
+            bn_s<ANISO>(L, n, m, zF, zH, sn, sm);
+            // drift (dW.dW) = drift dt (z.z) and the drift coefficients already carry dt
+            const R zz = dot(zF, zF) + dot(zH, zH);
+            dn = L.kn1 * n; dm = L.km * m;
+            if (!L.simple) dn = dn + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
+            in = fn + sn + zz * dn;
+            im = fm + sm + zz * dm;
+        }
+        y[0] += (double)in.x; y[1] += (double)in.y; y[2] += (double)in.z;
+        y[3] += (double)im.x; y[4] += (double)im.y; y[5] += (double)im.z;
+    }
+};
+
+// ================================================================ NeelSpherical
+// Problems/NeelRelaxationSpherical.h: chart rotation (:124-145, :546-602), chart
+// vectors and projection P (:162-236), det = P H (:239-265), S = sigma P
+// (:267-293), drift (-D/2 cos(theta)/sin(theta), 0) (:295-309), finish (:440-471),
+// output (sin t cos p, sin t sin p, cos t) (:499-516).
+template <class R> struct Trig { R st, ct, sp, cp; };
+__device__ __forceinline__ void trig_of(double th, double ph, Trig<double>& T) {
+    sincos(th, &T.st, &T.ct);
+    sincos(ph, &T.sp, &T.cp);
+}
+__device__ __forceinline__ void trig_of(double th, double ph, Trig<float>& T) {
+    // hardware sin/cos take revolutions; the reduction to one turn is done in double
+    double rt = th * INV_TWO_PI, rp = ph * INV_TWO_PI;
+    rt -= floor(rt); rp -= floor(rp);
+    const float ft = (float)rt, fp = (float)rp;
+    T.st = __builtin_amdgcn_sinf(ft); T.ct = __builtin_amdgcn_cosf(ft);
+    T.sp = __builtin_amdgcn_sinf(fp); T.cp = __builtin_amdgcn_cosf(fp);
+}
+
+template <int ANISO, class R> struct SphLane {
+    R a, b, dr, sg, dt;       // a, b, -D/2, sigma sqrt(dt), dt
+    AnisoLane<R> A;
+    double w;
+};
+template <class R> struct Chart { V3<R> P0, P1; R drift0; };
+
+// chart quantities from the (possibly rotated) coordinates' trig values
+template <int ANISO, class R>
+__device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Trig<R>& T, bool rot, V3<R> h, Chart<R>& c, V3<R>& det) {
+    R ods = (R)1 / T.st;
+    if (isinf(ods)) ods = (R)0;
+    V3<R> M, et, ep;
+    if (!rot) {
+        M = {T.st * T.cp, T.st * T.sp, T.ct};
+        et = {T.ct * T.cp, T.ct * T.sp, -T.st};
+        ep = {-T.sp, T.cp, (R)0};
+    } else {
+        M = {-T.ct, T.st * T.sp, T.st * T.cp};
+        et = {T.st, T.ct * T.sp, T.ct * T.cp};
+        ep = {(R)0, T.cp, -T.sp};
+    }
+    c.P0 = L.b * et - L.a * ep;
+    c.P1 = ods * (L.a * et + L.b * ep);
+    if (ods == (R)0) {
+        if (!rot) c.P1.z = L.a; else c.P1.x = -L.a;
+    }
+    c.drift0 = L.dr * T.ct * ods;     // cos_t = rot ? -M(0) : M(2) = ct in both charts
+    const V3<R> H = aniso_field<ANISO>(L.A, M) + h;
+    det = {dot(c.P0, H), dot(c.P1, H), (R)0};
+}
+
+template <int ANISO, int SOLVER, class R> struct SphCfg {
+    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4;
+    using Lane = SphLane<ANISO, R>;
+    using Cache = Trig<R>;          // trig of the current (unrotated) state
+    __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) {
+        L.a = (R)ld(rq, P_A, p); L.b = (R)ld(rq, P_B, p);
+        L.dr = (R)(-0.5 * ld(rq, P_D, p));
+        L.sg = (R)(ld(rq, P_SIG, p) * rq.sqdt);
+        L.dt = (R)rq.dt;
+        load_aniso<ANISO, R>(rq, p, L.A);
+        L.w = ld(rq, P_MU, p);
+    }
+    __device__ static void init_cache(const Lane&, const double* y, Cache& T) { trig_of(y[0], y[1], T); }
+    __device__ static void weights(const Lane& L, double w[3]) { w[0] = w[1] = w[2] = L.w; }
+    __device__ static void output(const Lane&, const double*, const Cache& T, double o[3]) {
+        o[0] = (double)(T.st * T.cp); o[1] = (double)(T.st * T.sp); o[2] = (double)T.ct;
+    }
+    // prepareCalculations: rotate 90 degrees about y when theta leaves [tmin, tmax] (:568-580)
+    __device__ __forceinline__ static bool prepare(const DevRequest& rq, const double* y, double Y[2], Trig<R>& T) {
+        const bool rot = (y[0] < rq.tmin) || (y[0] > rq.tmax);
+        Y[0] = y[0]; Y[1] = y[1];
+        if (rot) {
+            const double st = sin(y[0]), ct = cos(y[0]), sp = sin(y[1]), cp = cos(y[1]);
+            Y[0] = acos(-cp * st);
+            Y[1] = atan2(sp * st, ct);
+            Trig<double> Td;
+            trig_of(Y[0], Y[1], Td);
+            T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
+        }
+        return rot;
+    }
+    // finishCalculations (:440-471), then the trig of the new state
+    __device__ __forceinline__ static void finish(const double r[2], bool rot, double* y, Trig<R>& T) {
+        if (rot) {
+            const double st = sin(r[0]), ct = cos(r[0]), sp = sin(r[1]), cp = cos(r[1]);
+            y[0] = acos(cp * st);
+            y[1] = atan2(sp * st, -ct);
+        } else {
+            y[0] = fmod(r[0], TWO_PI);
+            y[1] = fmod(r[1], TWO_PI);
+        }
+        trig_of(y[0], y[1], T);
+    }
+    __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& T, const R* z, V3<double> hd) {
+        const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
+        const V3<R> zz = {z[0], z[1], z[2]};
+        double Y[2];
+        Trig<R> T0 = T;
+        const bool rot = prepare(rq, y, Y, T0);
+        Chart<R> c0; V3<R> det0;
+        make_chart<ANISO>(L, T0, rot, h, c0, det0);
+        const R s0 = L.sg * dot(c0.P0, zz), s1 = L.sg * dot(c0.P1, zz);
+        double r[2];
+        if (SOLVER == S_EM) {                         // EulerMaruyama.inl:41-72
+            r[0] = Y[0] + (double)((det0.x + c0.drift0) * L.dt + s0);
+            r[1] = Y[1] + (double)(det0.y * L.dt + s1);
+        } else if (SOLVER == S_HEUN) {                // chart of y_i kept for both stages
+            const R f00 = (det0.x + c0.drift0) * L.dt, f01 = det0.y * L.dt;
+            const double sup0 = Y[0] + (double)(f00 + s0), sup1 = Y[1] + (double)(f01 + s1);
+            Trig<R> T1;
+            if (rot) { Trig<double> Td; trig_of(sup0, sup1, Td); T1 = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp}; }
+            else trig_of(sup0, sup1, T1);
+            Chart<R> c1; V3<R> det1;
+            make_chart<ANISO>(L, T1, rot, h, c1, det1);
+            const R f10 = (det1.x + c1.drift0) * L.dt, f11 = det1.y * L.dt;
+            r[0] = Y[0] + (double)((R)0.5 * (f10 + f00) + s0);
+            r[1] = Y[1] + (double)((R)0.5 * (f11 + f01) + s1);
+        } else {                                      // Millstein: y + det dt + S dW + drift (dW.dW)
+            const R dWdW = L.dt * dot(zz, zz);
+            r[0] = Y[0] + (double)(det0.x * L.dt + s0 + c0.drift0 * dWdW);
+            r[1] = Y[1] + (double)(det0.y * L.dt + s1);
+        }
+        finish(r, rot, y, T);
+    }
+};
+
+// ================================================================ ImplicitMidpoint (FP64 Newton)
+// Implicit_Midpoint.inl:38-188: F(y) = y_i + a(ybar) dt + b(ybar) dW - y with
+// ybar = (y + y_i)/2, prepare/finish around the stage, field at t_{n+1} + dt/2.
+// The Jacobian is exact (forward-mode duals through the whole residual);
+// Newton stops when |dx| <= Accuracy_Goal or after Max_Iterations.
+template <int N> struct Dual {
+    double v, d[N];
+    __device__ Dual() {}
+    __device__ Dual(double x) : v(x) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) d[i] = 0.0;
+    }
+};
+template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v + b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i]; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v - b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i]; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a) { Dual<N> r; r.v = -a.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = -a.d[i]; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) { Dual<N> r; r.v = a.v * b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i]; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator*(double s, const Dual<N>& b) { Dual<N> r; r.v = s * b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = s * b.d[i]; return r; }
+template <int N> __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r; r.v = a.v / b.v; const double ib = 1.0 / b.v;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> chain(const Dual<N>& a, double f, double df) { Dual<N> r; r.v = f;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i]; return r; }
+template <int N> __device__ __forceinline__ void dsincos(const Dual<N>& a, Dual<N>& s, Dual<N>& c) {
+    double sv, cv; sincos(a.v, &sv, &cv); s = chain(a, sv, cv); c = chain(a, cv, -sv);
+}
+template <int N> __device__ __forceinline__ Dual<N> dacos(const Dual<N>& a) { return chain(a, acos(a.v), -1.0 / sqrt(1.0 - a.v * a.v)); }
+template <int N> __device__ __forceinline__ Dual<N> datan2(const Dual<N>& y, const Dual<N>& x) {
+    Dual<N> r; r.v = atan2(y.v, x.v); const double den = 1.0 / (x.v * x.v + y.v * y.v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) * den;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> dfmod(const Dual<N>& a, double m) { Dual<N> r = a; r.v = fmod(a.v, m); return r; }
+
+template <int N> struct DV3 { Dual<N> x, y, z; };
+template <int N> __device__ __forceinline__ DV3<N> operator+(const DV3<N>& a, const DV3<N>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <int N> __device__ __forceinline__ DV3<N> operator-(const DV3<N>& a, const DV3<N>& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <int N> __device__ __forceinline__ DV3<N> operator*(const Dual<N>& s, const DV3<N>& a) { return {s * a.x, s * a.y, s * a.z}; }
+template <int N> __device__ __forceinline__ DV3<N> operator*(double s, const DV3<N>& a) { return {s * a.x, s * a.y, s * a.z}; }
+template <int N> __device__ __forceinline__ Dual<N> ddot(const DV3<N>& a, const DV3<N>& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <int N> __device__ __forceinline__ DV3<N> dcross(const DV3<N>& a, const DV3<N>& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <int N> __device__ __forceinline__ DV3<N> dconst(V3<double> v) { return {Dual<N>(v.x), Dual<N>(v.y), Dual<N>(v.z)}; }
+template <int N, int ANISO> __device__ __forceinline__ DV3<N> daniso(const AnisoLane<double>& A, const DV3<N>& m) {
+    const DV3<N> ez = dconst<N>(A.ez);
+    if (ANISO == 1) return (A.k1 * ddot(m, ez)) * ez;
+    const DV3<N> ex = dconst<N>(A.ex), ey = dconst<N>(A.ey);
+    const Dual<N> c1 = ddot(m, ex), c2 = ddot(m, ey), c3 = ddot(m, ez);
+    const Dual<N> c1s = c1 * c1, c2s = c2 * c2, c3s = c3 * c3;
+    return A.k1 * (((c2s + c3s) * c1) * ex + ((c1s + c3s) * c2) * ey + ((c1s + c2s) * c3) * ez);
+}
+
+// linear solve A x = b (DY <= 6), Gaussian elimination with partial pivoting
+template <int N> __device__ __forceinline__ bool solve_lin(double A[N][N], double b[N], double x[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        int p = k; double best = fabs(A[k][k]);
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) { const double v = fabs(A[i][k]); if (v > best) { best = v; p = i; } }
+        if (best == 0.0) return false;
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {      // swap rows k and p without dynamic indexing
+            if (i == p) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) { const double t = A[k][j]; A[k][j] = A[i][j]; A[i][j] = t; }
+                const double t = b[k]; b[k] = b[i]; b[i] = t;
+            }
+        }
+        const double inv = 1.0 / A[k][k];
+#pragma unroll
+        for (int i = k + 1; i < N; ++i) {
+            const double f = A[i][k] * inv;
+#pragma unroll
+            for (int j = k; j < N; ++j) A[i][j] -= f * A[k][j];
+            b[i] -= f * b[k];
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; --i) {
+        double s = b[i];
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) s -= A[i][j] * x[j];
+        x[i] = s / A[i][i];
+    }
+    return true;
+}
+
+// residuals per problem -------------------------------------------------------------
+template <int ANISO> struct IMNeel {
+    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4;
+    double a, b, as, bs, dt; AnisoLane<double> A; double w;
+    __device__ void load(const DevRequest& rq, uint64_t p) {
+        a = ld(rq, P_A, p); b = ld(rq, P_B, p);
+        const double s = ld(rq, P_SIG, p);
+        as = a * s * rq.sqdt; bs = b * s * rq.sqdt; dt = rq.dt;
+        load_aniso<ANISO, double>(rq, p, A); w = ld(rq, P_MU, p);
+    }
+    __device__ void weights(double wv[3]) const { wv[0] = wv[1] = wv[2] = w; }
+    __device__ void output(const double* y, double o[3]) const { o[0] = y[0]; o[1] = y[1]; o[2] = y[2]; }
+    __device__ void residual(const double* yv, const double* yi, V3<double> h, const double* z, double F[3], double J[3][3]) const {
+        DV3<3> y, ybar;
+        Dual<3>* yp = &y.x;
+        for (int i = 0; i < 3; ++i) { yp[i] = Dual<3>(yv[i]); yp[i].d[i] = 1.0; }
+        const DV3<3> yid = dconst<3>({yi[0], yi[1], yi[2]});
+        ybar = 0.5 * (y + yid);
+        const DV3<3> H = daniso<3, ANISO>(A, ybar) + dconst<3>(h);
+        const DV3<3> mxH = dcross(ybar, H);
+        const DV3<3> det = (a * dt) * mxH - (b * dt) * dcross(ybar, mxH);
+        const DV3<3> zz = dconst<3>({z[0], z[1], z[2]});
+        const DV3<3> sdw = bs * (zz - ddot(ybar, zz) * ybar) + as * dcross(ybar, zz);
+        const DV3<3> r = ybar + det + sdw + (0.5 * yid - 1.5 * y);
+        const Dual<3>* rp = &r.x;
+        for (int i = 0; i < 3; ++i) { F[i] = rp[i].v; for (int j = 0; j < 3; ++j) J[i][j] = rp[i].d[j]; }
+    }
+};
+template <int ANISO> struct IMBN {
+    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2;
+    double pf, pt, a, b, c, d, cF, s, dt, sq; bool simple; double w;
+    __device__ void load(const DevRequest& rq, uint64_t p) {
+        a = ld(rq, P_A, p); b = ld(rq, P_B, p); s = ld(rq, P_SIG, p);
+        c = ld(rq, P_C, p); cF = ld(rq, P_CF, p); d = ld(rq, P_DD, p);
+        pf = ld(rq, P_K1, p); pt = ld(rq, P_PT, p); dt = rq.dt; sq = rq.sqdt;
+        simple = (rq.flags & RQ_BN_SIMPLE) != 0; w = ld(rq, P_MU, p);
+    }
+    __device__ void weights(double wv[6]) const { wv[0] = wv[1] = wv[2] = 1.0; wv[3] = wv[4] = wv[5] = w; }
+    __device__ void output(const double* y, double o[6]) const { for (int i = 0; i < 6; ++i) o[i] = y[i]; }
+    __device__ void residual(const double* yv, const double* yi, V3<double> h, const double* z, double F[6], double J[6][6]) const {
+        Dual<6> yd[6], yb[6];
+        for (int i = 0; i < 6; ++i) { yd[i] = Dual<6>(yv[i]); yd[i].d[i] = 1.0; yb[i] = 0.5 * (yd[i] + Dual<6>(yi[i])); }
+        const DV3<6> n = {yb[0], yb[1], yb[2]}, m = {yb[3], yb[4], yb[5]};
+        DV3<6> fn = dconst<6>({0, 0, 0}), fm = dconst<6>({0, 0, 0});
+        if (ANISO == 1) {
+            const Dual<6> mn = ddot(m, n);
+            const DV3<6> H = (pf * mn) * n + dconst<6>(h);
+            const DV3<6> Teff = (pt * mn) * dcross(n, m);
+            const DV3<6> mxH = dcross(m, H);
+            const DV3<6> wB = c * Teff + d * mxH;
+            const DV3<6> wN = ((b + d) * mxH + c * Teff) - a * H;
+            fn = dt * dcross(wB, n);
+            fm = dt * dcross(wN, m);
+        }
+        const DV3<6> zF = dconst<6>({z[0], z[1], z[2]}), zH = dconst<6>({z[3], z[4], z[5]});
+        DV3<6> sn = (cF * sq) * dcross(zF, n);
+        DV3<6> sm = ((simple ? b : b + d) * s * sq) * (zH - ddot(m, zH) * m) + (a * s * sq) * dcross(m, zH);
+        if (!simple) {
+            sn = sn + (d * s * sq) * (ddot(n, m) * zH - ddot(n, zH) * m);
+            sm = sm + (cF * sq) * dcross(zF, m);
+        }
+        const Dual<6>* np = &n.x; const Dual<6>* mp = &m.x;
+        const Dual<6>* fnp = &fn.x; const Dual<6>* fmp = &fm.x; const Dual<6>* snp = &sn.x; const Dual<6>* smp = &sm.x;
+        for (int i = 0; i < 3; ++i) {
+            const Dual<6> r0 = np[i] + fnp[i] + snp[i] + Dual<6>(0.5 * yi[i]) - 1.5 * yd[i];
+            const Dual<6> r1 = mp[i] + fmp[i] + smp[i] + Dual<6>(0.5 * yi[3 + i]) - 1.5 * yd[3 + i];
+            F[i] = r0.v; F[3 + i] = r1.v;
+            for (int j = 0; j < 6; ++j) { J[i][j] = r0.d[j]; J[3 + i][j] = r1.d[j]; }
+        }
+    }
+};
+template <int ANISO> struct IMSph {
+    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4;
+    double a, b, sg, dt, tmin, tmax; AnisoLane<double> A; double w;
+    __device__ void load(const DevRequest& rq, uint64_t p) {
+        a = ld(rq, P_A, p); b = ld(rq, P_B, p); sg = ld(rq, P_SIG, p) * rq.sqdt; dt = rq.dt;
+        tmin = rq.tmin; tmax = rq.tmax;
+        load_aniso<ANISO, double>(rq, p, A); w = ld(rq, P_MU, p);
+    }
+    __device__ void weights(double wv[3]) const { wv[0] = wv[1] = wv[2] = w; }
+    __device__ void output(const double* y, double o[3]) const {
+        double st, ct, sp, cp; sincos(y[0], &st, &ct); sincos(y[1], &sp, &cp);
+        o[0] = st * cp; o[1] = st * sp; o[2] = ct;
+    }
+    __device__ void residual(const double* yv, const double* yi, V3<double> h, const double* z, double F[2], double J[2][2]) const {
+        Dual<2> y0(yv[0]), y1(yv[1]);
+        y0.d[0] = 1.0; y1.d[1] = 1.0;
+        Dual<2> r0 = 0.5 * (y0 + Dual<2>(yi[0])), r1 = 0.5 * (y1 + Dual<2>(yi[1]));
+        const bool rot = (r0.v < tmin) || (r0.v > tmax);
+        Dual<2> st, ct, sp, cp;
+        if (rot) {
+            dsincos(r0, st, ct); dsincos(r1, sp, cp);
+            const Dual<2> n0 = dacos(-(cp * st)), n1 = datan2(sp * st, ct);
+            r0 = n0; r1 = n1;
+        }
+        dsincos(r0, st, ct); dsincos(r1, sp, cp);
+        Dual<2> ods = Dual<2>(1.0) / st;
+        if (isinf(ods.v)) ods = Dual<2>(0.0);
+        DV3<2> M, et, ep;
+        const Dual<2> zero(0.0);
+        if (!rot) { M = {st * cp, st * sp, ct}; et = {ct * cp, ct * sp, -st}; ep = {-sp, cp, zero}; }
+        else { M = {-ct, st * sp, st * cp}; et = {st, ct * sp, ct * cp}; ep = {zero, cp, -sp}; }
+        const DV3<2> P0 = b * et - a * ep;
+        DV3<2> P1 = ods * (a * et + b * ep);
+        if (ods.v == 0.0) { if (!rot) P1.z = Dual<2>(a); else P1.x = Dual<2>(-a); }
+        const DV3<2> H = daniso<2, ANISO>(A, M) + dconst<2>(h);
+        const DV3<2> zz = dconst<2>({z[0], z[1], z[2]});
+        Dual<2> q0 = r0 + dt * ddot(P0, H) + sg * ddot(P0, zz);
+        Dual<2> q1 = r1 + dt * ddot(P1, H) + sg * ddot(P1, zz);
+        if (rot) {
+            Dual<2> s0, c0, s1, c1;
+            dsincos(q0, s0, c0); dsincos(q1, s1, c1);
+            const Dual<2> n0 = dacos(c1 * s0), n1 = datan2(s1 * s0, -c0);
+            q0 = n0; q1 = n1;
+        } else { q0 = dfmod(q0, TWO_PI); q1 = dfmod(q1, TWO_PI); }
+        const Dual<2> f0 = q0 + Dual<2>(0.5 * yi[0]) - 1.5 * y0;
+        const Dual<2> f1 = q1 + Dual<2>(0.5 * yi[1]) - 1.5 * y1;
+        F[0] = f0.v; F[1] = f1.v;
+        J[0][0] = f0.d[0]; J[0][1] = f0.d[1]; J[1][0] = f1.d[0]; J[1][1] = f1.d[1];
+    }
+};
+
+template <class Prob> struct IMCfg {
+    static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G;
+    using Lane = Prob;
+    struct Cache { unsigned fails; };
+    __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { L.load(rq, p); }
+    __device__ static void init_cache(const Lane&, const double*, Cache& c) { c.fails = 0; }
+    __device__ static void weights(const Lane& L, double w[DOUT]) { L.weights(w); }
+    __device__ static void output(const Lane& L, const double* y, const Cache&, double o[DOUT]) { L.output(y, o); }
+    __device__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& c, const double* z, V3<double> h) {
+        double yi[DY], yj[DY];
+#pragma unroll
+        for (int i = 0; i < DY; ++i) { yi[i] = y[i]; yj[i] = y[i]; }
+        bool conv = false;
+        for (uint32_t it = 0; it < rq.newton_max_iter && !conv; ++it) {
+            double F[DY], J[DY][DY], dx[DY];
+            L.residual(yj, yi, h, z, F, J);
+#pragma unroll
+            for (int i = 0; i < DY; ++i) F[i] = -F[i];
+            if (!solve_lin<DY>(J, F, dx)) break;
+            double nrm = 0.0;
+#pragma unroll
+            for (int i = 0; i < DY; ++i) { yj[i] += dx[i]; nrm += dx[i] * dx[i]; }
+            conv = sqrt(nrm) <= rq.newton_tol;
+        }
+        if (!conv) ++c.fails;
+#pragma unroll
+        for (int i = 0; i < DY; ++i) y[i] = yj[i];
+    }
+};
+
+// ================================================================ tile driver
+// Runs steps [tw.step_begin, tw.step_end) of one 64-particle tile
+// (SingleParticleSimulator::startSimulation, :129-159, for 64 particles at
+// once) and writes one moment-weighted partial sum per window and component
+// (MeanSimulationResult::operator+=, :87-119, for the wave's 64 particles).
+template <class Cfg, class R, bool KAHAN, bool MIDPOINT>
+__device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& tw) {
+    constexpr int DY = Cfg::DY, DW = Cfg::DW, DOUT = Cfg::DOUT, G = Cfg::G;
+    const int lane = threadIdx.x & 63;
+    const uint64_t N = rq.n_particles;
+    const uint64_t pr = (uint64_t)tw.tile * 64 + lane;
+    const bool active = pr < N;
+    const uint64_t p = active ? pr : N - 1;
+
+    typename Cfg::Lane L;
+    Cfg::load(rq, p, L);
+    double y[DY];
+#pragma unroll
+    for (int i = 0; i < DY; ++i) y[i] = rq.state[(uint64_t)i * N + p];
+    typename Cfg::Cache cache;
+    Cfg::init_cache(L, y, cache);
+    double w[DOUT];
+    Cfg::weights(L, w);
+    if (!active) {
+#pragma unroll
+        for (int c = 0; c < DOUT; ++c) w[c] = 0.0;
+    }
+    const uint32_t OS = rq.oversampling;
+    uint32_t row = 0;
+    auto flush = [&](const double* v) {
+#pragma unroll
+        for (int c = 0; c < DOUT; ++c) {
+            const double s = wave_sum(v[c] * w[c]);
+            if (lane == 0) tw.partial[((uint64_t)row * DOUT + c) * tw.pstride + tw.pslot] = s;
+        }
+        ++row;
+    };
+    if (tw.step_begin == 0) {                       // window 0: the start state (:135-136)
+        double o[DOUT];
+        Cfg::output(L, y, cache, o);
+        flush(o);
+    }
+    FieldCache fc;
+    fc.init(rq);
+    double acc[DOUT], comp[DOUT];
+#pragma unroll
+    for (int c = 0; c < DOUT; ++c) { acc[c] = 0.0; comp[c] = 0.0; }
+    uint32_t l = 0;
+    for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
+        R z[12];
+        normals12<R>(rq, p, n0 * DW, z);
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            const uint64_t n = n0 + s;
+            if (n < tw.step_end) {
+                const V3<double> h = fc.get<MIDPOINT>(rq, n);
+                Cfg::step(rq, L, y, cache, z + s * DW, h);
+                double o[DOUT];
+                Cfg::output(L, y, cache, o);
+#pragma unroll
+                for (int c = 0; c < DOUT; ++c) {
+                    if (KAHAN) {                      // Kahan summation (:148-153)
+                        const double yk = o[c] - comp[c];
+                        const double t = acc[c] + yk;
+                        comp[c] = (t - acc[c]) - yk;
+                        acc[c] = t;
+                    } else {
+                        acc[c] += o[c];
+                    }
+                }
+                if (++l == OS) {
+                    double mean[DOUT];
+#pragma unroll
+                    for (int c = 0; c < DOUT; ++c) { mean[c] = acc[c] / (double)OS; acc[c] = 0.0; comp[c] = 0.0; }
+                    flush(mean);
+                    l = 0;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < DY; ++i)
+        if (active) rq.state[(uint64_t)i * N + p] = y[i];
+    if constexpr (MIDPOINT) {
+        const unsigned f = wave_sum_u(active ? cache.fails : 0u);
+        if (lane == 0 && f) atomicAdd(rq.newton_fail, (unsigned long long)f);
+    }
+}
+
+}  // namespace sp

@@ -1,0 +1,45 @@
+// sp_kernels.h -- host-side entry points of the compiled kernel classes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "sp_types.h"
+
+namespace sp {
+
+struct KernelInfo {
+    int dy, dw, dout, group;   // state, noise and output sizes; steps per noise group
+    int vgprs, sgprs;          // register use of the per-request kernel
+    const char* name;
+};
+
+// One request over its tiles (grid = one wave per 64-particle tile).
+bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq,
+                  uint64_t step_begin, uint64_t step_end, double* partial, uint32_t pstride);
+// Persistent: waves pull TileWork items from an atomic counter until the table is exhausted.
+bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
+                  const TileWork* work, uint32_t n_work, uint32_t* counter);
+// Persistent, several kernel classes in one launch (uneven batched ensembles).
+bool launch_batch_multi(int mixed, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
+                        const TileWork* work, uint32_t n_work, uint32_t* counter);
+bool multi_supports(uint32_t kind);
+bool kernel_info(uint32_t kind, KernelInfo* info);
+
+// per-problem registries (one translation unit each)
+bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
+bool launch_plain_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
+bool launch_plain_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
+bool launch_batch_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
+bool launch_batch_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
+bool launch_batch_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
+bool info_neel(uint32_t, KernelInfo*);
+bool info_sph(uint32_t, KernelInfo*);
+bool info_bn(uint32_t, KernelInfo*);
+
+// deterministic second-stage reduction; one block per (entry, row)
+void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows);
+// test hooks
+void launch_philox_test(hipStream_t s, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                        uint64_t count, uint32_t* out);
+void launch_normals_test(hipStream_t s, uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int mixed,
+                         double* out);
+
+}  // namespace sp

@@ -1,0 +1,36 @@
+// sp_kernels_bn.hip -- kernel classes of problem 3 (BNCfg): EulerMaruyama, Heun_Strong and
+// Millstein in FP64 and mixed precision, ImplicitMidpoint in FP64, uniaxial and cubic anisotropy.
+#include "sp_kernels_impl.h"
+
+namespace sp {
+
+#define SP_CLASSES(X)                                                                   \
+    X(kernel_kind(3, S_EM, 1, 0), BNCfg<1 SP_COMMA S_EM SP_COMMA double>, double, true, false)        \
+    X(kernel_kind(3, S_EM, 2, 0), BNCfg<2 SP_COMMA S_EM SP_COMMA double>, double, true, false)        \
+    X(kernel_kind(3, S_HEUN, 1, 0), BNCfg<1 SP_COMMA S_HEUN SP_COMMA double>, double, true, false)    \
+    X(kernel_kind(3, S_HEUN, 2, 0), BNCfg<2 SP_COMMA S_HEUN SP_COMMA double>, double, true, false)    \
+    X(kernel_kind(3, S_MIL, 1, 0), BNCfg<1 SP_COMMA S_MIL SP_COMMA double>, double, true, false)      \
+    X(kernel_kind(3, S_MIL, 2, 0), BNCfg<2 SP_COMMA S_MIL SP_COMMA double>, double, true, false)      \
+    X(kernel_kind(3, S_EM, 1, 1), BNCfg<1 SP_COMMA S_EM SP_COMMA float>, float, false, false)         \
+    X(kernel_kind(3, S_EM, 2, 1), BNCfg<2 SP_COMMA S_EM SP_COMMA float>, float, false, false)         \
+    X(kernel_kind(3, S_HEUN, 1, 1), BNCfg<1 SP_COMMA S_HEUN SP_COMMA float>, float, false, false)     \
+    X(kernel_kind(3, S_HEUN, 2, 1), BNCfg<2 SP_COMMA S_HEUN SP_COMMA float>, float, false, false)     \
+    X(kernel_kind(3, S_MIL, 1, 1), BNCfg<1 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
+    X(kernel_kind(3, S_MIL, 2, 1), BNCfg<2 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
+    X(kernel_kind(3, S_IM, 1, 0), IMCfg<IMBN<1>>, double, true, true)                                \
+    X(kernel_kind(3, S_IM, 2, 0), IMCfg<IMBN<2>>, double, true, true)
+#define SP_COMMA ,
+
+bool launch_plain_bn(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq, uint64_t sb,
+                     uint64_t se, double* partial, uint32_t pstride) {
+    switch (kind) { SP_CLASSES(SP_CASE_PLAIN) default: return false; }
+}
+bool launch_batch_bn(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
+                     const TileWork* work, uint32_t n, uint32_t* counter) {
+    switch (kind) { SP_CLASSES(SP_CASE_BATCH) default: return false; }
+}
+bool info_bn(uint32_t kind, KernelInfo* info) {
+    switch (kind) { SP_CLASSES(SP_CASE_INFO) default: return false; }
+}
+
+}  // namespace sp

@@ -1,0 +1,106 @@
+// sp_kernels_common.hip -- reduction, dispatch over the per-problem registries, test hooks.
+#include "sp_kernels_impl.h"
+
+namespace sp {
+
+bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq, uint64_t sb,
+                  uint64_t se, double* partial, uint32_t pstride) {
+    switch (kind / 64) {
+        case 1: return launch_plain_neel(kind, grid, block, s, rq, sb, se, partial, pstride);
+        case 2: return launch_plain_sph(kind, grid, block, s, rq, sb, se, partial, pstride);
+        case 3: return launch_plain_bn(kind, grid, block, s, rq, sb, se, partial, pstride);
+    }
+    return false;
+}
+bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs, const TileWork* work,
+                  uint32_t n, uint32_t* counter) {
+    switch (kind / 64) {
+        case 1: return launch_batch_neel(kind, grid, block, s, reqs, work, n, counter);
+        case 2: return launch_batch_sph(kind, grid, block, s, reqs, work, n, counter);
+        case 3: return launch_batch_bn(kind, grid, block, s, reqs, work, n, counter);
+    }
+    return false;
+}
+bool kernel_info(uint32_t kind, KernelInfo* info) {
+    switch (kind / 64) {
+        case 1: return info_neel(kind, info);
+        case 2: return info_sph(kind, info);
+        case 3: return info_bn(kind, info);
+    }
+    return false;
+}
+
+// Fixed-order reduction: thread t sums slots t, t+256, ... in order, then a
+// fixed LDS tree; the result depends only on the partial values, not on how
+// the tiles were scheduled.
+__global__ __launch_bounds__(256) void k_reduce(const ReduceWork* __restrict__ work) {
+    const ReduceWork rw = work[blockIdx.y];
+    const uint32_t r = blockIdx.x;
+    if (r >= rw.n_rows) return;
+    __shared__ double sh[256];
+    __shared__ double shc[256];
+    const double* row = rw.partial + (uint64_t)r * rw.pstride;
+    double s = 0.0, sc = 0.0;
+    for (uint32_t i = threadIdx.x; i < rw.pstride; i += 256) {
+        const double v = row[i];
+        s += v;
+        if (i < rw.n_check) sc += v;
+    }
+    sh[threadIdx.x] = s;
+    shc[threadIdx.x] = sc;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) { sh[threadIdx.x] += sh[threadIdx.x + w]; shc[threadIdx.x] += shc[threadIdx.x + w]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rw.out[rw.row_base + r] = sh[0];
+        if (rw.check_out) rw.check_out[rw.row_base + r] = shc[0];
+    }
+}
+
+void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows) {
+    if (!n_work || !max_rows) return;
+    k_reduce<<<dim3(max_rows, n_work), dim3(256), 0, s>>>(work);
+}
+
+__global__ void k_philox_test(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                              uint64_t count, uint32_t* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const U4 r = philox4x32_10(U4{c0 + (uint32_t)i, c1, c2, c3}, k0, k1);
+    out[4 * i + 0] = r.x; out[4 * i + 1] = r.y; out[4 * i + 2] = r.z; out[4 * i + 3] = r.w;
+}
+void launch_philox_test(hipStream_t s, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                        uint64_t count, uint32_t* out) {
+    if (!count) return;
+    k_philox_test<<<dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s>>>(c0, c1, c2, c3, k0, k1, count, out);
+}
+
+// normal k of particle p's stream, one thread per Philox block
+__global__ void k_normals_test(uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int mixed, double* out) {
+    const uint64_t b = (first >> 2) + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t last_block = (first + count - 1) >> 2;
+    if (b > last_block) return;
+    const U4 w = noise_block(seed, particle, b, 0u);
+    double z[4];
+    if (mixed) {
+        float zf[4];
+        box_muller4_f32(w, zf);
+        for (int j = 0; j < 4; ++j) z[j] = (double)zf[j];
+    } else {
+        box_muller4_f64(w, z);
+    }
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t k = 4 * b + j;
+        if (k >= first && k < first + count) out[k - first] = z[j];
+    }
+}
+void launch_normals_test(hipStream_t s, uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int mixed,
+                         double* out) {
+    if (!count) return;
+    const uint64_t blocks = ((first + count - 1) >> 2) - (first >> 2) + 1;
+    k_normals_test<<<dim3((unsigned)((blocks + 255) / 256)), dim3(256), 0, s>>>(seed, particle, first, count, mixed, out);
+}
+
+}  // namespace sp

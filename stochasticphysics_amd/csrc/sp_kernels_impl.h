@@ -1,0 +1,59 @@
+// sp_kernels_impl.h -- kernel templates; included by the per-problem translation units.
+#pragma once
+#include "sp_device.h"
+#include "sp_kernels.h"
+
+namespace sp {
+
+// One wave per 64-particle tile of one request.
+template <class Cfg, class R, bool KAHAN, bool MID>
+__global__ __launch_bounds__(256) void k_plain(DevRequest rq, uint64_t step_begin, uint64_t step_end,
+                                               double* partial, uint32_t pstride) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (wave >= rq.n_tiles) return;
+    const TileWork tw{0u, wave, step_begin, step_end, partial, pstride, wave};
+    run_tile<Cfg, R, KAHAN, MID>(rq, tw);
+}
+
+__device__ __forceinline__ uint32_t next_work(uint32_t* counter) {
+    uint32_t idx = 0;
+    if ((threadIdx.x & 63) == 0) idx = atomicAdd(counter, 1u);
+    return __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
+}
+
+// Persistent: every wave loops over the work table; the exit condition is reached
+// by all waves once the counter passes n_work.
+template <class Cfg, class R, bool KAHAN, bool MID>
+__global__ __launch_bounds__(256) void k_batch(const DevRequest* __restrict__ reqs, const TileWork* __restrict__ work,
+                                               uint32_t n_work, uint32_t* counter) {
+    for (;;) {
+        const uint32_t idx = next_work(counter);
+        if (idx >= n_work) break;
+        const TileWork tw = work[idx];
+        const DevRequest rq = reqs[tw.req];
+        run_tile<Cfg, R, KAHAN, MID>(rq, tw);
+    }
+}
+
+template <class Cfg, class R, bool KAHAN, bool MID>
+bool fill_info(KernelInfo* info, const char* name) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, (const void*)k_plain<Cfg, R, KAHAN, MID>) != hipSuccess) return false;
+    info->dy = Cfg::DY; info->dw = Cfg::DW; info->dout = Cfg::DOUT; info->group = Cfg::G;
+    info->vgprs = a.numRegs; info->sgprs = 0; info->name = name;
+    return true;
+}
+
+}  // namespace sp
+
+// expands the per-class switch cases for one (problem, solver, aniso, mixed) class
+#define SP_CASE_PLAIN(KIND, CFG, R, KAHAN, MID)                                                        \
+    case KIND:                                                                                           \
+        k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);               \
+        return true;
+#define SP_CASE_BATCH(KIND, CFG, R, KAHAN, MID)                                                        \
+    case KIND:                                                                                           \
+        k_batch<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(reqs, work, n, counter);                     \
+        return true;
+#define SP_CASE_INFO(KIND, CFG, R, KAHAN, MID) \
+    case KIND: return fill_info<CFG, R, KAHAN, MID>(info, #CFG);

@@ -1,0 +1,107 @@
+// sp_params.cpp -- per-particle parameter calculators and request shapes (host).
+//
+// Evaluates, once per particle, the reference's parameter calculators
+// (Problems/Helpers/ParameterCalculatorNeel.h:44-92, ParameterCalculatorBrown.h:39-58,
+// ParameterCalculatorBrownAndNeel.h:67-76, UniaxialAnisotropy.h:71-85,
+// CubicAnisotropy.h:92-94) into the param-major block the kernels load.
+#include "sp_host.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace sp {
+
+static constexpr double kBoltzmann = 1.3806485279E-23;      // ParameterCalculatorNeel.h:43
+static constexpr double kPi = 3.14159265358979323846;
+
+// MyCEL sphere volume 4/3 pi r^3 (MagneticProperties.h:159, HydrodynamicProperties.h:46)
+static inline double sphere_volume(double r) { return 4.0 / 3.0 * kPi * r * r * r; }
+
+int request_dims(int problem, uint32_t* dy, uint32_t* dw, uint32_t* dout) {
+    switch (problem) {
+        case SP_PROBLEM_NEEL: *dy = 3; *dw = 3; *dout = 3; return 0;
+        case SP_PROBLEM_NEEL_SPHERICAL: *dy = 2; *dw = 3; *dout = 3; return 0;
+        case SP_PROBLEM_BROWN_AND_NEEL: *dy = 6; *dw = 6; *dout = 6; return 0;
+    }
+    return -1;
+}
+
+void compute_params(const sp_request& rq, const sp_particles& pt, double* out, uint64_t stride) {
+    const uint64_t N = rq.n_particles;
+    const bool bn = rq.problem == SP_PROBLEM_BROWN_AND_NEEL;
+    for (uint64_t p = 0; p < N; ++p) {
+        const double r = pt.magnetic_radius[p], Ms = pt.saturation_magnetisation[p];
+        const double alpha = pt.damping[p], gamma = pt.gyromagnetic_ratio[p], T = pt.temperature[p];
+        const double K = pt.anisotropy_constant[p];
+        const double V = sphere_volume(r), mu = Ms * V;
+        // NeelCalculator::calcNeelParams
+        double diffsquare = 2.0 * alpha * kBoltzmann * T / (gamma * mu);
+        if (std::isinf(diffsquare) || std::isnan(diffsquare)) diffsquare = 0.0;
+        double a, b;
+        if (std::isinf(alpha)) { a = 0.0; b = 0.0; }
+        else { a = -gamma / (1.0 + std::pow(alpha, 2)); b = std::abs(a) * alpha; }
+        const double D = (diffsquare == 0.0) ? 0.0 : -1.0 * std::pow(a, 2) * (1.0 + std::pow(alpha, 2)) * diffsquare;
+        const double sig = std::sqrt(diffsquare);
+        out[P_A * stride + p] = a;
+        out[P_B * stride + p] = b;
+        out[P_SIG * stride + p] = sig;
+        out[P_D * stride + p] = D;
+        out[P_MU * stride + p] = mu;
+        out[P_K1 * stride + p] = -2.0 * K / Ms;
+        out[P_PT * stride + p] = -2.0 * K * V;
+        double c = 0.0, cF = 0.0, d = 0.0;
+        if (bn) {   // BrownianRotationCalculator::calcBrownRotationParams
+            const double Vh = sphere_volume(pt.hydrodynamic_radius[p]), eta = pt.viscosity[p];
+            c = 1.0 / (6.0 * eta * Vh);
+            double diff = std::sqrt(2.0 * kBoltzmann * T * (6.0 * eta * Vh));
+            if (std::isinf(diff) || std::isnan(diff)) diff = 0.0;
+            if (std::isnan(c)) c = 0.0;
+            cF = c * diff;
+            d = c * mu;   // BrownAndNeelRelaxation.inl:47-48
+        }
+        out[P_C * stride + p] = c;
+        out[P_CF * stride + p] = cF;
+        out[P_DD * stride + p] = d;
+        for (int k = 0; k < 9; ++k) {
+            double v = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+            if (pt.axes) v = pt.axes[p * 9 + k];
+            out[(P_AX + k) * stride + p] = v;
+        }
+    }
+}
+
+// SingleParticleSimulator::calculateTimeAndField (SingleParticleSimulator.h:236-273)
+void time_and_field(const sp_request& rq, double* times, double* fields) {
+    const uint64_t points = rq.n_steps / rq.oversampling;
+    if (points == 0) return;
+    const uint64_t OS = rq.oversampling;
+    auto field = [&](double t, double h[3]) {
+        if (rq.field == SP_FIELD_SINUSOIDAL) {
+            const double w = 2.0 * kPi * rq.field_freq[0];
+            const double s = std::sin(w * t + rq.field_phase[0]);
+            for (int i = 0; i < 3; ++i) h[i] = rq.field_amplitude[i] * s + rq.field_offset[i];
+        } else if (rq.field == SP_FIELD_LISSAJOUS) {
+            for (int i = 0; i < 3; ++i) h[i] = rq.field_offset[i] + rq.field_amplitude[i] * std::sin(2.0 * kPi * rq.field_freq[i] * t + rq.field_phase[i]);
+        } else {
+            h[0] = h[1] = h[2] = 0.0;
+        }
+    };
+    double start = 0.0, end = 0.0;
+    const double timeshift = rq.dt * ((double)OS) * 0.5;
+    times[0] = start;
+    field(start, fields);
+    uint64_t counter = 0;
+    for (uint64_t i = 1; i < points; ++i) {
+        double tmp[3] = {0.0, 0.0, 0.0}, h[3];
+        start = end + rq.dt;
+        for (uint64_t l = OS; l--;) {
+            field(rq.dt * ((double)++counter), h);
+            for (int c = 0; c < 3; ++c) tmp[c] += h[c];
+        }
+        end = rq.dt * ((double)counter) + start;
+        for (int c = 0; c < 3; ++c) fields[i * 3 + c] = tmp[c] / (double)OS;
+        times[i] = (end - start) - timeshift;
+    }
+}
+
+}  // namespace sp

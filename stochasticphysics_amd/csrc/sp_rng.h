@@ -141,7 +141,7 @@ SP_HD void box_muller4_f64(U4 w, double z[4]) {
     z[3] = r1 * s1;
 }
 
-#if defined(__HIP_DEVICE_COMPILE__)
+#if defined(__HIPCC__) || defined(__HIP__)
 // four standard normals (FP32) from the same Philox block: gfx950 hardware
 // transcendentals.  v_log_f32 gives log2, v_sin_f32/v_cos_f32 take the angle in
 // revolutions, so sin(2 pi u) is one instruction.

@@ -1,0 +1,76 @@
+// sp_types.h -- request and work descriptors shared by the host runtime and the kernels.
+#pragma once
+#include <stdint.h>
+
+namespace sp {
+
+// Per-particle parameter block, param-major in HBM: params[k * N + p].  Filled on
+// the host by the reference's parameter calculators (sp_params.cpp).
+enum ParamIndex : int {
+    P_A = 0,      // NeelFactor1  a = -gamma/(1+alpha^2)              ParameterCalculatorNeel.h:67
+    P_B = 1,      // NeelFactor2  b = |a| alpha                        :68
+    P_SIG = 2,    // NoisePrefactor sigma = sqrt(2 alpha kB T/(gamma mu)) :80-86
+    P_D = 3,      // DriftPrefactor D = -a^2 (1+alpha^2) sigma^2      :77
+    P_MU = 4,     // saturation moment mu = Ms V (ensemble weight)    MagneticProperties.h:166
+    P_K1 = 5,     // field prefactor -2K/Ms (UniaxialAnisotropy.h:71-77, CubicAnisotropy.h:93)
+    P_PT = 6,     // torque prefactor -2K V (UniaxialAnisotropy.h:79-85), BrownAndNeel only
+    P_C = 7,      // BrownPrefactor c = 1/(6 eta V_h)                 ParameterCalculatorBrown.h:43
+    P_CF = 8,     // Brown_F_Noise cF = c sqrt(2 kB T 6 eta V_h)     :45-56
+    P_DD = 9,     // d = c mu                                         BrownAndNeelRelaxation.inl:48
+    P_AX = 10,    // body axes x(3) y(3) z(3): params 10..18 (Neel, NeelSpherical)
+    NPARAM = 19
+};
+
+// Uniform (per request) description of one ensemble on the device.
+struct DevRequest {
+    double dt, sqdt;
+    double tmin, tmax;             // NeelSpherical chart rotation window (rotation when theta outside)
+    double newton_tol;
+    double f_off[3], f_amp[3], f_w[3], f_ph[3];   // field: offset, amplitude, angular frequency, phase
+    uint64_t n_particles;
+    uint64_t seed;
+    uint64_t replay_count;         // normals available per particle in replay mode
+    uint32_t oversampling;
+    uint32_t field_type;           // 1 zero, 3 sinusoidal, 4 lissajous
+    uint32_t flags;                // bit0 BN simple model, bit1 replay noise
+    uint32_t newton_max_iter;
+    uint32_t n_tiles;              // ceil(N / 64)
+    uint32_t kind;                 // kernel class (see kernel_kind)
+    const double* params;          // [NPARAM][N]
+    double* state;                 // [DY][N]
+    const double* replay;          // [K][N] standard normals (replay mode)
+    unsigned long long* newton_fail;
+};
+
+enum : uint32_t { RQ_BN_SIMPLE = 1u, RQ_REPLAY = 2u };
+
+// One unit of work for one wave: 64 particles of one request for a step range.
+// Window rows: row r of the partial buffer holds window first_window + r;
+// window 0 is the initial state, window w >= 1 averages steps ((w-1) OS, w OS].
+struct TileWork {
+    uint32_t req;              // index into the DevRequest table
+    uint32_t tile;             // 64-particle tile index within the request
+    uint64_t step_begin;       // steps [step_begin, step_end), multiples of OS; step_begin % 4 == 0
+    uint64_t step_end;
+    double* partial;           // partial[(row * DOUT + c) * pstride + pslot]
+    uint32_t pstride;
+    uint32_t pslot;
+};
+
+// kernel class id = problem * 64 + solver * 4 + aniso * 2 + mixed  (host side bookkeeping)
+inline constexpr uint32_t kernel_kind(int problem, int solver, int aniso, int mixed) {
+    return (uint32_t)(problem * 64 + solver * 4 + (aniso - 1) * 2 + mixed);
+}
+
+// deterministic reduction of partial sums (one block per output row)
+struct ReduceWork {
+    const double* partial;     // rows of pstride doubles
+    uint32_t pstride;
+    uint32_t n_check;          // leading slots also summed into check_out (FP64 shadow comparison)
+    uint32_t n_rows;           // rows (windows * DOUT)
+    uint32_t row_base;         // first output row
+    double* out;               // out[row_base + r]
+    double* check_out;         // nullable
+};
+
+}  // namespace sp

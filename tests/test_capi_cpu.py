@@ -1,0 +1,85 @@
+"""CPU-side checks of the C ABI library (no GPU calls): it loads, exports every
+function include/stophys_gpu.h declares, and its host-side pieces agree with the oracle."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from stochasticphysics_amd import capi, configs, compute_params, host_normals
+from stochasticphysics_amd.capi import lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "stophys_gpu.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(sp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 20
+    L = lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(capi.EXPORTS) <= set(names)
+
+
+def test_abi_version():
+    assert lib().sp_abi_version() == 1
+
+
+@pytest.mark.parametrize("kind,name,value", [("solver", "EulerMaruyama", 1), ("solver", "Implicit_Midpoint", 3),
+                                             ("solver", "Millstein", 6), ("solver", "Heun_Strong", 7),
+                                             ("problem", "Neel", 1), ("problem", "NeelSpherical", 2),
+                                             ("problem", "BrownAndNeel", 3), ("field", "none", 1),
+                                             ("field", "sinusoidal", 3), ("field", "lissajous", 4),
+                                             ("anisotropy", "uniaxial", 1), ("anisotropy", "cubic", 2)])
+def test_enum_strings(kind, name, value):
+    v = C.c_int32()
+    assert lib().sp_parse_enum(kind.encode(), name.encode(), C.byref(v)) == 0 and v.value == value
+    assert lib().sp_enum_name(kind.encode(), value).decode() == name
+
+
+def test_unknown_enum_is_an_error():
+    v = C.c_int32()
+    assert lib().sp_parse_enum(b"solver", b"NoSuchSolver", C.byref(v)) == 1
+    assert b"NoSuchSolver" in lib().sp_last_error()
+
+
+def test_host_normals_equal_oracle():
+    for seed, p, first, count in [(0, 0, 0, 1000), (2 ** 40 + 3, 12345, 7, 333)]:
+        assert np.array_equal(host_normals(seed, p, first, count), oracle.normals(seed, p, first, count))
+
+
+def test_particle_params_match_oracle():
+    e = configs.config2(n=16, n_steps=200, oversampling=50)
+    P = compute_params(e)
+    for p in range(4):
+        q = oracle.neel_params(e.radius[p], e.Ms[p], e.alpha[p], e.gamma[p], e.T[p])
+        b = oracle.brown_params(e.r_h[p], e.viscosity[p], e.T[p])
+        assert P[0, p] == q["NeelFactor1"] and P[1, p] == q["NeelFactor2"]
+        assert P[2, p] == q["NoisePrefactor"] and P[3, p] == q["DriftPrefactor"]
+        mu = e.Ms[p] * 4.0 / 3.0 * math.pi * e.radius[p] ** 3
+        assert P[4, p] == pytest.approx(mu, rel=1e-15)
+        assert P[7, p] == b["BrownPrefactor"] and P[8, p] == b["Brown_F_Noise"]
+        assert P[9, p] == pytest.approx(b["BrownPrefactor"] * mu, rel=1e-15)
+
+
+def test_request_validation_errors():
+    e = configs.config1(n=8, n_steps=100, oversampling=10)
+    r, p, keep = e.to_c()
+    r.oversampling = 0
+    assert lib().sp_request_shape(C.byref(r), None, None, None, None) == 1
+    r.oversampling = 10
+    n_out, dy, dw, do = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+    assert lib().sp_request_shape(C.byref(r), C.byref(n_out), C.byref(dy), C.byref(dw), C.byref(do)) == 0
+    assert (n_out.value, dy.value, dw.value, do.value) == (10, 3, 3, 3)
+    out = np.zeros(capi.NPARAM * 8)
+    n_par = C.c_uint32()
+    r.solver = 2      # EulerMaruyama_Normalized: not built for the GPU backend
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), C.byref(n_par)) == 2

@@ -1,0 +1,217 @@
+"""GPU parity of the gfx950 kernels against the CPU oracle (run with -m gpu on an MI355X).
+
+Bars (stated in each test):
+  * Philox4x32-10: bit-exact vs the published KAT vectors and the oracle's copy.
+  * FP64 normals: bit-exact vs the oracle's Box-Muller (same stream definition).
+  * FP64 trajectories, replayed or Philox noise: |gpu - oracle| <= 1e-10 (state and <m> are
+    O(1) quantities, so this is a relative bar).
+  * zero temperature (deterministic LLG): <= 1e-12.
+  * mixed precision: max |<m>_mixed - <m>_oracle| <= 1e-3 on the same Philox stream.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from stochasticphysics_amd import Context, Ensemble, configs
+from stochasticphysics_amd.capi import DIM_W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def run_oracle(e: Ensemble, keep_final=True, threads=8):
+    return oracle.simulate(e.pid, {"EulerMaruyama": 1, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7}[e.solver],
+                           {"uniaxial": 1, "cubic": 2}[e.anisotropy], e.n_steps, e.oversampling, e.dt, e.seed,
+                           e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
+                           coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
+                           newton_max_iter=e.newton_max_iter, replay=e.replay, threads=threads, keep_final=keep_final)
+
+
+# ---------------------------------------------------------------- noise
+KAT = [([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+       ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+       ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1])]
+
+
+@pytest.mark.parametrize("ctr,key,exp", KAT)
+def test_philox_kat_device(ctx, ctr, key, exp):
+    out = ctx.dev_philox(ctr, key, 1)[0]
+    assert [int(x) for x in out] == exp
+
+
+def test_philox_device_vs_oracle(ctx):
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        ctr = [int(x) for x in rng.integers(0, 2 ** 32, 4)]
+        key = [int(x) for x in rng.integers(0, 2 ** 32, 2)]
+        dev = ctx.dev_philox(ctr, key, 257)
+        for i in (0, 1, 100, 256):
+            c = list(ctr)
+            c[0] = (c[0] + i) % 2 ** 32
+            assert np.array_equal(dev[i], oracle.philox(c, key))
+
+
+def test_normals_fp64_bit_exact(ctx):
+    for seed, particle, first, count in [(1, 0, 0, 4096), (123456789, 77777, 13, 1001), (2 ** 63 + 5, 2 ** 32 - 1, 2 ** 33, 64)]:
+        dev = ctx.dev_normals(seed, particle, first, count, "fp64")
+        ref = oracle.normals(seed, particle, first, count)
+        assert np.array_equal(dev, ref), np.max(np.abs(dev - ref))
+
+
+def test_normals_fp32_close(ctx):
+    dev32 = ctx.dev_normals(9, 4, 0, 1 << 16, "mixed")
+    ref = oracle.normals(9, 4, 0, 1 << 16)
+    # FP32 hardware log2/sqrt/sin/cos on the same uniforms: ~1e-6 relative, worst in the far tail
+    assert np.max(np.abs(dev32 - ref)) < 5e-5
+    assert abs(dev32.mean()) < 0.02 and abs(dev32.std() - 1) < 0.02
+
+
+# ---------------------------------------------------------------- helpers to build small ensembles
+def neel_ensemble(n=96, n_steps=2000, os_=10, solver="EulerMaruyama", aniso="uniaxial", field="none",
+                  T=295.0, seed=3, precision="fp64"):
+    rng = np.random.default_rng(seed)
+    axes = configs.random_axes(rng, n) if aniso == "cubic" else configs.easy_axis_axes(configs.unit_sphere(rng, n))
+    e = Ensemble(problem="Neel", solver=solver, field=field, anisotropy=aniso, precision=precision, n_steps=n_steps,
+                 oversampling=os_, dt=1e-12, seed=seed, radius=rng.uniform(8e-9, 12e-9, n), Ms=np.full(n, 4.8e5),
+                 alpha=rng.uniform(0.05, 0.3, n), gamma=np.full(n, 1.76e11), T=np.full(n, T),
+                 K=np.full(n, 1e4 if aniso == "uniaxial" else -1.1e4), axes=axes, y0=configs.unit_sphere(rng, n),
+                 field_amplitude=(0.004, 0.007, 0.012), field_freq=(3e8, 1.5e8, 4e8) if field == "lissajous" else (2e8, 0, 0),
+                 field_phase=(0.1, 0.2, 0.3), field_offset=(0.001, 0.0, -0.002))
+    return e
+
+
+def bn_ensemble(n=80, n_steps=1000, os_=10, solver="Heun_Strong", simple=False, aniso="uniaxial", field="sinusoidal",
+                T=295.0, seed=4, precision="fp64"):
+    rng = np.random.default_rng(seed)
+    return Ensemble(problem="BrownAndNeel", solver=solver, field=field, anisotropy=aniso, precision=precision,
+                    n_steps=n_steps, oversampling=os_, dt=1e-11, seed=seed, radius=rng.uniform(9e-9, 13e-9, n),
+                    Ms=np.full(n, 4.8e5), alpha=rng.uniform(0.05, 0.3, n), gamma=np.full(n, 1.76e11), T=np.full(n, T),
+                    K=np.full(n, 5e3), r_h=rng.uniform(20e-9, 30e-9, n), viscosity=np.full(n, 1e-3),
+                    y0=np.concatenate([configs.unit_sphere(rng, n), configs.unit_sphere(rng, n)], axis=1),
+                    bn_simple=simple, field_amplitude=(0.0, 0.003, 0.012), field_freq=(2e7, 0, 0))
+
+
+def sph_ensemble(n=96, n_steps=2000, os_=10, solver="Heun_Strong", aniso="cubic", field="lissajous", T=300.0, seed=5,
+                 rotate=True, precision="fp64"):
+    rng = np.random.default_rng(seed)
+    m0 = configs.unit_sphere(rng, n)
+    m0[:8] = [[0, 0, 1]] * 4 + [[0, 0, -1]] * 4   # start some particles at the poles
+    m0[:8] += 0.01 * rng.standard_normal((8, 3))
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    return Ensemble(problem="NeelSpherical", solver=solver, field=field, anisotropy=aniso, precision=precision,
+                    n_steps=n_steps, oversampling=os_, dt=2e-12, seed=seed, radius=11e-9 * np.exp(0.1 * rng.standard_normal(n)),
+                    Ms=np.full(n, 4.8e5), alpha=rng.uniform(0.05, 0.3, n), gamma=np.full(n, 1.76e11), T=np.full(n, T),
+                    K=np.full(n, -1.1e4 if aniso == "cubic" else 1e4), axes=configs.random_axes(rng, n),
+                    y0=configs.to_spherical(m0), coord_transform=rotate, min_angle=math.pi / 36,
+                    field_amplitude=(0.012, 0.012, 0.0), field_freq=(2.5e8, 2.45e8, 0.0))
+
+
+def with_replay(e: Ensemble, seed=99):
+    rng = np.random.default_rng(seed)
+    e.replay = rng.standard_normal((e.total_steps * DIM_W[e.pid], e.n))
+    return e
+
+
+def compare(g, o, tol, state=True, spherical=False):
+    assert np.max(np.abs(g["mean"] - o["mean"])) <= tol, np.max(np.abs(g["mean"] - o["mean"]))
+    assert np.allclose(g["weight"], o["weight"], rtol=1e-13, atol=0)
+    assert np.allclose(g["time"], o["times"], rtol=0, atol=0)
+    assert np.allclose(g["field"], o["fields"], rtol=1e-12, atol=1e-15)
+    if state:
+        gs, os_ = g["final_state"], o["final_state"]
+        if spherical:   # compare the points on the sphere (angles are chart-dependent near poles)
+            def pt(y):
+                return np.stack([np.sin(y[:, 0]) * np.cos(y[:, 1]), np.sin(y[:, 0]) * np.sin(y[:, 1]), np.cos(y[:, 0])], 1)
+            gs, os_ = pt(gs), pt(os_)
+        assert np.max(np.abs(gs - os_)) <= tol, np.max(np.abs(gs - os_))
+
+
+# ---------------------------------------------------------------- FP64 trajectory parity
+@pytest.mark.parametrize("solver", ["EulerMaruyama", "Heun_Strong", "Millstein"])
+@pytest.mark.parametrize("aniso", ["uniaxial", "cubic"])
+def test_neel_replay_parity(ctx, solver, aniso):
+    e = with_replay(neel_ensemble(solver=solver, aniso=aniso, field="lissajous"))
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+@pytest.mark.parametrize("solver", ["EulerMaruyama", "Heun_Strong", "Millstein"])
+def test_neel_philox_parity(ctx, solver):
+    e = neel_ensemble(solver=solver, field="sinusoidal", n=130)   # ragged: 3 tiles, last one partial
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+@pytest.mark.parametrize("solver,simple", [("EulerMaruyama", False), ("Heun_Strong", False), ("Heun_Strong", True),
+                                           ("Millstein", False)])
+def test_bn_parity(ctx, solver, simple):
+    e = bn_ensemble(solver=solver, simple=simple)
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+def test_bn_cubic_zero_det(ctx):
+    """BrownAndNeel + cubic: the reference's deterministic vector is zero (BrownAndNeelRelaxation.inl:225-229)."""
+    e = bn_ensemble(aniso="cubic", solver="EulerMaruyama", n=64)
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+@pytest.mark.parametrize("solver", ["EulerMaruyama", "Heun_Strong", "Millstein"])
+@pytest.mark.parametrize("aniso", ["uniaxial", "cubic"])
+def test_spherical_parity(ctx, solver, aniso):
+    e = sph_ensemble(solver=solver, aniso=aniso)
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10, spherical=True)
+
+
+def test_spherical_no_rotation(ctx):
+    e = sph_ensemble(solver="EulerMaruyama", rotate=False, aniso="uniaxial", field="none")
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10, spherical=True)
+
+
+@pytest.mark.parametrize("problem", ["Neel", "NeelSpherical", "BrownAndNeel"])
+def test_implicit_midpoint_parity(ctx, problem):
+    if problem == "Neel":
+        e = neel_ensemble(solver="Implicit_Midpoint", n=64, n_steps=500, field="sinusoidal")
+    elif problem == "NeelSpherical":
+        e = sph_ensemble(solver="Implicit_Midpoint", n=64, n_steps=500)
+    else:
+        e = bn_ensemble(solver="Implicit_Midpoint", n=64, n_steps=300)
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    assert g["newton_failures"] == 0 and o["newton_failures"] == 0
+    compare(g, o, 1e-10, spherical=problem == "NeelSpherical")
+
+
+def test_zero_temperature_parity(ctx):
+    """T = 0: deterministic LLG; GPU and oracle agree to 1e-12 over 20000 steps."""
+    e = neel_ensemble(n=64, n_steps=20000, os_=100, solver="Heun_Strong", field="lissajous", T=0.0)
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-12)
+
+
+def test_zero_temperature_fixed_point(ctx):
+    """Config-1 setup at T = 0: m0 on the easy axis stays exactly there."""
+    e = configs.config1(n=64, n_steps=2000, oversampling=100)
+    e.T = np.zeros(64)
+    g = ctx.run(e, keep_final=True)
+    assert np.array_equal(g["final_state"], np.tile([0.0, 0.0, 1.0], (64, 1)))
+    assert np.array_equal(g["mean"], np.tile([0.0, 0.0, 1.0], (e.n_out, 1)))
+
+
+# ---------------------------------------------------------------- mixed precision
+@pytest.mark.parametrize("make", [lambda: neel_ensemble(solver="Heun_Strong", n=256, n_steps=4000, field="sinusoidal"),
+                                  lambda: bn_ensemble(solver="Heun_Strong", n=256, n_steps=2000),
+                                  lambda: sph_ensemble(solver="Heun_Strong", n=256, n_steps=4000),
+                                  lambda: sph_ensemble(solver="Millstein", n=256, n_steps=4000)])
+def test_mixed_precision_error(ctx, make):
+    e = make()
+    e.precision = "mixed"
+    g = ctx.run(e)
+    o = run_oracle(e, keep_final=False)
+    err = np.max(np.abs(g["mean"] - o["mean"]))
+    assert g["precision_used"] == 1
+    assert err <= 1e-3, err
