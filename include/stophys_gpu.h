@@ -215,6 +215,18 @@ int sp_cancel(sp_ctx* ctx, uint64_t id);   /* replaces SimulationManager::abort 
 int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending, iterations,
                                                  busy_ms, wall_ms, particle_steps, fallbacks */);
 
+/* ---- parameter files: the reference's .ini schema (Settings/SimulationManagerSettings.h:34-186,
+ *      Provider/ParticleProvider.h:230-295) read into one request; CPU only ---- */
+typedef struct sp_settings sp_settings;
+int sp_settings_load(const char* ini_path, sp_settings** out);
+/* fills req and points parts at arrays owned by the settings object */
+int sp_settings_request(const sp_settings* s, sp_request* req, sp_particles* parts);
+const char* sp_settings_savefile(const sp_settings* s);
+/* writes Mean_Results {Time, Field, Results, Weight, Mean_Properties} as JSON
+ * (replaces the SerAr MAT/HDF5 writer, Results/SimulationResultManager.h:97-113) */
+int sp_settings_write_result(const sp_settings* s, const sp_result* res, const char* path);
+void sp_settings_free(sp_settings* s);
+
 /* ---- autotuner: times candidate tunings on a short synthetic run ---- */
 int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts,
                 sp_tuning* best, double* best_rate);

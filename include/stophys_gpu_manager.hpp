@@ -1,0 +1,115 @@
+// stophys_gpu_manager.hpp -- C++ SimulationManager surface over the C ABI.
+//
+// Header-only front-end with the public surface of the reference's
+// SimulationApplication::SimulationManager<prec, InstructionSet>
+// (Particle_Simulation/Simulator/SimulationManager.h:545-615): constructor from
+// the parameters, StartSimulationManager, waitUntilFinsihed (sic), isFinished,
+// abort and the static progress accessors.  The whole ensemble runs on one GPU
+// through libstophys_gpu.so; results are written like the reference's
+// Mean_Results when the manager finishes.  Errors are rethrown as
+// std::runtime_error, as the reference does.
+#pragma once
+#include <atomic>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "stophys_gpu.h"
+
+namespace SimulationApplication {
+
+class SimulationManagerGFX950 {
+public:
+    using Parameters = std::string;   // path of a SimulationManagerSettings parameter file
+
+    explicit SimulationManagerGFX950(const Parameters& parameter_file, int device = 0, std::string result_file = {})
+        : result_file_(std::move(result_file)) {
+        check(sp_settings_load(parameter_file.c_str(), &settings_));
+        check(sp_settings_request(settings_, &req_, &parts_));
+        if (result_file_.empty()) {
+            const char* s = sp_settings_savefile(settings_);
+            result_file_ = s && *s ? std::string(s) : std::string("Mean_Results");
+            const auto dot = result_file_.find_last_of('.');
+            if (dot != std::string::npos) result_file_ = result_file_.substr(0, dot);
+            result_file_ += ".json";
+        }
+        check(sp_ctx_create(device, &ctx_));
+    }
+    SimulationManagerGFX950(const SimulationManagerGFX950&) = delete;
+    SimulationManagerGFX950& operator=(const SimulationManagerGFX950&) = delete;
+    ~SimulationManagerGFX950() {
+        if (ctx_) sp_ctx_destroy(ctx_);
+        if (settings_) sp_settings_free(settings_);
+    }
+
+    // SimulationManager.h:549-574: starts the ensemble (asynchronously here)
+    void StartSimulationManager() {
+        if (started_) return;
+        check(sp_submit(ctx_, &req_, &parts_, &id_));
+        started_ = true;
+    }
+    // SimulationManager.h:577-584
+    void waitUntilFinsihed() {
+        if (!started_) throw std::runtime_error("SimulationManager: not started");
+        if (collected_) return;
+        check(sp_wait(ctx_, id_, -1));
+        collect();
+    }
+    // SimulationManager.h:587-592
+    bool isFinished() {
+        if (!started_) return false;
+        if (collected_) return true;
+        if (sp_wait(ctx_, id_, 0) != SP_OK) return false;
+        collect();
+        return true;
+    }
+    // SimulationManager.h:595-599
+    void abort() {
+        if (started_ && !collected_) sp_cancel(ctx_, id_);
+    }
+    // SimulationManager.h:601-615
+    static double& ProgressModifier() { static double v{0.0}; return v; }
+    static double& ProgressFactor() { static double v{1.0}; return v; }
+    static std::atomic<std::size_t>& ProgressCache() { static std::atomic<std::size_t> v{0}; return v; }
+
+    const sp_result& result() const { return res_; }
+    const std::vector<double>& time() const { return time_; }
+    const std::vector<double>& mean() const { return mean_; }
+    const std::string& result_file() const { return result_file_; }
+
+private:
+    static void check(int rc) {
+        if (rc != SP_OK) throw std::runtime_error(std::string("stophys_gpu: ") + sp_last_error());
+    }
+    void collect() {
+        uint64_t n_out = 0;
+        uint32_t dy = 0, dw = 0, dout = 0;
+        check(sp_request_shape(&req_, &n_out, &dy, &dw, &dout));
+        time_.assign(n_out, 0.0);
+        field_.assign(n_out * 3, 0.0);
+        mean_.assign(n_out * dout, 0.0);
+        weight_.assign(dout, 0.0);
+        res_ = sp_result{};
+        res_.time = time_.data();
+        res_.field = field_.data();
+        res_.mean = mean_.data();
+        res_.weight = weight_.data();
+        check(sp_result_get(ctx_, id_, &res_));
+        check(sp_settings_write_result(settings_, &res_, result_file_.c_str()));
+        ProgressCache().store(100);
+        collected_ = true;
+    }
+
+    sp_settings* settings_ = nullptr;
+    sp_ctx* ctx_ = nullptr;
+    sp_request req_{};
+    sp_particles parts_{};
+    uint64_t id_ = 0;
+    bool started_ = false, collected_ = false;
+    std::string result_file_;
+    sp_result res_{};
+    std::vector<double> time_, field_, mean_, weight_;
+};
+
+}  // namespace SimulationApplication

@@ -19,6 +19,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build"
 LIB = PKG / "libstophys_gpu.so"
+APP = PKG / "stophys_gpu_app"
 ARCH = "gfx950"
 
 # packed FP32 instructions are disabled for every HIP translation unit
@@ -29,6 +30,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
 SOURCES = [
     "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_common.hip",
     "sp_kernels_multi.hip", "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
+    "frontend/ini.cpp", "frontend/settings.cpp",
 ]
 
 
@@ -65,8 +67,16 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # command-line front-end (StoPhysApp_MultiArch options), linked against the library
+    app_tmp = APP.with_suffix(".tmp")
+    cmd = [hipcc(), "-O2", "-std=c++17", str(CSRC / "frontend" / "stophys_gpu_app.cpp"), "-o", str(app_tmp),
+           f"-L{PKG}", "-lstophys_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"app link failed:\n{r.stderr}")
+    os.replace(app_tmp, APP)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {APP}")
     return LIB
 
 

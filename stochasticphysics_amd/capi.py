@@ -67,7 +67,8 @@ EXPORTS = [
     "sp_compute_particle_params", "sp_ctx_create", "sp_ctx_destroy", "sp_ctx_set_tuning", "sp_ctx_get_tuning",
     "sp_ctx_set_memory_budget", "sp_ctx_last_run_stats", "sp_run", "sp_run_batch", "sp_submit", "sp_wait",
     "sp_result_get", "sp_cancel", "sp_server_stats", "sp_autotune", "sp_dev_philox", "sp_dev_normals",
-    "sp_host_normals", "sp_prepare", "sp_execute", "sp_release",
+    "sp_host_normals", "sp_prepare", "sp_execute", "sp_release", "sp_settings_load", "sp_settings_request",
+    "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free",
 ]
 
 _LIB = None
@@ -107,6 +108,13 @@ def lib():
         L.sp_execute.argtypes = [C.c_void_p, P(Result)]
         L.sp_release.argtypes = [C.c_void_p]
         L.sp_release.restype = None
+        L.sp_settings_load.argtypes = [C.c_char_p, P(C.c_void_p)]
+        L.sp_settings_request.argtypes = [C.c_void_p, P(Request), P(Particles)]
+        L.sp_settings_savefile.argtypes = [C.c_void_p]
+        L.sp_settings_savefile.restype = C.c_char_p
+        L.sp_settings_write_result.argtypes = [C.c_void_p, P(Result), C.c_char_p]
+        L.sp_settings_free.argtypes = [C.c_void_p]
+        L.sp_settings_free.restype = None
         L.sp_host_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d]
         _LIB = L
     return _LIB

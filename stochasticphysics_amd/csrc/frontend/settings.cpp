@@ -1,0 +1,354 @@
+// settings.cpp -- the reference's parameter files as engine requests.
+//
+// Reads a SimulationManagerSettings parameter file (Settings/SimulationManagerSettings.h:34-186)
+// and the per-particle files its Particle_Provider lists (Provider/ParticleProvider.h:230-295)
+// and produces one sp_request plus per-particle property arrays, i.e. what the
+// reference's SimulationManager builds one Problem object per particle from
+// (SimulationManager.h:373-478).  Key names follow the serialize() functions of
+// Settings/*.h and Properties/*.h; a few older keys of Examples/*.ini are accepted too.
+//
+// Choices where the reference draws from std::random_device (unreproducible):
+// random initial orientations/magnetisations and the discrete particle-type
+// selection use Philox stream 1 under the request seed.  Radius, shell and
+// anisotropy distributions are delta, normal or lognormal with the given mean and
+// width (for lognormal: the normal parameters that give that mean and standard
+// deviation); MyCEL's own parameterisation is not available to compare with.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../sp_host.h"
+#include "../sp_rng.h"
+#include "ini.h"
+
+namespace sp::frontend {
+
+struct Dist {
+    std::string type = "delta";
+    double width = 0.0;
+    bool relative = true;
+};
+
+struct ParticleFile {
+    double T = 295.0, radius = 0, Ms = 0, alpha = 0, gamma = 0, K = 0, r_h = 0, visc = 0;
+    int aniso = SP_ANISO_UNIAXIAL;
+    bool rand_orient = false, rand_mag = false;
+    double euler[3] = {0, 0, 0}, mag[3] = {0, 0, 1};
+    Dist rad, shell, aniso_dist;
+    uint64_t count = 0;
+};
+
+struct Settings {
+    sp_request rq{};
+    bool discrete = false;
+    std::vector<ParticleFile> particles;
+    std::string savefile;
+    // materialised arrays
+    std::vector<double> radius, Ms, alpha, gamma, T, K, r_h, visc, axes, y0;
+};
+
+static const std::string* opt(const ini::File& f, const std::string& sec, std::initializer_list<const char*> keys) {
+    for (const char* k : keys)
+        if (const std::string* v = f.find(sec, k)) return v;
+    return nullptr;
+}
+static double num(const ini::File& f, const std::string& sec, std::initializer_list<const char*> keys, double dflt, bool required = false) {
+    if (const std::string* v = opt(f, sec, keys)) return ini::to_double(*v);
+    if (required) throw std::runtime_error(std::string("missing key ") + *keys.begin() + " in [" + sec + "]");
+    return dflt;
+}
+static bool flag(const ini::File& f, const std::string& sec, std::initializer_list<const char*> keys, bool dflt) {
+    if (const std::string* v = opt(f, sec, keys)) return ini::to_bool(*v);
+    return dflt;
+}
+static std::vector<double> vec(const ini::File& f, const std::string& sec, std::initializer_list<const char*> keys) {
+    if (const std::string* v = opt(f, sec, keys)) return ini::to_vector(*v);
+    return {};
+}
+static int parse_enum_or_throw(const char* kind, const std::string& name) {
+    int32_t v = 0;
+    if (sp_parse_enum(kind, name.c_str(), &v) != SP_OK) throw std::runtime_error(std::string("unknown ") + kind + " '" + name + "'");
+    return v;
+}
+
+static ParticleFile load_particle(const std::string& path) {
+    const ini::File f = ini::parse_file(path);
+    ParticleFile p;
+    const std::string PP = "Particle_Properties";
+    p.T = num(f, PP, {"Temperature"}, 295.0, true);
+    const std::string MP = PP + ".Magnetic_Properties";
+    p.radius = num(f, MP, {"Magnetic_radius"}, 0, true);
+    p.Ms = num(f, MP, {"Saturation_magnetisation"}, 0, true);
+    p.alpha = num(f, MP, {"Damping_constant"}, 0, true);
+    p.gamma = num(f, MP, {"Gyromagnetic_ratio"}, 0, true);
+    const std::string* at = opt(f, MP, {"Type_of_anisotropy"});
+    p.aniso = at ? parse_enum_or_throw("anisotropy", *at) : SP_ANISO_UNIAXIAL;
+    p.K = num(f, PP, {p.aniso == SP_ANISO_CUBIC ? "K_cubic" : "K_uniaxial", "Anisotropy_1"}, 0, true);
+    const std::string HP = PP + ".Hydrodynamic_Properties";
+    p.r_h = num(f, HP, {"Hydrodynamic_Radius"}, 0.0);
+    p.visc = num(f, HP, {"Viscosity"}, 0.0);
+    const std::string IN = "Particle_Simulation_Initialization";   // ParticleSimulationInitSettings.h:70-91
+    p.rand_orient = flag(f, IN, {"Use_random_initial_particle_orientation"}, false);
+    p.rand_mag = flag(f, IN, {"Use_random_initial_particle_magnetisation"}, false);
+    auto e = vec(f, IN, {"Initial_particle_orientation"});
+    auto m = vec(f, IN, {"Initial_magnetisation_direction"});
+    for (int i = 0; i < 3; ++i) { if (i < (int)e.size()) p.euler[i] = e[i]; if (i < (int)m.size()) p.mag[i] = m[i]; }
+    const std::string SS = "Particle_Simulation_Settings";         // ParticleSimulationSettings.h:156-200
+    if (const std::string* v = opt(f, SS, {"Magnetic_radius_distribution_type"})) p.rad.type = *v;
+    p.rad.width = num(f, SS, {"Magnetic_radius_distribution_width"}, 0.0);
+    p.rad.relative = flag(f, SS, {"Use_relative_distribution_width_for_magnetic_radius"}, true);
+    if (const std::string* v = opt(f, SS, {"Hydrodynamic_shell_distribution_type"})) p.shell.type = *v;
+    p.shell.width = num(f, SS, {"Hydrodynamic_shell_distribution_width"}, 0.0);
+    p.shell.relative = flag(f, SS, {"Use_relative_distribution_width_for_hydrodynamic_shell"}, true);
+    if (const std::string* v = opt(f, SS, {"Distribution_type", "Anisotropy_constants_distribution_type"})) p.aniso_dist.type = *v;
+    const auto w = vec(f, SS, {"Sigma_K_uniaxial", "Sigma_K_cubic", "Anisotropy_constants_distribution_width"});
+    p.aniso_dist.width = w.empty() ? 0.0 : w[0];
+    p.aniso_dist.relative = flag(f, SS, {"Use_relative_distribution_width", "Use_relative_distribution_width_for_anisotropy_constants"}, true);
+    return p;
+}
+
+// ---- seeded draws (Philox stream tag 1: particle p, draw index k)
+struct Draw {
+    uint64_t seed, p; uint64_t k = 0;
+    double uniform() {   // (0,1)
+        const U4 w = noise_block(seed, p, k++, 1u);
+        return u01_f64(w.x);
+    }
+    double normal() {
+        const U4 w = noise_block(seed, p, k++, 1u);
+        double z[4];
+        box_muller4_f64(w, z);
+        return z[0];
+    }
+};
+
+static double draw_dist(const Dist& d, double mean, Draw& rng) {
+    if (d.type == "delta" || d.width == 0.0) return mean;
+    const double sd = d.relative ? d.width * mean : d.width;
+    if (d.type == "normal") return mean + sd * rng.normal();
+    if (d.type == "lognormal") {
+        const double s2 = std::log1p((sd * sd) / (mean * mean));
+        return mean * std::exp(std::sqrt(s2) * rng.normal() - 0.5 * s2);
+    }
+    throw std::runtime_error("unknown distribution type '" + d.type + "'");
+}
+
+Settings* load(const std::string& path) {
+    const ini::File f = ini::parse_file(path);
+    std::unique_ptr<Settings> s(new Settings());
+    sp_request& r = s->rq;
+    std::memset(&r, 0, sizeof(r));
+    // [Solver_Settings] (SolverSettings.h:106-142)
+    const std::string* sv = opt(f, "Solver_Settings", {"Solver"});
+    if (!sv) throw std::runtime_error("missing [Solver_Settings] Solver");
+    r.solver = parse_enum_or_throw("solver", *sv);
+    r.newton_max_iter = (uint32_t)num(f, "Solver_Settings", {"Max_Iterations"}, 100);
+    r.newton_tol = num(f, "Solver_Settings", {"Accuracy_Goal"}, 1e-10);
+    // [Simulation_Settings] (SimulationSettings.h:67-79)
+    const std::string SIM = "Simulation_Settings";
+    r.dt = num(f, SIM, {"Timestep"}, 0, true);
+    r.n_steps = (uint64_t)num(f, SIM, {"Number_of_steps"}, 0, true);
+    r.oversampling = (uint64_t)num(f, SIM, {"Oversampling"}, 1);
+    r.n_particles = (uint64_t)num(f, SIM, {"Number_of_simulations"}, 0, true);
+    r.seed = (uint64_t)num(f, SIM, {"Seed"}, 1);
+    r.precision = SP_PREC_FP64;
+    if (const std::string* pv = opt(f, SIM, {"Precision"})) r.precision = (*pv == "mixed") ? SP_PREC_MIXED : SP_PREC_FP64;
+    r.check_tol = 1e-3;
+    // [Problem_Settings] (ProblemSettings.h:55-237)
+    const std::string PS = "Problem_Settings";
+    const std::string* pt = opt(f, PS, {"Type_of_problem", "Problem_type"});
+    if (!pt) throw std::runtime_error("missing [Problem_Settings] Type_of_problem");
+    r.problem = parse_enum_or_throw("problem", *pt);
+    if (flag(f, PS, {"Use_simple_model"}, false)) r.flags |= SP_FLAG_BN_SIMPLE_MODEL;
+    if (flag(f, PS, {"Use_coordinate_transformation"}, false)) r.flags |= SP_FLAG_COORD_TRANSFORM;
+    r.min_angle = num(f, PS, {"Min_angle_transformation"}, 0.0);
+    // [Field_Parameters] (FieldProperties.h:61, Properties/Fields/*.hpp)
+    std::string FS = f.has_section("Field_Parameters") ? "Field_Parameters" : "Field_Properties";
+    const std::string* ft = opt(f, FS, {"Type_of_field"});
+    r.field = ft ? parse_enum_or_throw("field", *ft) : SP_FIELD_ZERO;
+    auto fill3 = [](double* dst, const std::vector<double>& v, bool broadcast) {
+        for (int i = 0; i < 3; ++i) dst[i] = i < (int)v.size() ? v[i] : (broadcast && v.size() == 1 ? v[0] : 0.0);
+    };
+    if (r.field == SP_FIELD_SINUSOIDAL || r.field == SP_FIELD_LISSAJOUS) {
+        fill3(r.field_offset, vec(f, FS, {"Offset_Field"}), false);
+        fill3(r.field_amplitude, vec(f, FS, {"Amplitude"}), false);
+        if (r.field == SP_FIELD_SINUSOIDAL) {
+            r.field_freq[0] = num(f, FS, {"Frequency"}, 0.0);
+            r.field_phase[0] = num(f, FS, {"Phase"}, 0.0);
+        } else {
+            const auto fr = vec(f, FS, {"Frequencies"});
+            if (fr.size() != 3) throw std::runtime_error("LissajousField: Frequencies needs 3 values");   // LissajousField.h:54-57
+            fill3(r.field_freq, fr, false);
+            const auto ph = vec(f, FS, {"Phases"});
+            if (!(ph.size() == 1 || ph.size() == 3)) throw std::runtime_error("LissajousField: Phases needs 1 or 3 values");
+            fill3(r.field_phase, ph, true);
+        }
+    }
+    // [Result_Settings] (ResultSettings.h:90-118)
+    if (const std::string* sf = opt(f, "Result_Settings", {"Savefile"})) s->savefile = *sf;
+    // [Particle_Provider] (ParticleProvider.h:230-295)
+    const std::string PP = "Particle_Provider";
+    s->discrete = flag(f, PP, {"Use_discrete_distribution_to_select_particle"}, false);
+    auto list = f.sections.find(PP + ".Particle_List");
+    auto nums = f.sections.find(PP + ".Particle_Numbers");
+    if (list == f.sections.end()) throw std::runtime_error("missing [Particle_Provider.Particle_List]");
+    for (auto& [name, file] : list->second) {
+        const std::string full = (!file.empty() && file[0] == '/') ? file : f.path + "/" + file;
+        ParticleFile pf = load_particle(full);
+        if (nums != f.sections.end() && nums->second.count(name)) pf.count = (uint64_t)ini::to_double(nums->second.at(name));
+        s->particles.push_back(pf);
+    }
+    if (s->particles.empty()) throw std::runtime_error("empty particle list");
+    // materialise N particles (Provider::getProvidedObject, ParticleSimulationSettings::applySettingsToParticleProperties)
+    const uint64_t N = r.n_particles;
+    uint32_t dy = 0, dw = 0, dout = 0;
+    if (request_dims(r.problem, &dy, &dw, &dout)) throw std::runtime_error("problem type not available on the GPU backend");
+    s->radius.resize(N); s->Ms.resize(N); s->alpha.resize(N); s->gamma.resize(N); s->T.resize(N); s->K.resize(N);
+    s->r_h.resize(N); s->visc.resize(N); s->axes.resize(N * 9); s->y0.resize(N * dy);
+    std::vector<uint64_t> used(s->particles.size(), 0);
+    uint64_t total_count = 0;
+    for (auto& p : s->particles) total_count += p.count;
+    const int aniso0 = s->particles[0].aniso;
+    for (uint64_t i = 0; i < N; ++i) {
+        Draw rng{r.seed, i};
+        size_t sel = 0;
+        if (s->discrete && total_count) {
+            double u = rng.uniform() * (double)total_count, acc = 0;
+            for (sel = 0; sel + 1 < s->particles.size(); ++sel) { acc += (double)s->particles[sel].count; if (u < acc) break; }
+        } else {
+            for (sel = 0; sel + 1 < s->particles.size() && used[sel] >= s->particles[sel].count; ++sel) {}
+        }
+        ++used[sel];
+        const ParticleFile& pf = s->particles[sel];
+        if (pf.aniso != aniso0) throw std::runtime_error("all particle types of one run must share the anisotropy type");
+        const double rad = draw_dist(pf.rad, pf.radius, rng);
+        const double shell = draw_dist(pf.shell, pf.r_h - pf.radius, rng);
+        s->radius[i] = rad; s->Ms[i] = pf.Ms; s->alpha[i] = pf.alpha; s->gamma[i] = pf.gamma; s->T[i] = pf.T;
+        s->K[i] = draw_dist(pf.aniso_dist, pf.K, rng);
+        s->r_h[i] = shell + rad; s->visc[i] = pf.visc;
+        // axes from Euler angles (NeelRelaxation.h:223-261); random: each angle U(0, 2 pi)
+        double e[3] = {pf.euler[0], pf.euler[1], pf.euler[2]};
+        if (pf.rand_orient) for (double& x : e) x = rng.uniform() * 6.283185307179586;
+        const double cphi = std::cos(e[0]), cth = std::cos(e[1]), cpsi = std::cos(e[2]);
+        const double sphi = std::sin(e[0]), sth = std::sin(e[1]), spsi = std::sin(e[2]);
+        const double ax[9] = {cphi * cpsi - cth * sphi * spsi, -sphi * cpsi - cth * cphi * spsi, sth * spsi,
+                              cth * sphi * cpsi + cphi * spsi, cth * cphi * cpsi - sphi * spsi, -sth * cpsi,
+                              sth * sphi, sth * cphi, cth};
+        for (int k = 0; k < 9; ++k) s->axes[i * 9 + k] = ax[k];
+        double m[3] = {pf.mag[0], pf.mag[1], pf.mag[2]};
+        if (pf.rand_mag) for (double& x : m) x = rng.normal();
+        const double mn = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (double& x : m) x /= mn;
+        if (r.problem == SP_PROBLEM_NEEL) { for (int k = 0; k < 3; ++k) s->y0[i * 3 + k] = m[k]; }
+        else if (r.problem == SP_PROBLEM_NEEL_SPHERICAL) {   // ParticleStateInitializer.h:132-144
+            s->y0[i * 2 + 0] = std::acos(m[2]);
+            s->y0[i * 2 + 1] = std::atan2(m[1], m[0]);
+        } else {                                             // BrownAndNeelRelaxation.inl:277-324
+            double n[3];
+            if (pf.rand_orient) { for (double& x : n) x = rng.normal(); }
+            else {
+                const double a = e[0], b = e[1], g = e[2];
+                n[0] = std::cos(a) * std::cos(g) - std::sin(a) * std::cos(b) * std::sin(g);
+                n[1] = -std::cos(a) * std::sin(g) - std::sin(a) * std::cos(b) * std::cos(g);
+                n[2] = std::sin(a) * std::sin(b);
+            }
+            const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            for (int k = 0; k < 3; ++k) { s->y0[i * 6 + k] = n[k] / nn; s->y0[i * 6 + 3 + k] = m[k]; }
+        }
+    }
+    r.anisotropy = aniso0;
+    return s.release();
+}
+
+static void write_array(std::ostream& o, const double* v, size_t n) {
+    o << '[';
+    char buf[40];
+    for (size_t i = 0; i < n; ++i) { std::snprintf(buf, sizeof buf, "%.17g", v[i]); o << (i ? "," : "") << buf; }
+    o << ']';
+}
+
+// Mean_Results {Time, Field, Results, Weight, Mean_Properties} (MeanSimulationResult.h:131-141), as JSON
+int write_result(const Settings& s, const sp_result& res, const std::string& path) {
+    std::ofstream o(path);
+    if (!o) return set_error(SP_ERR_INVALID_ARG, "cannot write " + path);
+    o << "{\"Mean_Results\":{\"Time\":";
+    write_array(o, res.time, res.n_out);
+    o << ",\"Field\":[";
+    for (uint64_t k = 0; k < res.n_out; ++k) { if (k) o << ','; write_array(o, res.field + 3 * k, 3); }
+    o << "],\"Results\":[";
+    for (uint64_t k = 0; k < res.n_out; ++k) { if (k) o << ','; write_array(o, res.mean + res.dim_out * k, res.dim_out); }
+    o << "],\"Weight\":";
+    write_array(o, res.weight, res.dim_out);
+    auto mean = [](const std::vector<double>& v) { double a = 0; for (double x : v) a += x; return v.empty() ? 0.0 : a / (double)v.size(); };
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  ",\"Mean_Properties\":{\"Temperature\":%.17g,\"Magnetic_radius\":%.17g,\"Saturation_magnetisation\":%.17g,"
+                  "\"Damping_constant\":%.17g,\"Gyromagnetic_ratio\":%.17g,\"Anisotropy_constant\":%.17g,"
+                  "\"Hydrodynamic_Radius\":%.17g,\"Viscosity\":%.17g}}",
+                  mean(s.T), mean(s.radius), mean(s.Ms), mean(s.alpha), mean(s.gamma), mean(s.K), mean(s.r_h), mean(s.visc));
+    o << buf;
+    std::snprintf(buf, sizeof buf, ",\"Engine\":{\"device_ms\":%.6f,\"particle_steps\":%.17g,\"precision_used\":%u,\"check_error\":%.6g}}",
+                  res.device_ms, res.particle_steps, res.precision_used, res.check_error);
+    o << buf << '\n';
+    return o ? SP_OK : set_error(SP_ERR_INVALID_ARG, "write failed: " + path);
+}
+
+}  // namespace sp::frontend
+
+struct sp_settings {
+    sp::frontend::Settings* s = nullptr;
+};
+
+extern "C" {
+
+int sp_settings_load(const char* ini_path, sp_settings** out) {
+    if (!ini_path || !out) return sp::set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    try {
+        sp_settings* h = new sp_settings();
+        h->s = sp::frontend::load(ini_path);
+        *out = h;
+        return SP_OK;
+    } catch (const std::exception& e) {
+        return sp::set_error(SP_ERR_INVALID_ARG, e.what());
+    }
+}
+
+int sp_settings_request(const sp_settings* h, sp_request* req, sp_particles* parts) {
+    if (!h || !h->s) return sp::set_error(SP_ERR_INVALID_ARG, "NULL settings");
+    const auto& s = *h->s;
+    if (req) *req = s.rq;
+    if (parts) {
+        parts->magnetic_radius = s.radius.data();
+        parts->saturation_magnetisation = s.Ms.data();
+        parts->damping = s.alpha.data();
+        parts->gyromagnetic_ratio = s.gamma.data();
+        parts->temperature = s.T.data();
+        parts->anisotropy_constant = s.K.data();
+        parts->hydrodynamic_radius = s.r_h.data();
+        parts->viscosity = s.visc.data();
+        parts->axes = s.axes.data();
+        parts->initial_state = s.y0.data();
+    }
+    return SP_OK;
+}
+
+const char* sp_settings_savefile(const sp_settings* h) { return (h && h->s) ? h->s->savefile.c_str() : nullptr; }
+
+int sp_settings_write_result(const sp_settings* h, const sp_result* res, const char* path) {
+    if (!h || !h->s || !res || !path) return sp::set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    return sp::frontend::write_result(*h->s, *res, path);
+}
+
+void sp_settings_free(sp_settings* h) {
+    if (!h) return;
+    delete h->s;
+    delete h;
+}
+
+}  // extern "C"

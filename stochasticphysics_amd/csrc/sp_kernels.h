@@ -36,6 +36,8 @@ bool info_bn(uint32_t, KernelInfo*);
 
 // deterministic second-stage reduction; one block per (entry, row)
 void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows);
+// zero n counters (a kernel, so that it can sit in a captured graph)
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n);
 // test hooks
 void launch_philox_test(hipStream_t s, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                         uint64_t count, uint32_t* out);

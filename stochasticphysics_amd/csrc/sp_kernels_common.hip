@@ -64,6 +64,15 @@ void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint3
     k_reduce<<<dim3(max_rows, n_work), dim3(256), 0, s>>>(work);
 }
 
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+void launch_zero_u32(hipStream_t s, uint32_t* p, uint32_t n) {
+    if (!n) return;
+    k_zero_u32<<<dim3((n + 255) / 256), dim3(256), 0, s>>>(p, n);
+}
+
 __global__ void k_philox_test(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                               uint64_t count, uint32_t* out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

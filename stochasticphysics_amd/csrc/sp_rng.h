@@ -31,6 +31,7 @@
 #include <math.h>
 
 #if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
 #define SP_HD __host__ __device__ __forceinline__
 #else
 #define SP_HD inline

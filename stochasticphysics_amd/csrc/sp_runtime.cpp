@@ -7,25 +7,7 @@
 // task per particle and a mutex-protected merge of per-particle trajectories,
 // one host thread drives a HIP stream, particles run 64 to a wave, and the
 // ensemble mean is reduced on the device.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <atomic>
-#include <chrono>
-#include <cmath>
-#include <condition_variable>
-#include <cstring>
-#include <deque>
-#include <map>
-#include <memory>
-#include <mutex>
-#include <numeric>
-#include <thread>
-#include <vector>
-
-#include "sp_host.h"
-#include "sp_kernels.h"
-#include "sp_rng.h"
+#include "sp_engine.h"
 
 namespace sp {
 
@@ -35,386 +17,18 @@ int set_error(int code, const std::string& msg) {
     return code;
 }
 
-#define SP_HIP(x)                                                                                 \
-    do {                                                                                          \
-        hipError_t e_ = (x);                                                                      \
-        if (e_ != hipSuccess) return set_error(SP_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-using Clock = std::chrono::steady_clock;
-static double ms_since(Clock::time_point t0, Clock::time_point t1) {
-    return std::chrono::duration<double, std::milli>(t1 - t0).count();
-}
-
-// ------------------------------------------------------------------ validated request
-struct HostRequest {
-    sp_request rq{};
-    uint32_t dy = 0, dw = 0, dout = 0, kind = 0;
-    bool mixed = false;
-    uint64_t N = 0, n_out = 0, total_steps = 0;
-    uint32_t n_tiles = 0;
-    std::vector<double> params;   // [NPARAM][N]
-    std::vector<double> y0;       // [dy][N]
-    std::vector<double> replay;   // [K][N]
-    double wsum[6] = {0}, wsum_check[6] = {0};
-    uint32_t n_check_tiles = 0;
-};
-
-static uint64_t gcd64(uint64_t a, uint64_t b) { while (b) { const uint64_t t = a % b; a = b; b = t; } return a; }
-static uint64_t lcm64(uint64_t a, uint64_t b) { return a / gcd64(a, b) * b; }
-
-static int validate(const sp_request* rq, const sp_particles* pt) {
-    if (!rq || !pt) return set_error(SP_ERR_INVALID_ARG, "request or particles is NULL");
-    uint32_t dy, dw, dout;
-    if (request_dims(rq->problem, &dy, &dw, &dout)) return set_error(SP_ERR_UNSUPPORTED, "unknown problem type");
-    if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN &&
-        rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT)
-        return set_error(SP_ERR_UNSUPPORTED, "solver not available on the GPU backend");
-    if (rq->field != SP_FIELD_ZERO && rq->field != SP_FIELD_SINUSOIDAL && rq->field != SP_FIELD_LISSAJOUS)
-        return set_error(SP_ERR_UNSUPPORTED, "field type not available on the GPU backend");
-    if (rq->anisotropy != SP_ANISO_UNIAXIAL && rq->anisotropy != SP_ANISO_CUBIC)
-        return set_error(SP_ERR_UNSUPPORTED, "anisotropy type not available on the GPU backend");
-    if (rq->precision != SP_PREC_FP64 && rq->precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
-    if (rq->noise != SP_NOISE_PHILOX && rq->noise != SP_NOISE_REPLAY) return set_error(SP_ERR_INVALID_ARG, "bad noise source");
-    if (rq->n_particles == 0 || rq->n_particles > 0xFFFFFFFFull) return set_error(SP_ERR_INVALID_ARG, "n_particles must be in [1, 2^32)");
-    if (rq->oversampling == 0) return set_error(SP_ERR_INVALID_ARG, "oversampling must be positive");
-    if (rq->n_steps < rq->oversampling) return set_error(SP_ERR_INVALID_ARG, "n_steps must be >= oversampling");
-    if (!(rq->dt > 0.0) || !std::isfinite(rq->dt)) return set_error(SP_ERR_INVALID_ARG, "dt must be positive");
-    if (!pt->magnetic_radius || !pt->saturation_magnetisation || !pt->damping || !pt->gyromagnetic_ratio ||
-        !pt->temperature || !pt->anisotropy_constant || !pt->initial_state)
-        return set_error(SP_ERR_INVALID_ARG, "missing particle property array");
-    if (rq->problem == SP_PROBLEM_BROWN_AND_NEEL && (!pt->hydrodynamic_radius || !pt->viscosity))
-        return set_error(SP_ERR_INVALID_ARG, "BrownAndNeel needs hydrodynamic_radius and viscosity");
-    if (rq->solver == SP_SOLVER_IMPLICIT_MIDPOINT && (rq->newton_max_iter == 0 || !(rq->newton_tol > 0.0)))
-        return set_error(SP_ERR_INVALID_ARG, "Implicit_Midpoint needs Max_Iterations > 0 and Accuracy_Goal > 0");
-    const uint64_t total = (rq->n_steps / rq->oversampling - 1) * rq->oversampling;
-    if (rq->noise == SP_NOISE_REPLAY && (!rq->replay_normals || rq->replay_count < total * dw))
-        return set_error(SP_ERR_INVALID_ARG, "replay needs n_particles*steps*dim_W normals");
-    return SP_OK;
-}
-
-static int prepare_request(const sp_request* rq, const sp_particles* pt, HostRequest& hr) {
-    if (int e = validate(rq, pt)) return e;
-    hr.rq = *rq;
-    request_dims(rq->problem, &hr.dy, &hr.dw, &hr.dout);
-    hr.N = rq->n_particles;
-    hr.n_out = rq->n_steps / rq->oversampling;
-    hr.total_steps = (hr.n_out - 1) * rq->oversampling;
-    hr.n_tiles = (uint32_t)((hr.N + 63) / 64);
-    hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT;
-    hr.kind = kernel_kind(rq->problem, rq->solver, rq->anisotropy, hr.mixed ? 1 : 0);
-    hr.params.assign((size_t)NPARAM * hr.N, 0.0);
-    compute_params(*rq, *pt, hr.params.data(), hr.N);
-    hr.y0.resize((size_t)hr.dy * hr.N);
-    for (uint64_t p = 0; p < hr.N; ++p)
-        for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
-    if (rq->noise == SP_NOISE_REPLAY) {
-        const uint64_t K = hr.total_steps * hr.dw;
-        hr.replay.assign(rq->replay_normals, rq->replay_normals + K * hr.N);
-        hr.rq.replay_count = K;
-    }
-    hr.rq.replay_normals = nullptr;
-    hr.n_check_tiles = std::min<uint32_t>(hr.n_tiles, 4);
-    const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
-    for (uint32_t c = 0; c < hr.dout; ++c) { hr.wsum[c] = 0.0; hr.wsum_check[c] = 0.0; }
-    for (uint64_t p = 0; p < hr.N; ++p) {
-        const double mu = hr.params[P_MU * hr.N + p];
-        for (uint32_t c = 0; c < hr.dout; ++c) {
-            const double w = (hr.dout == 6 && c < 3) ? 1.0 : mu;   // BrownAndNeelRelaxation.h:152-159
-            hr.wsum[c] += w;
-            if (p < n_check) hr.wsum_check[c] += w;
-        }
-    }
-    return SP_OK;
-}
-
-// ------------------------------------------------------------------ device side of one request
-struct ReqDev {
-    double* params = nullptr;
-    double* state = nullptr;
-    double* replay = nullptr;
-    double* out = nullptr;        // [n_out*dout] weighted sums
-    double* check = nullptr;      // [n_out*dout] weighted sums over the first n_check tiles
-    unsigned long long* newton_fail = nullptr;
-    size_t bytes = 0;
-};
-
-static size_t request_bytes(const HostRequest& hr) {
-    return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 + 2 * hr.n_out * hr.dout * 8 + 64;
-}
-
-static void free_dev(ReqDev& d) {
-    if (d.params) hipFree(d.params);
-    if (d.state) hipFree(d.state);
-    if (d.replay) hipFree(d.replay);
-    if (d.out) hipFree(d.out);
-    if (d.check) hipFree(d.check);
-    if (d.newton_fail) hipFree(d.newton_fail);
-    d = ReqDev{};
-}
-
-static int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
-    auto fail = [&](const char* what, hipError_t e) {
-        free_dev(d);
-        return set_error(e == hipErrorOutOfMemory ? SP_ERR_OUT_OF_MEMORY : SP_ERR_DEVICE,
-                         std::string("allocating ") + what + ": " + hipGetErrorString(e));
-    };
-    hipError_t e;
-    if ((e = hipMalloc(&d.params, hr.params.size() * 8)) != hipSuccess) return fail("params", e);
-    if ((e = hipMalloc(&d.state, hr.y0.size() * 8)) != hipSuccess) return fail("state", e);
-    if (!hr.replay.empty() && (e = hipMalloc(&d.replay, hr.replay.size() * 8)) != hipSuccess) return fail("replay", e);
-    if ((e = hipMalloc(&d.out, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("out", e);
-    if ((e = hipMalloc(&d.check, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("check", e);
-    if ((e = hipMalloc(&d.newton_fail, 8)) != hipSuccess) return fail("counter", e);
-    d.bytes = request_bytes(hr);
-    SP_HIP(hipMemcpyAsync(d.params, hr.params.data(), hr.params.size() * 8, hipMemcpyHostToDevice, s));
-    SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
-    if (!hr.replay.empty()) SP_HIP(hipMemcpyAsync(d.replay, hr.replay.data(), hr.replay.size() * 8, hipMemcpyHostToDevice, s));
-    SP_HIP(hipMemsetAsync(d.out, 0, hr.n_out * hr.dout * 8, s));
-    SP_HIP(hipMemsetAsync(d.check, 0, hr.n_out * hr.dout * 8, s));
-    SP_HIP(hipMemsetAsync(d.newton_fail, 0, 8, s));
-    return SP_OK;
-}
-
-static DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
-    const sp_request& r = hr.rq;
-    DevRequest dr{};
-    dr.dt = r.dt;
-    dr.sqdt = std::sqrt(r.dt);
-    const bool rot = (r.flags & SP_FLAG_COORD_TRANSFORM) != 0;
-    dr.tmin = rot ? r.min_angle : -1.0;
-    dr.tmax = rot ? (3.14159265358979323846 - r.min_angle) : 1e300;
-    dr.newton_tol = r.newton_tol;
-    for (int i = 0; i < 3; ++i) {
-        dr.f_off[i] = r.field_offset[i];
-        dr.f_amp[i] = r.field_amplitude[i];
-        dr.f_ph[i] = r.field_phase[i];
-        dr.f_w[i] = 2.0 * 3.14159265358979323846 * r.field_freq[i];   // SinusoidalField.h:39, LissajousField.h:50-64
-    }
-    if (r.field == SP_FIELD_SINUSOIDAL)
-        for (int i = 1; i < 3; ++i) { dr.f_w[i] = dr.f_w[0]; dr.f_ph[i] = dr.f_ph[0]; }
-    dr.n_particles = hr.N;
-    dr.seed = r.seed;
-    dr.replay_count = hr.replay.empty() ? 0 : r.replay_count;
-    dr.oversampling = (uint32_t)r.oversampling;
-    dr.field_type = (uint32_t)r.field;
-    dr.flags = ((r.flags & SP_FLAG_BN_SIMPLE_MODEL) ? RQ_BN_SIMPLE : 0u) | (hr.replay.empty() ? 0u : RQ_REPLAY);
-    dr.newton_max_iter = r.newton_max_iter;
-    dr.n_tiles = hr.n_tiles;
-    dr.kind = hr.kind;
-    dr.params = d.params;
-    dr.state = d.state;
-    dr.replay = d.replay;
-    dr.newton_fail = d.newton_fail;
-    return dr;
-}
-
-// ------------------------------------------------------------------ results
-struct Outputs {
-    std::vector<double> time, field, mean, weight, final_state, check_mean;
-    double device_ms = 0.0, latency_ms = 0.0, particle_steps = 0.0, check_error = -1.0;
-    uint32_t used_fallback = 0, precision_used = SP_PREC_FP64;
-    uint64_t newton_failures = 0;
-};
-
-static int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t s, bool keep_final, bool want_check, Outputs& o) {
-    const uint64_t M = hr.n_out * hr.dout;
-    std::vector<double> sums(M), csum;
-    SP_HIP(hipMemcpyAsync(sums.data(), d.out, M * 8, hipMemcpyDeviceToHost, s));
-    if (want_check) { csum.resize(M); SP_HIP(hipMemcpyAsync(csum.data(), d.check, M * 8, hipMemcpyDeviceToHost, s)); }
-    unsigned long long nf = 0;
-    SP_HIP(hipMemcpyAsync(&nf, d.newton_fail, 8, hipMemcpyDeviceToHost, s));
-    if (keep_final) {
-        o.final_state.resize(hr.N * hr.dy);
-        std::vector<double> soa(hr.N * hr.dy);
-        SP_HIP(hipMemcpyAsync(soa.data(), d.state, soa.size() * 8, hipMemcpyDeviceToHost, s));
-        SP_HIP(hipStreamSynchronize(s));
-        for (uint64_t p = 0; p < hr.N; ++p)
-            for (uint32_t i = 0; i < hr.dy; ++i) o.final_state[p * hr.dy + i] = soa[i * hr.N + p];
-    }
-    SP_HIP(hipStreamSynchronize(s));
-    o.newton_failures = nf;
-    // MeanSimulationResult::normalize (MeanSimulationResult.h:63-75)
-    bool zero = true;
-    for (uint32_t c = 0; c < hr.dout; ++c) if (hr.wsum[c] != 0.0) zero = false;
-    o.mean.assign(M, 0.0);
-    for (uint64_t k = 0; k < hr.n_out; ++k)
-        for (uint32_t c = 0; c < hr.dout; ++c) o.mean[k * hr.dout + c] = zero ? sums[k * hr.dout + c] : sums[k * hr.dout + c] / hr.wsum[c];
-    if (want_check) {
-        o.check_mean.assign(M, 0.0);
-        for (uint64_t k = 0; k < hr.n_out; ++k)
-            for (uint32_t c = 0; c < hr.dout; ++c)
-                o.check_mean[k * hr.dout + c] = hr.wsum_check[c] != 0.0 ? csum[k * hr.dout + c] / hr.wsum_check[c] : 0.0;
-    }
-    o.weight.assign(hr.wsum, hr.wsum + hr.dout);
-    o.time.resize(hr.n_out);
-    o.field.resize(hr.n_out * 3);
-    time_and_field(hr.rq, o.time.data(), o.field.data());
-    o.particle_steps = (double)hr.N * (double)hr.total_steps;
-    o.precision_used = hr.mixed ? SP_PREC_MIXED : SP_PREC_FP64;
-    return SP_OK;
-}
-
-static void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
-    r->n_out = hr.n_out;
-    r->dim_out = hr.dout;
-    r->dim_y = hr.dy;
-    if (r->time) std::copy(o.time.begin(), o.time.end(), r->time);
-    if (r->field) std::copy(o.field.begin(), o.field.end(), r->field);
-    if (r->mean) std::copy(o.mean.begin(), o.mean.end(), r->mean);
-    if (r->weight) std::copy(o.weight.begin(), o.weight.end(), r->weight);
-    if (r->final_state && !o.final_state.empty()) std::copy(o.final_state.begin(), o.final_state.end(), r->final_state);
-    r->device_ms = o.device_ms;
-    r->latency_ms = o.latency_ms;
-    r->particle_steps = o.particle_steps;
-    r->check_error = o.check_error;
-    r->used_fallback = o.used_fallback;
-    r->precision_used = o.precision_used;
-    r->newton_failures = o.newton_failures;
-}
-
-// ------------------------------------------------------------------ context
-struct Job {
-    uint64_t id = 0;
-    HostRequest hr;
-    Outputs out;
-    int status = SP_ERR_NOT_READY;       // SP_ERR_NOT_READY while pending/running
-    bool cancel = false;
-    bool keep_final = false;
-    Clock::time_point t_submit;
-    double device_ms_acc = 0.0;
-};
-
-struct Slot {                 // an admitted request (or the FP64 shadow of a mixed one)
-    std::shared_ptr<Job> job;
-    HostRequest* hr = nullptr;
-    HostRequest shadow_req;   // storage when this slot is a shadow
-    ReqDev dev;
-    DevRequest dr{};
-    uint64_t next_step = 0;
-    uint64_t chunk = 0;       // steps per iteration
-    bool first = true;
-    bool done = false;
-    bool is_shadow = false;
-    Slot* shadow = nullptr;   // FP64 check slot of a mixed request
-    Slot* parent = nullptr;
-    double work_share = 0.0;  // particle-steps in the current iteration
-};
-
-struct sp_ctx_impl;
-}  // namespace sp
-
-struct sp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipDeviceProp_t prop{};
-    std::mutex mu;                          // protects everything below
-    std::map<uint64_t, sp_tuning> tuning;   // by (problem, solver, precision)
-    uint64_t mem_budget = 0;
-    uint32_t max_active = 64;
-    // server
-    std::thread worker;
-    bool worker_started = false;
-    bool stop = false;
-    std::condition_variable cv;             // wakes the worker
-    std::condition_variable done_cv;        // wakes waiters
-    std::deque<std::shared_ptr<sp::Job>> pending;
-    std::map<uint64_t, std::shared_ptr<sp::Job>> jobs;
-    uint64_t next_id = 1;
-    std::vector<std::unique_ptr<sp::Slot>> active;
-    uint64_t mem_used = 0;
-    // statistics
-    double stat_completed = 0, stat_iterations = 0, stat_busy_ms = 0, stat_particle_steps = 0, stat_fallbacks = 0;
-    sp::Clock::time_point t_start = sp::Clock::now();
-    // scratch owned by the worker (or by a synchronous call holding run_mu)
-    std::mutex run_mu;                      // serialises device use between sp_run and the worker
-    double* partial = nullptr;
-    size_t partial_bytes = 0;
-    void* table = nullptr;                  // device copy of request / work / reduce tables
-    size_t table_bytes = 0;
-    void* pinned = nullptr;                 // pinned host staging of the tables
-    size_t pinned_bytes = 0;
-    uint32_t* counters = nullptr;           // per-launch work counters
-    size_t n_counters = 0;
-    // last synchronous run statistics
-    double last_stats[8] = {0};
-};
-
-namespace sp {
-
-static uint64_t tuning_key(int problem, int solver, int precision) { return ((uint64_t)problem << 16) | ((uint64_t)solver << 8) | (uint64_t)precision; }
-
-static sp_tuning default_tuning(const sp_ctx* ctx, const HostRequest& hr) {
-    sp_tuning t{};
-    t.block_size = 256;
-    t.particles_per_lane = 1;
-    t.chunk_steps = 4000;
-    t.use_graph = 1;
-    t.persistent_waves_per_simd = 2;
-    (void)ctx; (void)hr;
-    return t;
-}
-
-static sp_tuning get_tuning(sp_ctx* ctx, const HostRequest& hr) {
-    auto it = ctx->tuning.find(tuning_key(hr.rq.problem, hr.rq.solver, hr.mixed ? SP_PREC_MIXED : SP_PREC_FP64));
-    return it != ctx->tuning.end() ? it->second : default_tuning(ctx, hr);
-}
-
-static int ensure_partial(sp_ctx* ctx, size_t bytes) {
-    if (bytes <= ctx->partial_bytes) return SP_OK;
-    if (ctx->partial) hipFree(ctx->partial);
-    ctx->partial = nullptr;
-    ctx->partial_bytes = 0;
-    hipError_t e = hipMalloc(&ctx->partial, bytes);
-    if (e != hipSuccess) return set_error(SP_ERR_OUT_OF_MEMORY, std::string("partial buffer: ") + hipGetErrorString(e));
-    ctx->partial_bytes = bytes;
-    return SP_OK;
-}
-static int ensure_table(sp_ctx* ctx, size_t bytes) {
-    if (bytes > ctx->table_bytes) {
-        if (ctx->table) hipFree(ctx->table);
-        ctx->table = nullptr;
-        ctx->table_bytes = 0;
-        SP_HIP(hipMalloc(&ctx->table, bytes));
-        ctx->table_bytes = bytes;
-    }
-    if (bytes > ctx->pinned_bytes) {
-        if (ctx->pinned) hipHostFree(ctx->pinned);
-        ctx->pinned = nullptr;
-        ctx->pinned_bytes = 0;
-        SP_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
-        ctx->pinned_bytes = bytes;
-    }
-    return SP_OK;
-}
-static int ensure_counters(sp_ctx* ctx, size_t n) {
-    if (n <= ctx->n_counters) return SP_OK;
-    if (ctx->counters) hipFree(ctx->counters);
-    ctx->counters = nullptr;
-    ctx->n_counters = 0;
-    SP_HIP(hipMalloc(&ctx->counters, n * sizeof(uint32_t)));
-    ctx->n_counters = n;
-    return SP_OK;
-}
 
 // ------------------------------------------------------------------ plain path
 // One request, one kernel launch per chunk over all its tiles, then the reduction.
 // A Prepared request keeps its parameters and initial state resident in HBM, so
 // it can be executed repeatedly without host-to-device traffic.
-struct Prepared {
-    HostRequest hr;
-    ReqDev d;
-    double* state0 = nullptr;     // device copy of the initial state
-    bool keep_final = false;
-};
-
-static void free_prepared(Prepared& pr) {
+void free_prepared(Prepared& pr) {
     free_dev(pr.d);
     if (pr.state0) hipFree(pr.state0);
     pr.state0 = nullptr;
 }
 
-static int prepare_plain(sp_ctx* ctx, Prepared& pr) {
+int prepare_plain(sp_ctx* ctx, Prepared& pr) {
     SP_HIP(hipSetDevice(ctx->device));
     if (int e = alloc_dev(pr.hr, pr.d, ctx->stream)) return e;
     hipError_t e = hipMalloc(&pr.state0, pr.hr.y0.size() * 8);
@@ -424,7 +38,7 @@ static int prepare_plain(sp_ctx* ctx, Prepared& pr) {
     return SP_OK;
 }
 
-static int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
+int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     std::lock_guard<std::mutex> lk(ctx->run_mu);
     HostRequest& hr = pr.hr;
     ReqDev& d = pr.d;
@@ -500,7 +114,7 @@ static int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning&
     return SP_OK;
 }
 
-static int run_plain(sp_ctx* ctx, HostRequest& hr, Outputs& o, bool keep_final, const sp_tuning& tun) {
+int run_plain(sp_ctx* ctx, HostRequest& hr, Outputs& o, bool keep_final, const sp_tuning& tun) {
     Prepared pr;
     pr.hr = std::move(hr);
     pr.keep_final = keep_final;
@@ -694,6 +308,7 @@ int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first
 
 extern "C" void sp_ctx_destroy(sp_ctx* ctx) {
     if (!ctx) return;
+    stop_worker(ctx);
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     if (ctx->partial) hipFree(ctx->partial);

@@ -1,0 +1,711 @@
+// sp_server.cpp -- batched execution and the continuous-batching request server.
+//
+// Both run the same iteration: every admitted request ("slot") advances by one
+// chunk of steps; all 64-particle tiles of all slots of one kernel class form a
+// work table that one persistent launch drains (waves pull tiles from an atomic
+// counter, longest classes first), then one reduction launch folds every
+// slot's per-wave partial sums into its output rows.  Particle state stays in
+// HBM between chunks; Philox noise depends only on (seed, particle, step), so a
+// request's result does not depend on what it shares the device with.
+//
+//  * sp_run_batch: the whole schedule of a static batch is planned up front and
+//    captured in one hipGraph (chunk launches + reductions), launched once.
+//  * sp_submit/sp_wait/sp_result_get: a worker thread admits requests by device
+//    memory footprint as slots free up and runs iterations back to back; new
+//    requests join at the next iteration boundary.
+//  * mixed precision: each mixed request gets an FP64 shadow over its first 256
+//    particles on the same Philox stream; the request's own mean over those
+//    particles is compared with it, and a failed check re-runs it in FP64.
+#include "sp_engine.h"
+
+namespace sp {
+
+// relative cost of one particle-step per kernel class (for longest-first ordering)
+static double class_cost(const HostRequest& hr) {
+    double c = hr.rq.problem == SP_PROBLEM_BROWN_AND_NEEL ? 3.0 : (hr.rq.problem == SP_PROBLEM_NEEL_SPHERICAL ? 1.6 : 1.0);
+    if (hr.rq.solver == SP_SOLVER_HEUN_STRONG) c *= 1.5;
+    if (hr.rq.solver == SP_SOLVER_IMPLICIT_MIDPOINT) c *= 6.0;
+    if (hr.mixed) c *= 0.6;
+    return c;
+}
+
+struct IterPlan {
+    std::vector<uint32_t> kinds;                      // one persistent launch per kind
+    std::vector<uint32_t> kind_begin, kind_count;     // ranges into work
+    std::vector<TileWork> work;
+    std::vector<ReduceWork> reduce;
+    uint32_t max_rows = 0;
+    size_t partial_doubles = 0;
+    std::vector<Slot*> finishing;                     // slots whose last chunk is in this iteration
+    double particle_steps = 0.0;
+};
+
+// Plan one iteration for the given slots (indices into the DevRequest table = position in `slots`).
+// Partial offsets are relative (pointer = base + offset, patched by the caller).
+static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
+    P = IterPlan{};
+    struct Item { double cost; uint32_t kind; uint32_t slot; };
+    std::vector<Item> items;
+    for (uint32_t i = 0; i < slots.size(); ++i) {
+        Slot* s = slots[i];
+        if (s->done) continue;
+        items.push_back({class_cost(*s->hr) * (double)std::min<uint64_t>(s->chunk, s->hr->total_steps - s->next_step + 1), s->hr->kind, i});
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) { return a.kind < b.kind || (a.kind == b.kind && a.cost > b.cost); });
+    size_t off = 0;
+    for (size_t k = 0; k < items.size(); ++k) {
+        Slot* s = slots[items[k].slot];
+        const HostRequest& hr = *s->hr;
+        if (P.kinds.empty() || P.kinds.back() != hr.kind) {
+            P.kinds.push_back(hr.kind);
+            P.kind_begin.push_back((uint32_t)P.work.size());
+            P.kind_count.push_back(0);
+        }
+        const uint64_t OS = hr.rq.oversampling;
+        const uint64_t sb = s->next_step;
+        const uint64_t se = std::min(sb + s->chunk, hr.total_steps);
+        const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
+        const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
+        for (uint32_t t = 0; t < hr.n_tiles; ++t) {
+            TileWork tw{};
+            tw.req = items[k].slot;
+            tw.tile = t;
+            tw.step_begin = sb;
+            tw.step_end = se;
+            tw.partial = (double*)(uintptr_t)(off * 8);   // relative, patched later
+            tw.pstride = hr.n_tiles;
+            tw.pslot = t;
+            P.work.push_back(tw);
+        }
+        P.kind_count.back() += hr.n_tiles;
+        ReduceWork rw{};
+        rw.partial = (const double*)(uintptr_t)(off * 8);
+        rw.pstride = hr.n_tiles;
+        rw.n_check = hr.n_check_tiles;
+        rw.n_rows = (uint32_t)(rows * hr.dout);
+        rw.row_base = (uint32_t)(first_window * hr.dout);
+        rw.out = s->dev.out;
+        rw.check_out = s->dev.check;
+        P.reduce.push_back(rw);
+        P.max_rows = std::max(P.max_rows, rw.n_rows);
+        off += (size_t)rows * hr.dout * hr.n_tiles;
+        P.particle_steps += (double)hr.N * (double)(se - sb);
+        s->work_share = (double)hr.N * (double)(se - sb);
+        s->next_step = se;
+        if (se >= hr.total_steps) P.finishing.push_back(s);
+    }
+    P.partial_doubles = off;
+}
+
+// persistent grid for one kind
+static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning& t) {
+    const uint32_t wpb = std::max<uint32_t>(1, t.block_size / 64);
+    const uint32_t per_simd = std::max<uint32_t>(1, t.persistent_waves_per_simd);
+    const uint32_t resident = (uint32_t)ctx->prop.multiProcessorCount * 4u * per_simd;   // waves
+    const uint32_t waves = std::min(n_work, resident);
+    return dim3(std::max<uint32_t>(1, (waves + wpb - 1) / wpb));
+}
+
+// Issue the launches of one iteration whose tables are already on the device.
+static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const TileWork* d_work,
+                            const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun) {
+    if (!P.kinds.empty()) launch_zero_u32(s, d_counters, (uint32_t)P.kinds.size());
+    for (size_t k = 0; k < P.kinds.size(); ++k) {
+        const sp_tuning& t = tun[k];
+        const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(1024, t.block_size / 64 * 64));
+        const dim3 grid = persistent_grid(ctx, P.kind_count[k], t);
+        if (!launch_batch(P.kinds[k], grid, dim3(block), s, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], d_counters + k))
+            return set_error(SP_ERR_UNSUPPORTED, "no batch kernel for a requested class");
+    }
+    if (!P.reduce.empty()) launch_reduce(s, d_reduce, (uint32_t)P.reduce.size(), P.max_rows);
+    return SP_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Device tables of one iteration: [DevRequest x nreq][TileWork x nwork][ReduceWork x nred]
+struct TableLayout {
+    size_t req_off, work_off, red_off, bytes;
+};
+static TableLayout layout_for(size_t nreq, size_t nwork, size_t nred) {
+    TableLayout L;
+    L.req_off = 0;
+    L.work_off = align256(nreq * sizeof(DevRequest));
+    L.red_off = L.work_off + align256(nwork * sizeof(TileWork));
+    L.bytes = L.red_off + align256(nred * sizeof(ReduceWork));
+    return L;
+}
+static void fill_tables(char* host, const TableLayout& L, const std::vector<DevRequest>& reqs, const IterPlan& P,
+                        double* partial_base) {
+    std::memcpy(host + L.req_off, reqs.data(), reqs.size() * sizeof(DevRequest));
+    TileWork* w = (TileWork*)(host + L.work_off);
+    for (size_t i = 0; i < P.work.size(); ++i) {
+        w[i] = P.work[i];
+        w[i].partial = (double*)((char*)partial_base + (uintptr_t)P.work[i].partial);
+    }
+    ReduceWork* r = (ReduceWork*)(host + L.red_off);
+    for (size_t i = 0; i < P.reduce.size(); ++i) {
+        r[i] = P.reduce[i];
+        r[i].partial = (const double*)((const char*)partial_base + (uintptr_t)P.reduce[i].partial);
+    }
+}
+
+static std::vector<sp_tuning> tunings_for(sp_ctx* ctx, const IterPlan& P, const std::vector<Slot*>& slots) {
+    std::vector<sp_tuning> t(P.kinds.size());
+    for (size_t k = 0; k < P.kinds.size(); ++k) {
+        const TileWork& w = P.work[P.kind_begin[k]];
+        t[k] = get_tuning(ctx, *slots[w.req]->hr);
+    }
+    return t;
+}
+
+static void make_shadow(const HostRequest& parent, HostRequest& sh) {
+    sh = HostRequest{};
+    sh.rq = parent.rq;
+    sh.rq.precision = SP_PREC_FP64;
+    sh.rq.flags |= SP_FLAG_SKIP_CHECK;
+    sh.dy = parent.dy; sh.dw = parent.dw; sh.dout = parent.dout;
+    sh.N = std::min<uint64_t>(parent.N, 64ull * parent.n_check_tiles);
+    sh.rq.n_particles = sh.N;
+    sh.n_out = parent.n_out;
+    sh.total_steps = parent.total_steps;
+    sh.n_tiles = (uint32_t)((sh.N + 63) / 64);
+    sh.mixed = false;
+    sh.kind = kernel_kind(parent.rq.problem, parent.rq.solver, parent.rq.anisotropy, 0);
+    sh.params.resize((size_t)NPARAM * sh.N);
+    for (uint32_t k = 0; k < NPARAM; ++k)
+        for (uint64_t p = 0; p < sh.N; ++p) sh.params[k * sh.N + p] = parent.params[k * parent.N + p];
+    sh.y0.resize((size_t)sh.dy * sh.N);
+    for (uint32_t i = 0; i < sh.dy; ++i)
+        for (uint64_t p = 0; p < sh.N; ++p) sh.y0[i * sh.N + p] = parent.y0[i * parent.N + p];
+    sh.n_check_tiles = 0;
+    for (uint32_t c = 0; c < sh.dout; ++c) sh.wsum[c] = parent.wsum_check[c];
+}
+
+static double check_error_of(const Outputs& parent, const Outputs& shadow) {
+    double e = 0.0;
+    for (size_t i = 0; i < parent.check_mean.size() && i < shadow.mean.size(); ++i)
+        e = std::max(e, std::fabs(parent.check_mean[i] - shadow.mean[i]));
+    return e;
+}
+
+static uint64_t chunk_for(const HostRequest& hr, const sp_tuning& t) {
+    const uint64_t align = lcm64(hr.rq.oversampling, 4);
+    const uint64_t c = std::max<uint64_t>(align, (uint64_t)std::max<uint32_t>(t.chunk_steps, 1) / align * align);
+    return c;
+}
+
+static bool wants_shadow(const HostRequest& hr) { return hr.mixed && !(hr.rq.flags & SP_FLAG_SKIP_CHECK); }
+
+// ------------------------------------------------------------------ static batch (graph)
+static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::vector<Outputs>& outs, bool keep_final, bool use_graph) {
+    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    hipStream_t s = ctx->stream;
+    SP_HIP(hipSetDevice(ctx->device));
+    const size_t n = reqs.size();
+    std::vector<std::unique_ptr<Slot>> slots;
+    struct Cleanup { std::vector<std::unique_ptr<Slot>>* v; ~Cleanup() { for (auto& x : *v) free_dev(x->dev); } } cleanup{&slots};
+    std::vector<Slot*> order;
+    for (size_t i = 0; i < n; ++i) {
+        auto sl = std::make_unique<Slot>();
+        sl->hr = &reqs[i];
+        if (int e = alloc_dev(reqs[i], sl->dev, s)) return e;
+        sl->chunk = chunk_for(reqs[i], get_tuning(ctx, reqs[i]));
+        slots.push_back(std::move(sl));
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (!wants_shadow(reqs[i])) continue;
+        auto sh = std::make_unique<Slot>();
+        make_shadow(reqs[i], sh->shadow_req);
+        sh->hr = &sh->shadow_req;
+        sh->is_shadow = true;
+        sh->parent = slots[i].get();
+        slots[i]->shadow = sh.get();
+        if (int e = alloc_dev(sh->shadow_req, sh->dev, s)) return e;
+        sh->chunk = slots[i]->chunk;
+        slots.push_back(std::move(sh));
+    }
+    for (auto& x : slots) order.push_back(x.get());
+    std::vector<DevRequest> dreqs;
+    for (Slot* x : order) dreqs.push_back(make_dev_request(*x->hr, x->dev));
+    // plan every iteration
+    std::vector<IterPlan> plans;
+    for (;;) {
+        IterPlan P;
+        plan_iteration(order, P);
+        if (P.work.empty()) break;
+        for (Slot* f : P.finishing) f->done = true;
+        plans.push_back(std::move(P));
+    }
+    size_t max_partial = 0;
+    std::vector<TableLayout> lays;
+    size_t table_total = 0;
+    for (auto& P : plans) {
+        max_partial = std::max(max_partial, P.partial_doubles);
+        lays.push_back(layout_for(dreqs.size(), P.work.size(), P.reduce.size()));
+        table_total += lays.back().bytes;
+    }
+    if (int e = ensure_partial(ctx, std::max<size_t>(8, max_partial * 8))) return e;
+    if (int e = ensure_table(ctx, std::max<size_t>(256, table_total))) return e;
+    if (int e = ensure_counters(ctx, 16)) return e;
+    size_t off = 0;
+    std::vector<size_t> offs;
+    for (size_t i = 0; i < plans.size(); ++i) {
+        fill_tables((char*)ctx->pinned + off, lays[i], dreqs, plans[i], ctx->partial);
+        offs.push_back(off);
+        off += lays[i].bytes;
+    }
+    if (off) SP_HIP(hipMemcpyAsync(ctx->table, ctx->pinned, off, hipMemcpyHostToDevice, s));
+    std::vector<std::vector<sp_tuning>> tuns;
+    for (auto& P : plans) tuns.push_back(tunings_for(ctx, P, order));
+    auto record_all = [&](hipStream_t st) -> int {
+        for (size_t i = 0; i < plans.size(); ++i) {
+            char* base = (char*)ctx->table + offs[i];
+            if (int e = record_iteration(ctx, st, plans[i], (const DevRequest*)(base + lays[i].req_off),
+                                         (const TileWork*)(base + lays[i].work_off), (const ReduceWork*)(base + lays[i].red_off),
+                                         ctx->counters, tuns[i]))
+                return e;
+        }
+        return SP_OK;
+    };
+    hipEvent_t e0, e1;
+    SP_HIP(hipEventCreate(&e0));
+    SP_HIP(hipEventCreate(&e1));
+    const auto t0 = Clock::now();
+    SP_HIP(hipStreamSynchronize(s));
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    if (use_graph && !plans.empty()) {
+        SP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = record_all(s);
+        hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+        SP_HIP(ce);
+        SP_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        SP_HIP(hipEventRecord(e0, s));
+        SP_HIP(hipGraphLaunch(exec, s));
+        SP_HIP(hipEventRecord(e1, s));
+    } else {
+        SP_HIP(hipEventRecord(e0, s));
+        if (int e = record_all(s)) return e;
+        SP_HIP(hipEventRecord(e1, s));
+    }
+    SP_HIP(hipStreamSynchronize(s));
+    float dev_ms = 0.f;
+    SP_HIP(hipEventElapsedTime(&dev_ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    const double wall = ms_since(t0, Clock::now());
+    ctx->last_stats[0] = (double)plans.size();
+    ctx->last_stats[1] = dev_ms;
+    ctx->last_stats[2] = 0.0;
+    ctx->last_stats[3] = wall;
+    double ps = 0.0;
+    for (auto& P : plans) ps += P.particle_steps;
+    ctx->last_stats[4] = ps;
+    outs.assign(n, Outputs{});
+    double total_ps = 0.0;
+    for (size_t i = 0; i < n; ++i) total_ps += (double)reqs[i].N * (double)reqs[i].total_steps;
+    for (size_t i = 0; i < n; ++i) {
+        Slot* sl = slots[i].get();
+        if (int e = download_outputs(reqs[i], sl->dev, s, keep_final, sl->shadow != nullptr, outs[i])) return e;
+        outs[i].device_ms = dev_ms * ((double)reqs[i].N * (double)reqs[i].total_steps) / std::max(1.0, total_ps);
+        outs[i].latency_ms = wall;
+        if (sl->shadow) {
+            Outputs so;
+            if (int e = download_outputs(*sl->shadow->hr, sl->shadow->dev, s, false, false, so)) return e;
+            outs[i].check_error = check_error_of(outs[i], so);
+        }
+    }
+    return SP_OK;
+}
+
+// ------------------------------------------------------------------ server worker
+static void finish_job(sp_ctx* ctx, Slot* sl) {
+    // called by the worker without ctx->mu; publishes under the lock
+    std::shared_ptr<Job> job = sl->job;
+    Outputs o;
+    int rc = download_outputs(*sl->hr, sl->dev, ctx->stream, job->keep_final, sl->shadow != nullptr, o);
+    bool requeue = false;
+    if (rc == SP_OK && sl->shadow) {
+        Outputs so;
+        rc = download_outputs(*sl->shadow->hr, sl->shadow->dev, ctx->stream, false, false, so);
+        if (rc == SP_OK) {
+            o.check_error = check_error_of(o, so);
+            const double tol = sl->hr->rq.check_tol > 0.0 ? sl->hr->rq.check_tol : 1e-3;
+            if (o.check_error > tol && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
+        }
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (requeue) {
+        // per-request FP64 fallback: run the request again in FP64 ahead of the queue
+        job->out.check_error = o.check_error;
+        job->out.used_fallback = 1;
+        job->hr.rq.precision = SP_PREC_FP64;
+        job->hr.mixed = false;
+        job->hr.kind = kernel_kind(job->hr.rq.problem, job->hr.rq.solver, job->hr.rq.anisotropy, 0);
+        ctx->pending.push_front(job);
+        ctx->stat_fallbacks += 1;
+        return;
+    }
+    if (rc == SP_OK) {
+        const double fallback_err = job->out.check_error;
+        const uint32_t fb = job->out.used_fallback;
+        job->out = std::move(o);
+        if (fb) { job->out.used_fallback = 1; job->out.check_error = fallback_err; }
+        job->out.device_ms = job->device_ms_acc;
+        job->out.latency_ms = ms_since(job->t_submit, Clock::now());
+        job->status = SP_OK;
+        ctx->stat_completed += 1;
+        ctx->stat_particle_steps += job->out.particle_steps;
+    } else {
+        job->status = rc;
+    }
+    ctx->done_cv.notify_all();
+}
+
+static void worker_main(sp_ctx* ctx) {
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    std::vector<std::unique_ptr<Slot>> active;
+    for (;;) {
+        std::vector<std::shared_ptr<Job>> admitted;
+        {
+            std::unique_lock<std::mutex> lk(ctx->mu);
+            ctx->cv.wait(lk, [&] { return ctx->stop || !ctx->pending.empty() || !active.empty(); });
+            if (ctx->stop) {
+                for (auto& j : ctx->pending) { j->status = SP_ERR_CANCELLED; }
+                ctx->pending.clear();
+                for (auto& a : active) if (!a->is_shadow && a->job) a->job->status = SP_ERR_CANCELLED;
+                ctx->done_cv.notify_all();
+                break;
+            }
+            // drop cancelled active requests (the active list is the worker's own)
+            for (auto& a : active) {
+                if (!a->is_shadow && a->job && a->job->cancel && !a->done) {
+                    a->done = true;
+                    if (a->shadow) a->shadow->done = true;
+                    a->job->status = SP_ERR_CANCELLED;
+                    ctx->done_cv.notify_all();
+                }
+            }
+            // admission by device-memory footprint (state + parameters + outputs, shadow included)
+            while (!ctx->pending.empty()) {
+                auto& j = ctx->pending.front();
+                if (j->cancel) { j->status = SP_ERR_CANCELLED; ctx->pending.pop_front(); ctx->done_cv.notify_all(); continue; }
+                size_t need = request_bytes(j->hr) + (wants_shadow(j->hr) ? request_bytes(j->hr) / std::max<uint64_t>(1, j->hr.n_tiles) * 4 : 0);
+                size_t n_active = admitted.size();
+                for (auto& a : active) if (!a->is_shadow && !a->done) ++n_active;
+                if (n_active > 0 && (ctx->mem_used + need > ctx->mem_budget || n_active >= ctx->max_active)) break;
+                admitted.push_back(j);
+                ctx->mem_used += need;
+                ctx->pending.pop_front();
+            }
+        }
+        std::unique_lock<std::mutex> device_lock(ctx->run_mu);   // the worker owns the stream and scratch from here
+        for (auto& j : admitted) {
+            auto sl = std::make_unique<Slot>();
+            sl->job = j;
+            sl->hr = &j->hr;
+            int rc = alloc_dev(j->hr, sl->dev, s);
+            sp_tuning t;
+            {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                t = get_tuning(ctx, j->hr);
+            }
+            sl->chunk = chunk_for(j->hr, t);
+            if (rc == SP_OK && wants_shadow(j->hr)) {
+                auto sh = std::make_unique<Slot>();
+                make_shadow(j->hr, sh->shadow_req);
+                sh->hr = &sh->shadow_req;
+                sh->is_shadow = true;
+                sh->parent = sl.get();
+                sh->chunk = sl->chunk;
+                rc = alloc_dev(sh->shadow_req, sh->dev, s);
+                sl->shadow = sh.get();
+                if (rc == SP_OK) active.push_back(std::move(sh));
+            }
+            if (rc != SP_OK) {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                j->status = rc;
+                ctx->done_cv.notify_all();
+                free_dev(sl->dev);
+                continue;
+            }
+            active.push_back(std::move(sl));
+        }
+        // one iteration over all active slots
+        std::vector<Slot*> slots;
+        for (auto& a : active) slots.push_back(a.get());
+        IterPlan P;
+        plan_iteration(slots, P);
+        if (!P.work.empty()) {
+            std::vector<DevRequest> dreqs;
+            for (Slot* x : slots) dreqs.push_back(make_dev_request(*x->hr, x->dev));
+            const TableLayout L = layout_for(dreqs.size(), P.work.size(), P.reduce.size());
+            std::vector<sp_tuning> tun;
+            {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                tun = tunings_for(ctx, P, slots);
+            }
+            int rc = ensure_partial(ctx, std::max<size_t>(8, P.partial_doubles * 8));
+            if (rc == SP_OK) rc = ensure_table(ctx, L.bytes);
+            if (rc == SP_OK) rc = ensure_counters(ctx, 16);
+            if (rc == SP_OK) {
+                fill_tables((char*)ctx->pinned, L, dreqs, P, ctx->partial);
+                hipMemcpyAsync(ctx->table, ctx->pinned, L.bytes, hipMemcpyHostToDevice, s);
+                hipEventRecord(e0, s);
+                char* base = (char*)ctx->table;
+                rc = record_iteration(ctx, s, P, (const DevRequest*)(base + L.req_off), (const TileWork*)(base + L.work_off),
+                                      (const ReduceWork*)(base + L.red_off), ctx->counters, tun);
+                hipEventRecord(e1, s);
+                if (hipStreamSynchronize(s) != hipSuccess && rc == SP_OK) rc = SP_ERR_DEVICE;
+            }
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                ctx->stat_iterations += 1;
+                ctx->stat_busy_ms += ms;
+            }
+            for (Slot* x : slots) {     // device time share of this iteration by particle-steps
+                if (x->is_shadow || !x->job || P.particle_steps <= 0.0) continue;
+                x->job->device_ms_acc += ms * x->work_share / P.particle_steps;
+                x->work_share = 0.0;
+            }
+            if (rc != SP_OK) {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                for (Slot* x : slots) if (x->job && !x->done) { x->job->status = rc; x->done = true; }
+                ctx->done_cv.notify_all();
+            } else {
+                for (Slot* f : P.finishing) f->done = true;
+                for (Slot* f : P.finishing)
+                    if (!f->is_shadow && f->job && (!f->shadow || f->shadow->done)) finish_job(ctx, f);
+                for (Slot* f : P.finishing)
+                    if (f->is_shadow && f->parent && f->parent->done && f->parent->job &&
+                        std::find(P.finishing.begin(), P.finishing.end(), f->parent) == P.finishing.end())
+                        finish_job(ctx, f->parent);
+            }
+        }
+        // release finished slots (parent and shadow together)
+        std::vector<std::unique_ptr<Slot>> keep;
+        size_t freed = 0;
+        for (auto& a : active) {
+            const bool done_pair = a->done && (a->is_shadow ? (a->parent == nullptr || a->parent->done) : (!a->shadow || a->shadow->done));
+            if (done_pair) { freed += a->dev.bytes; free_dev(a->dev); }
+            else keep.push_back(std::move(a));
+        }
+        // parents point at shadows: clear dangling links of the survivors
+        for (auto& a : keep) {
+            bool has = false;
+            for (auto& b : keep) if (b.get() == a->shadow || b.get() == a->parent) has = true;
+            if (!has) { a->shadow = nullptr; a->parent = nullptr; }
+        }
+        active.swap(keep);
+        device_lock.unlock();
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            size_t acc = 0, n_act = 0;
+            for (auto& a : active) { acc += a->dev.bytes; if (!a->is_shadow && !a->done) ++n_act; }
+            if (freed) ctx->mem_used = acc;
+            ctx->stat_active = (double)n_act;
+        }
+    }
+    for (auto& a : active) free_dev(a->dev);
+    active.clear();
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+}
+
+static void ensure_worker(sp_ctx* ctx) {
+    if (!ctx->worker_started) {
+        ctx->worker_started = true;
+        ctx->t_start = Clock::now();
+        ctx->worker = std::thread(worker_main, ctx);
+    }
+}
+
+void stop_worker(sp_ctx* ctx) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->stop = true;
+    }
+    ctx->cv.notify_all();
+    if (ctx->worker.joinable()) ctx->worker.join();
+}
+
+}  // namespace sp
+
+using namespace sp;
+
+extern "C" {
+
+int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_particles* parts, sp_result* res) {
+    if (!ctx || (!reqs && n_req) || (!parts && n_req) || (!res && n_req)) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    std::vector<HostRequest> hrs(n_req);
+    bool keep_final = false, use_graph = true;
+    for (uint32_t i = 0; i < n_req; ++i) {
+        if (int e = prepare_request(&reqs[i], &parts[i], hrs[i])) return e;
+        keep_final = keep_final || res[i].final_state != nullptr || (reqs[i].flags & SP_FLAG_KEEP_FINAL_STATE);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        use_graph = use_graph && get_tuning(ctx, hrs[i]).use_graph;
+    }
+    std::vector<Outputs> outs;
+    if (int e = run_batch_static(ctx, hrs, outs, keep_final, use_graph)) return e;
+    // per-request FP64 fallback for mixed results that failed the shadow check
+    for (uint32_t i = 0; i < n_req; ++i) {
+        const double tol = hrs[i].rq.check_tol > 0.0 ? hrs[i].rq.check_tol : 1e-3;
+        if (outs[i].check_error > tol && !(hrs[i].rq.flags & SP_FLAG_NO_FALLBACK)) {
+            HostRequest h2;
+            sp_request r2 = reqs[i];
+            r2.precision = SP_PREC_FP64;
+            if (int e = prepare_request(&r2, &parts[i], h2)) return e;
+            Outputs o2;
+            sp_tuning t;
+            {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                t = get_tuning(ctx, h2);
+            }
+            if (int e = run_plain(ctx, h2, o2, keep_final, t)) return e;
+            o2.check_error = outs[i].check_error;
+            o2.used_fallback = 1;
+            outs[i] = std::move(o2);
+        }
+        copy_result(hrs[i], outs[i], &res[i]);
+    }
+    return SP_OK;
+}
+
+int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uint64_t* id) {
+    if (!ctx || !id) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    auto job = std::make_shared<Job>();
+    if (int e = prepare_request(req, parts, job->hr)) return e;
+    job->keep_final = (req->flags & SP_FLAG_KEEP_FINAL_STATE) != 0;
+    job->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->stop) return set_error(SP_ERR_CANCELLED, "context is shutting down");
+        job->id = ctx->next_id++;
+        ctx->jobs[job->id] = job;
+        ctx->pending.push_back(job);
+        ensure_worker(ctx);
+    }
+    ctx->cv.notify_all();
+    *id = job->id;
+    return SP_OK;
+}
+
+int sp_wait(sp_ctx* ctx, uint64_t id, int timeout_ms) {
+    if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
+    std::unique_lock<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(id);
+    if (it == ctx->jobs.end()) return set_error(SP_ERR_NOT_FOUND, "unknown request id");
+    std::shared_ptr<Job> job = it->second;
+    auto ready = [&] { return job->status != SP_ERR_NOT_READY; };
+    if (timeout_ms < 0) ctx->done_cv.wait(lk, ready);
+    else if (!ctx->done_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return set_error(SP_ERR_TIMEOUT, "request not finished");
+    if (job->status != SP_OK) return set_error(job->status, "request failed or was cancelled");
+    return SP_OK;
+}
+
+int sp_result_get(sp_ctx* ctx, uint64_t id, sp_result* res) {
+    if (!ctx || !res) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(id);
+    if (it == ctx->jobs.end()) return set_error(SP_ERR_NOT_FOUND, "unknown request id");
+    std::shared_ptr<Job> job = it->second;
+    if (job->status == SP_ERR_NOT_READY) return set_error(SP_ERR_NOT_READY, "request not finished");
+    if (job->status != SP_OK) {
+        const int st = job->status;
+        ctx->jobs.erase(it);
+        return set_error(st, "request failed or was cancelled");
+    }
+    copy_result(job->hr, job->out, res);
+    ctx->jobs.erase(it);
+    return SP_OK;
+}
+
+int sp_cancel(sp_ctx* ctx, uint64_t id) {
+    if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->jobs.find(id);
+        if (it == ctx->jobs.end()) return set_error(SP_ERR_NOT_FOUND, "unknown request id");
+        it->second->cancel = true;
+    }
+    ctx->cv.notify_all();
+    return SP_OK;
+}
+
+int sp_server_stats(sp_ctx* ctx, double* out) {
+    if (!ctx || !out) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    out[0] = ctx->stat_completed;
+    out[1] = ctx->stat_active;
+    out[2] = (double)ctx->pending.size();
+    out[3] = ctx->stat_iterations;
+    out[4] = ctx->stat_busy_ms;
+    out[5] = ctx->worker_started ? ms_since(ctx->t_start, Clock::now()) : 0.0;
+    out[6] = ctx->stat_particle_steps;
+    out[7] = ctx->stat_fallbacks;
+    return SP_OK;
+}
+
+// Autotuner: times the batch path on a shortened copy of the request for each
+// candidate (work-group size x persistent waves per SIMD x chunk length) and
+// stores the fastest for the request's (problem, solver, precision) class.
+int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_tuning* best, double* best_rate) {
+    if (!ctx || !req || !parts || !best) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    HostRequest base;
+    if (int e = prepare_request(req, parts, base)) return e;
+    // shorten to at most ~2^31 particle-steps per trial
+    sp_request r = *req;
+    const uint64_t OS = r.oversampling;
+    const uint64_t max_steps = std::max<uint64_t>(8 * OS, (uint64_t)(2e9 / (double)base.N) / OS * OS);
+    if (r.n_steps > max_steps) r.n_steps = max_steps;
+    r.flags |= SP_FLAG_SKIP_CHECK;
+    HostRequest probe;
+    if (int e = prepare_request(&r, parts, probe)) return e;
+    const int prec = probe.mixed ? SP_PREC_MIXED : SP_PREC_FP64;
+    const uint32_t blocks[] = {64, 128, 256, 512};
+    const uint32_t waves[] = {1, 2, 4};
+    const uint32_t chunks[] = {2000, 8000, 32000};
+    double best_r = -1.0;
+    sp_tuning bt{};
+    auto trial = [&](const sp_tuning& t) -> double {
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            ctx->tuning[tuning_key(r.problem, r.solver, prec)] = t;
+        }
+        std::vector<HostRequest> one(1);
+        if (prepare_request(&r, parts, one[0])) return -1.0;
+        std::vector<Outputs> outs;
+        const auto t0 = Clock::now();
+        if (run_batch_static(ctx, one, outs, false, t.use_graph != 0)) return -1.0;
+        const double ms = ms_since(t0, Clock::now());
+        return (double)probe.N * (double)probe.total_steps / (ms * 1e-3);
+    };
+    for (uint32_t c : chunks)
+        for (uint32_t b : blocks)
+            for (uint32_t w : waves) {
+                sp_tuning t{b, 1, c, 1, w};
+                trial(t);                        // warm (allocations, code load)
+                const double rate = trial(t);
+                if (rate > best_r) { best_r = rate; bt = t; }
+            }
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->tuning[tuning_key(r.problem, r.solver, prec)] = bt;
+    }
+    *best = bt;
+    if (best_rate) *best_rate = best_r;
+    return best_r > 0 ? SP_OK : set_error(SP_ERR_DEVICE, "autotuner: every trial failed");
+}
+
+}  // extern "C"

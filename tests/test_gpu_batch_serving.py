@@ -1,0 +1,133 @@
+"""Batched persistent path and continuous-batching server on the GPU.
+
+Results of a request must not depend on what it runs with: Philox noise is keyed
+by (seed, particle, step) and partial sums are reduced per 64-particle tile in a
+fixed order, so alone / batched / mixed-with-others results are bit-identical.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from stochasticphysics_amd import Context, configs
+from stochasticphysics_amd.capi import SP_OK, SpError
+
+from test_gpu_parity import bn_ensemble, neel_ensemble, run_oracle, sph_ensemble
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def mixed_bag():
+    return [neel_ensemble(n=200, n_steps=3000, solver="EulerMaruyama", field="sinusoidal", seed=21),
+            neel_ensemble(n=70, n_steps=5000, solver="Heun_Strong", aniso="cubic", seed=22),
+            bn_ensemble(n=130, n_steps=1500, solver="Heun_Strong", seed=23),
+            sph_ensemble(n=150, n_steps=2500, solver="Millstein", seed=24),
+            sph_ensemble(n=64, n_steps=1000, solver="Implicit_Midpoint", seed=25)]
+
+
+def same(a, b):
+    assert np.array_equal(a["mean"], b["mean"])
+    assert np.array_equal(a["weight"], b["weight"])
+    assert np.array_equal(a["time"], b["time"])
+
+
+def test_batch_equals_plain(ctx):
+    bag = mixed_bag()
+    plain = [ctx.run(e) for e in bag]
+    batch = ctx.run_batch(bag)
+    for p, b in zip(plain, batch):
+        same(p, b)
+
+
+def test_batch_graph_and_direct_launch_agree(ctx):
+    bag = mixed_bag()[:3]
+    g = ctx.run_batch(bag)
+    for e in bag:
+        ctx.set_tuning(e.problem, e.solver, e.precision, block_size=128, particles_per_lane=1, chunk_steps=1000,
+                       use_graph=0, persistent_waves_per_simd=1)
+    d = ctx.run_batch(bag)
+    for x, y in zip(g, d):
+        same(x, y)
+
+
+def test_batch_mixed_precision_against_oracle(ctx):
+    bag = [neel_ensemble(n=256, n_steps=4000, solver="Heun_Strong", field="sinusoidal", seed=31),
+           bn_ensemble(n=128, n_steps=2000, solver="Heun_Strong", seed=32)]
+    for e in bag:
+        e.precision = "mixed"
+    res = ctx.run_batch(bag)
+    for e, r in zip(bag, res):
+        o = run_oracle(e, keep_final=False)
+        err = float(np.max(np.abs(r["mean"] - o["mean"])))
+        assert r["precision_used"] == 1 and not r["used_fallback"]
+        assert err <= 1e-3, err
+        # the FP64 shadow check measured the same kind of error on its 256-particle subset
+        assert 0.0 <= r["check_error"] <= 1e-3
+
+
+def test_forced_fp64_fallback(ctx):
+    e = neel_ensemble(n=256, n_steps=2000, solver="Heun_Strong", field="sinusoidal", seed=41)
+    ref = ctx.run(e)                       # FP64 plain
+    e.precision = "mixed"
+    e.check_tol = 1e-300                   # every mixed result fails the check
+    r = ctx.run_batch([e])[0]
+    assert r["used_fallback"] == 1 and r["precision_used"] == 0 and r["check_error"] > 0
+    same(r, ref)
+
+
+def test_server_alone_vs_mix(ctx):
+    bag = mixed_bag()
+    target = bag[2]
+    rid = ctx.submit(target)
+    assert ctx.wait(rid, 120_000) == SP_OK
+    alone = ctx.result(rid, target)
+    ids = [ctx.submit(e) for e in bag]        # the target again, now among four other requests
+    outs = []
+    for rid, e in zip(ids, bag):
+        assert ctx.wait(rid, 120_000) == SP_OK
+        outs.append(ctx.result(rid, e))
+    same(alone, outs[2])
+    plain = [ctx.run(e) for e in bag]
+    for p, o in zip(plain, outs):
+        same(p, o)
+    st = ctx.server_stats()
+    assert st["completed"] >= 6 and st["iterations"] >= 1
+
+
+def test_server_admission_by_memory(ctx):
+    # budget smaller than two requests: they must run one after the other and still complete
+    bag = [neel_ensemble(n=512, n_steps=2000, seed=50 + i) for i in range(4)]
+    ctx.set_memory_budget(1, max_active=1)
+    try:
+        ids = [ctx.submit(e) for e in bag]
+        outs = []
+        for rid, e in zip(ids, bag):
+            assert ctx.wait(rid, 120_000) == SP_OK
+            outs.append(ctx.result(rid, e))
+    finally:
+        ctx.set_memory_budget(8 << 30, max_active=64)
+    for e, o in zip(bag, outs):
+        same(ctx.run(e), o)
+
+
+def test_server_cancel(ctx):
+    big = neel_ensemble(n=4096, n_steps=400_000, os_=100, seed=60)
+    rid = ctx.submit(big)
+    time.sleep(0.05)
+    assert ctx.cancel(rid) == SP_OK
+    rc = ctx.wait(rid, 120_000)
+    assert rc != SP_OK
+    with pytest.raises(SpError):
+        ctx.result(rid, big)
+    # the context keeps serving after a cancellation
+    small = neel_ensemble(n=64, n_steps=500, seed=61)
+    rid2 = ctx.submit(small)
+    assert ctx.wait(rid2, 60_000) == SP_OK
+    same(ctx.result(rid2, small), ctx.run(small))

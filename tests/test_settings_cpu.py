@@ -1,0 +1,78 @@
+"""Parameter files in the reference's .ini schema -> engine request (CPU only)."""
+import ctypes as C
+import json
+import math
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from stochasticphysics_amd.capi import Particles, Request, lib
+
+INI = Path(__file__).parent / "golden" / "ini"
+
+
+def load(name):
+    h = C.c_void_p()
+    rc = lib().sp_settings_load(str(INI / name).encode(), C.byref(h))
+    assert rc == 0, lib().sp_last_error()
+    r, p = Request(), Particles()
+    assert lib().sp_settings_request(h, C.byref(r), C.byref(p)) == 0
+    return h, r, p
+
+
+def arr(ptr, n):
+    return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+
+def test_neel_config1_ini():
+    h, r, p = load("neel_config1.ini")
+    try:
+        assert (r.problem, r.solver, r.field, r.anisotropy, r.precision) == (1, 1, 1, 1, 0)
+        assert (r.n_particles, r.n_steps, r.oversampling, r.seed) == (130, 2000, 100, 11)
+        assert r.dt == 1e-12
+        n = 130
+        assert np.all(arr(p.magnetic_radius, n) == 1e-8) and np.all(arr(p.anisotropy_constant, n) == 1e4)
+        assert np.all(arr(p.temperature, n) == 295.0)
+        axes = arr(p.axes, 9 * n).reshape(n, 9)
+        assert np.allclose(axes, np.tile(np.eye(3).ravel(), (n, 1)), atol=1e-15)   # Euler (0,0,0): identity
+        y0 = arr(p.initial_state, 3 * n).reshape(n, 3)
+        assert np.array_equal(y0, np.tile([0.0, 0.0, 1.0], (n, 1)))
+        assert lib().sp_settings_savefile(h) == b"neel_config1.mat"
+    finally:
+        lib().sp_settings_free(h)
+
+
+def test_bn_lissajous_ini_with_distributions():
+    h, r, p = load("bn_lissajous.ini")
+    try:
+        assert (r.problem, r.solver, r.field) == (3, 7, 4)
+        assert list(r.field_freq) == [25.25e3, 24.5e3, 0.0] and list(r.field_phase) == [0.0, 0.0, 0.0]
+        assert list(r.field_offset) == [0.001, 0.0, 0.0]
+        n = 96
+        rad = arr(p.magnetic_radius, n)
+        # discrete selection between 10 nm (300) and 14 nm (100) types, lognormal 10 % relative width
+        assert 0.7e-8 < rad.min() and rad.max() < 2.0e-8
+        assert np.std(np.log(rad[rad < 1.2e-8] / 1e-8)) == pytest.approx(0.1, rel=0.35)
+        y0 = arr(p.initial_state, 6 * n).reshape(n, 6)
+        assert np.allclose(np.linalg.norm(y0[:, :3], axis=1), 1) and np.allclose(np.linalg.norm(y0[:, 3:], axis=1), 1)
+        rh = arr(p.hydrodynamic_radius, n)
+        assert np.all(rh > rad)
+    finally:
+        lib().sp_settings_free(h)
+
+
+def test_bad_solver_name_is_an_error(tmp_path):
+    f = tmp_path / "bad.ini"
+    f.write_text((INI / "neel_config1.ini").read_text().replace("'EulerMaruyama'", "'NoSuchSolver'"))
+    h = C.c_void_p()
+    assert lib().sp_settings_load(str(f).encode(), C.byref(h)) == 1
+    assert b"NoSuchSolver" in lib().sp_last_error()
+
+
+def test_app_rejects_other_instruction_sets():
+    app = Path(__file__).resolve().parent.parent / "stochasticphysics_amd" / "stophys_gpu_app"
+    r = subprocess.run([str(app), "--parameter_file", str(INI / "neel_config1.ini"), "--instruction_set", "AVX2"],
+                       capture_output=True, text=True)
+    assert r.returncode == 2 and "GFX950" in r.stderr
