@@ -14,22 +14,19 @@ struct KernelInfo {
 // One request over its tiles (grid = one wave per 64-particle tile).
 bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq,
                   uint64_t step_begin, uint64_t step_end, double* partial, uint32_t pstride);
-// Persistent: waves pull TileWork items from an atomic counter until the table is exhausted.
+// Persistent: waves pull global tile indices from an atomic counter until all tiles of the
+// launch's slots are taken.
 bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
-                  const TileWork* work, uint32_t n_work, uint32_t* counter);
-// Persistent, several kernel classes in one launch (uneven batched ensembles).
-bool launch_batch_multi(int mixed, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
-                        const TileWork* work, uint32_t n_work, uint32_t* counter);
-bool multi_supports(uint32_t kind);
+                  const SlotWork* slots, uint32_t n_slots, uint32_t n_tiles, uint32_t* counter);
 bool kernel_info(uint32_t kind, KernelInfo* info);
 
 // per-problem registries (one translation unit each)
 bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
 bool launch_plain_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
 bool launch_plain_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
-bool launch_batch_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
-bool launch_batch_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
-bool launch_batch_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const TileWork*, uint32_t, uint32_t*);
+bool launch_batch_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
+bool launch_batch_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
+bool launch_batch_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool info_neel(uint32_t, KernelInfo*);
 bool info_sph(uint32_t, KernelInfo*);
 bool info_bn(uint32_t, KernelInfo*);

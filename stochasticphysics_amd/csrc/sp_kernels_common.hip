@@ -12,12 +12,12 @@ bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
     }
     return false;
 }
-bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs, const TileWork* work,
-                  uint32_t n, uint32_t* counter) {
+bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs, const SlotWork* slots,
+                  uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
     switch (kind / 64) {
-        case 1: return launch_batch_neel(kind, grid, block, s, reqs, work, n, counter);
-        case 2: return launch_batch_sph(kind, grid, block, s, reqs, work, n, counter);
-        case 3: return launch_batch_bn(kind, grid, block, s, reqs, work, n, counter);
+        case 1: return launch_batch_neel(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+        case 2: return launch_batch_sph(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+        case 3: return launch_batch_bn(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
     }
     return false;
 }

@@ -21,16 +21,28 @@ __device__ __forceinline__ uint32_t next_work(uint32_t* counter) {
     return __builtin_amdgcn_readfirstlane(__shfl(idx, 0, 64));
 }
 
-// Persistent: every wave loops over the work table; the exit condition is reached
-// by all waves once the counter passes n_work.
+// slot owning global tile idx: last s with tile_base <= idx (slots sorted by tile_base)
+__device__ __forceinline__ uint32_t find_slot(const SlotWork* __restrict__ slots, uint32_t n_slots, uint32_t idx) {
+    uint32_t lo = 0, hi = n_slots;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (slots[mid].tile_base <= idx) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Persistent: every wave pulls global tile indices from the counter until all
+// n_tiles tiles of the launch are taken; every wave reaches that exit.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(256) void k_batch(const DevRequest* __restrict__ reqs, const TileWork* __restrict__ work,
-                                               uint32_t n_work, uint32_t* counter) {
+__global__ __launch_bounds__(256) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
+                                               uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
     for (;;) {
         const uint32_t idx = next_work(counter);
-        if (idx >= n_work) break;
-        const TileWork tw = work[idx];
-        const DevRequest rq = reqs[tw.req];
+        if (idx >= n_tiles) break;
+        const SlotWork sw = slots[find_slot(slots, n_slots, idx)];
+        const uint32_t tile = idx - sw.tile_base;
+        const TileWork tw{sw.req, tile, sw.step_begin, sw.step_end, sw.partial, sw.n_tiles, tile};
+        const DevRequest rq = reqs[sw.req];
         run_tile<Cfg, R, KAHAN, MID>(rq, tw);
     }
 }
@@ -53,7 +65,7 @@ bool fill_info(KernelInfo* info, const char* name) {
         return true;
 #define SP_CASE_BATCH(KIND, CFG, R, KAHAN, MID)                                                        \
     case KIND:                                                                                           \
-        k_batch<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(reqs, work, n, counter);                     \
+        k_batch<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(reqs, slots, n_slots, n_tiles, counter);    \
         return true;
 #define SP_CASE_INFO(KIND, CFG, R, KAHAN, MID) \
     case KIND: return fill_info<CFG, R, KAHAN, MID>(info, #CFG);

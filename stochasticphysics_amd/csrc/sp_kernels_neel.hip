@@ -26,7 +26,7 @@ bool launch_plain_neel(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, cons
     switch (kind) { SP_CLASSES(SP_CASE_PLAIN) default: return false; }
 }
 bool launch_batch_neel(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
-                     const TileWork* work, uint32_t n, uint32_t* counter) {
+                     const SlotWork* slots, uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
     switch (kind) { SP_CLASSES(SP_CASE_BATCH) default: return false; }
 }
 bool info_neel(uint32_t kind, KernelInfo* info) {

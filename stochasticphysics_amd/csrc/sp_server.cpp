@@ -31,8 +31,9 @@ static double class_cost(const HostRequest& hr) {
 
 struct IterPlan {
     std::vector<uint32_t> kinds;                      // one persistent launch per kind
-    std::vector<uint32_t> kind_begin, kind_count;     // ranges into work
-    std::vector<TileWork> work;
+    std::vector<uint32_t> kind_begin, kind_count;     // ranges into work (slot entries)
+    std::vector<uint32_t> kind_tiles;                 // tiles per kind launch
+    std::vector<SlotWork> work;
     std::vector<ReduceWork> reduce;
     uint32_t max_rows = 0;
     size_t partial_doubles = 0;
@@ -60,24 +61,23 @@ static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
             P.kinds.push_back(hr.kind);
             P.kind_begin.push_back((uint32_t)P.work.size());
             P.kind_count.push_back(0);
+            P.kind_tiles.push_back(0);
         }
         const uint64_t OS = hr.rq.oversampling;
         const uint64_t sb = s->next_step;
         const uint64_t se = std::min(sb + s->chunk, hr.total_steps);
         const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
         const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
-        for (uint32_t t = 0; t < hr.n_tiles; ++t) {
-            TileWork tw{};
-            tw.req = items[k].slot;
-            tw.tile = t;
-            tw.step_begin = sb;
-            tw.step_end = se;
-            tw.partial = (double*)(uintptr_t)(off * 8);   // relative, patched later
-            tw.pstride = hr.n_tiles;
-            tw.pslot = t;
-            P.work.push_back(tw);
-        }
-        P.kind_count.back() += hr.n_tiles;
+        SlotWork sw{};
+        sw.step_begin = sb;
+        sw.step_end = se;
+        sw.partial = (double*)(uintptr_t)(off * 8);   // relative, patched later
+        sw.req = items[k].slot;
+        sw.tile_base = P.kind_tiles.back();
+        sw.n_tiles = hr.n_tiles;
+        P.work.push_back(sw);
+        P.kind_count.back() += 1;
+        P.kind_tiles.back() += hr.n_tiles;
         ReduceWork rw{};
         rw.partial = (const double*)(uintptr_t)(off * 8);
         rw.pstride = hr.n_tiles;
@@ -107,14 +107,15 @@ static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning&
 }
 
 // Issue the launches of one iteration whose tables are already on the device.
-static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const TileWork* d_work,
+static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const SlotWork* d_work,
                             const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun) {
     if (!P.kinds.empty()) launch_zero_u32(s, d_counters, (uint32_t)P.kinds.size());
     for (size_t k = 0; k < P.kinds.size(); ++k) {
         const sp_tuning& t = tun[k];
         const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(1024, t.block_size / 64 * 64));
-        const dim3 grid = persistent_grid(ctx, P.kind_count[k], t);
-        if (!launch_batch(P.kinds[k], grid, dim3(block), s, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], d_counters + k))
+        const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
+        if (!launch_batch(P.kinds[k], grid, dim3(block), s, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
+                          d_counters + k))
             return set_error(SP_ERR_UNSUPPORTED, "no batch kernel for a requested class");
     }
     if (!P.reduce.empty()) launch_reduce(s, d_reduce, (uint32_t)P.reduce.size(), P.max_rows);
@@ -123,7 +124,7 @@ static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const
 
 static size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-// Device tables of one iteration: [DevRequest x nreq][TileWork x nwork][ReduceWork x nred]
+// Device tables of one iteration: [DevRequest x nreq][SlotWork x nwork][ReduceWork x nred]
 struct TableLayout {
     size_t req_off, work_off, red_off, bytes;
 };
@@ -131,14 +132,14 @@ static TableLayout layout_for(size_t nreq, size_t nwork, size_t nred) {
     TableLayout L;
     L.req_off = 0;
     L.work_off = align256(nreq * sizeof(DevRequest));
-    L.red_off = L.work_off + align256(nwork * sizeof(TileWork));
+    L.red_off = L.work_off + align256(nwork * sizeof(SlotWork));
     L.bytes = L.red_off + align256(nred * sizeof(ReduceWork));
     return L;
 }
 static void fill_tables(char* host, const TableLayout& L, const std::vector<DevRequest>& reqs, const IterPlan& P,
                         double* partial_base) {
     std::memcpy(host + L.req_off, reqs.data(), reqs.size() * sizeof(DevRequest));
-    TileWork* w = (TileWork*)(host + L.work_off);
+    SlotWork* w = (SlotWork*)(host + L.work_off);
     for (size_t i = 0; i < P.work.size(); ++i) {
         w[i] = P.work[i];
         w[i].partial = (double*)((char*)partial_base + (uintptr_t)P.work[i].partial);
@@ -153,7 +154,7 @@ static void fill_tables(char* host, const TableLayout& L, const std::vector<DevR
 static std::vector<sp_tuning> tunings_for(sp_ctx* ctx, const IterPlan& P, const std::vector<Slot*>& slots) {
     std::vector<sp_tuning> t(P.kinds.size());
     for (size_t k = 0; k < P.kinds.size(); ++k) {
-        const TileWork& w = P.work[P.kind_begin[k]];
+        const SlotWork& w = P.work[P.kind_begin[k]];
         t[k] = get_tuning(ctx, *slots[w.req]->hr);
     }
     return t;
@@ -247,7 +248,9 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
     }
     if (int e = ensure_partial(ctx, std::max<size_t>(8, max_partial * 8))) return e;
     if (int e = ensure_table(ctx, std::max<size_t>(256, table_total))) return e;
-    if (int e = ensure_counters(ctx, 16)) return e;
+    size_t max_kinds = 1;
+    for (auto& P : plans) max_kinds = std::max(max_kinds, P.kinds.size());
+    if (int e = ensure_counters(ctx, max_kinds)) return e;
     size_t off = 0;
     std::vector<size_t> offs;
     for (size_t i = 0; i < plans.size(); ++i) {
@@ -262,7 +265,7 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
         for (size_t i = 0; i < plans.size(); ++i) {
             char* base = (char*)ctx->table + offs[i];
             if (int e = record_iteration(ctx, st, plans[i], (const DevRequest*)(base + lays[i].req_off),
-                                         (const TileWork*)(base + lays[i].work_off), (const ReduceWork*)(base + lays[i].red_off),
+                                         (const SlotWork*)(base + lays[i].work_off), (const ReduceWork*)(base + lays[i].red_off),
                                          ctx->counters, tuns[i]))
                 return e;
         }
@@ -455,13 +458,13 @@ static void worker_main(sp_ctx* ctx) {
             }
             int rc = ensure_partial(ctx, std::max<size_t>(8, P.partial_doubles * 8));
             if (rc == SP_OK) rc = ensure_table(ctx, L.bytes);
-            if (rc == SP_OK) rc = ensure_counters(ctx, 16);
+            if (rc == SP_OK) rc = ensure_counters(ctx, std::max<size_t>(1, P.kinds.size()));
             if (rc == SP_OK) {
                 fill_tables((char*)ctx->pinned, L, dreqs, P, ctx->partial);
                 hipMemcpyAsync(ctx->table, ctx->pinned, L.bytes, hipMemcpyHostToDevice, s);
                 hipEventRecord(e0, s);
                 char* base = (char*)ctx->table;
-                rc = record_iteration(ctx, s, P, (const DevRequest*)(base + L.req_off), (const TileWork*)(base + L.work_off),
+                rc = record_iteration(ctx, s, P, (const DevRequest*)(base + L.req_off), (const SlotWork*)(base + L.work_off),
                                       (const ReduceWork*)(base + L.red_off), ctx->counters, tun);
                 hipEventRecord(e1, s);
                 if (hipStreamSynchronize(s) != hipSuccess && rc == SP_OK) rc = SP_ERR_DEVICE;

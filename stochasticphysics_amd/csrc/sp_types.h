@@ -57,6 +57,17 @@ struct TileWork {
     uint32_t pslot;
 };
 
+// One request's share of a persistent launch: tiles [tile_base, tile_base + n_tiles) of
+// the launch's global tile range belong to request `req`, steps [step_begin, step_end).
+struct SlotWork {
+    uint64_t step_begin, step_end;
+    double* partial;           // partial[(row * DOUT + c) * n_tiles + tile]
+    uint32_t req;
+    uint32_t tile_base;
+    uint32_t n_tiles;
+    uint32_t pad;
+};
+
 // kernel class id = problem * 64 + solver * 4 + aniso * 2 + mixed  (host side bookkeeping)
 inline constexpr uint32_t kernel_kind(int problem, int solver, int aniso, int mixed) {
     return (uint32_t)(problem * 64 + solver * 4 + (aniso - 1) * 2 + mixed);
