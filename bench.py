@@ -1,22 +1,32 @@
 """Benchmark of the gfx950 ensemble SDE engine.
 
-Workload (one "step" = one full pass of the hot path over one batch): the
-coupled Brown+Neel ensemble, Heun_Strong, FP64 -- 65,536 particles, 2e5 steps,
-dt 1e-11 s, 12 mT sinusoidal z-field at 100 kHz, output every 50 steps
-(1.3104e10 particle-steps per pass).  Inputs are uploaded once (sp_prepare);
-the timed region runs K passes with the inputs resident in HBM.
+Workload of the headline number (one "step" = one full pass of the hot path
+over one batch): the coupled Brown+Neel ensemble, BrownAndNeel full model,
+uniaxial, Heun_Strong, FP64 -- 65,536 particles, 2e5 steps (199,950 executed:
+output[0] is the start state), dt 1e-11 s, 12 mT sinusoidal z-field at
+100 kHz, output every 50 steps: 1.3104e10 particle-steps per pass.  Inputs
+are uploaded once; the timed region runs K passes with the inputs resident
+in HBM.
 
-Printed JSON line: particle-steps/s of the plain FP64 path (value), plus the
-mixed-precision batched path and its <m> error, serving p50/p99 latency, the
-vector-ALU roofline of the dominant kernel and a CPU baseline on host cores.
+One JSON line on stdout: particle-steps/s of the plain FP64 path (value), the
+vector-ALU roofline of its kernel, a CPU baseline on host cores, and
+  * "mixed": the same ensemble through the batched persistent path in mixed
+    precision against the plain FP64 path, both with per-step renormalisation
+    (see "reference_scheme" for why), error of <m> against FP64;
+  * "serving": Poisson arrivals of spherical-cubic requests (Implicit_Midpoint
+    for damping >= 0.3) through the continuous-batching server, p50/p99
+    end-to-end latency and throughput.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S]
 """
 import argparse
 import json
+import math
 import os
+import re
 import subprocess
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -25,12 +35,13 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# algorithmic FP64 flops per particle-step of the BrownAndNeel Heun_Strong step
-# (add/sub/mul = 1, fma = 2; transcendentals and RNG integer work excluded),
-# counted from the kernel formulation; breakdown in DESIGN.md.
-FLOPS_PER_STEP = {"BrownAndNeel/Heun_Strong": 382.0}
+# Algorithmic FP64 flops per particle-step of the BrownAndNeel (full, uniaxial) Heun_Strong step
+# (add/sub/mul = 1, fma = 2; divisions, square roots, transcendentals and the RNG's integer work
+# excluded): two det+drift evaluations 2 x 107, one S.dW 76, the Heun combination 36, Kahan 24.
+FLOPS_PER_STEP = 350.0
 PEAK_FP64_VALU_TFLOPS = 78.6      # MI355X vector FP64
 PEAK_HBM_GBS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_plain_bn_heun_fp64_traffic.md"
 
 
 def log(msg):
@@ -39,7 +50,7 @@ def log(msg):
 
 def cpu_baseline(e, budget_s=12.0):
     """Oracle (reference-style C++: one particle per task, one thread per core) on a bounded
-    sample of the same workload: all 2e5 steps, fewer particles; rate in particle-steps/s."""
+    sample of the same workload: all steps, fewer particles; rate in particle-steps/s."""
     import oracle
     subprocess.run(["make", "-s", "-B", "-C", str(ROOT / "oracle"), "liboracle_native.so"], check=True,
                    stdout=subprocess.DEVNULL)
@@ -68,6 +79,80 @@ def cpu_baseline(e, budget_s=12.0):
                        f"{cores} threads)")
 
 
+def traffic_per_launch():
+    """HBM bytes per launch of the plain kernel from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a copy kernel), or None when it is absent."""
+    try:
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", TRAFFIC_PROFILE.read_text())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def mixed_leg(ctx, configs, n_steps):
+    """Batched mixed vs plain FP64 on the headline ensemble with renormalisation on both sides."""
+    e64 = configs.config2(n_steps=n_steps)
+    e64.renormalize = True
+    p64 = ctx.prepare(e64)
+    p64.execute()
+    t0 = time.perf_counter()
+    r64 = p64.execute()
+    t64 = time.perf_counter() - t0
+    p64.release()
+    em = configs.config2(n_steps=n_steps, precision="mixed")
+    em.renormalize = True
+    ctx.run_batch([em])                    # warm-up (kernel load, scratch buffers)
+    t0 = time.perf_counter()
+    rm = ctx.run_batch([em])[0]
+    tm = time.perf_counter() - t0
+    ps = e64.particle_steps()
+    err = np.abs(rm["mean"] - r64["mean"])
+    log(f"bench: mixed batched {ps / tm:.3e} vs FP64 plain {ps / t64:.3e} particle-steps/s (renormalised), "
+        f"max |<m> err| {err.max():.2e}, check {rm['check_error']:.2e}, fallback {rm['used_fallback']}")
+    return dict(value=ps / tm, unit="particle-steps/s", path="batched persistent launch", dtype="fp32 bulk, fp64 state",
+                fp64_plain_value=ps / t64, speedup_vs_fp64_plain=t64 / tm, err_max_vs_fp64=float(err.max()),
+                err_rms_vs_fp64=float(np.sqrt(np.mean(err ** 2))), check_error=rm["check_error"],
+                used_fallback=bool(rm["used_fallback"]), renormalized=True)
+
+
+def serving_leg(ctx, configs, seconds, rate, seed=55):
+    """Open-loop Poisson arrivals into the continuous-batching server; latency = submit -> result ready."""
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(rng.poisson(rate * seconds)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = [configs.serving_request(rng, i) for i in range(n_req)]
+    log(f"bench: serving {n_req} requests over {arrivals[-1]:.1f} s ({rate:.0f} req/s offered)")
+    ids = [None] * n_req
+    t0 = time.perf_counter()
+    for i, e in enumerate(reqs):
+        delay = arrivals[i] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids[i] = ctx.submit(e)
+    t_sub = time.perf_counter() - t0
+    lat, ps, fallbacks, im = [], 0.0, 0, 0
+    for i, e in enumerate(reqs):
+        ctx.wait(ids[i], -1)
+        r = ctx.result(ids[i], e)
+        lat.append(r["latency_ms"])
+        ps += r["particle_steps"]
+        fallbacks += int(r["used_fallback"])
+        im += int(e.solver == "Implicit_Midpoint")
+    wall = time.perf_counter() - t0
+    st = ctx.server_stats()
+    lat = np.array(lat)
+    offered = sum(e.particle_steps() for e in reqs) / arrivals[-1]
+    log(f"bench: serving p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, "
+        f"{ps / wall:.3e} particle-steps/s, busy {st['busy_ms'] / 1e3:.1f} s of {wall:.1f} s")
+    return dict(requests=n_req, offered_req_per_s=rate, arrival_window_s=float(arrivals[-1]), submit_s=t_sub,
+                wall_s=wall, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_ms=float(lat.mean()), throughput=ps / wall, unit="particle-steps/s",
+                offered_particle_steps_per_s=offered, device_busy_frac=st["busy_ms"] / (wall * 1e3),
+                implicit_midpoint_requests=im, fp64_fallbacks=fallbacks,
+                request_sizes="N log-uniform [256, 8192], steps U[1e4, 5e4], OS 100; NeelSpherical cubic Lissajous, "
+                              "mixed (Heun_Strong) or FP64 Implicit_Midpoint for damping >= 0.3")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,18 +162,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the ensemble engine shards by particle range across GPUs "
-              f"(design only, see DESIGN.md); --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the ensemble engine shards by "
+              f"particle range across GPUs only by design (one exchange of the window sums per output)",
+              file=sys.stderr)
         sys.exit(2)
 
     from stochasticphysics_amd import Context, configs
     n_steps = 20_000 if args.quick else 200_000
     e = configs.config2(n_steps=n_steps)
     ctx = Context(0)
-    log(f"bench: config2 BrownAndNeel Heun_Strong FP64 N={e.n} S={e.total_steps}")
-    prep = ctx.prepare(e)
+    log(f"bench: BrownAndNeel Heun_Strong FP64 N={e.n} S={e.total_steps}")
+    prep = ctx.prepare(e, keep_final=True)
     for _ in range(args.warmup):
         prep.execute()
     kernel_ms, launches = 0.0, 0
@@ -100,37 +188,33 @@ def main():
         launches += int(st["launches"])
         log(f"bench: pass {i + 1}/{args.steps} done ({st['kernel_ms']:.1f} ms kernel)")
     elapsed = time.perf_counter() - t0
+    prep.release()
     ps = e.particle_steps()
     value = args.steps * ps / elapsed
     kern_avg_ms = kernel_ms / max(1, launches)
-    flops = FLOPS_PER_STEP["BrownAndNeel/Heun_Strong"]
     ps_per_launch = ps / max(1, launches // args.steps)
-    achieved = flops * ps_per_launch / (kern_avg_ms * 1e-3) / 1e12
+    achieved = FLOPS_PER_STEP * ps_per_launch / (kern_avg_ms * 1e-3) / 1e12
+    bad = int((~np.isfinite(ref["final_state"]).all(axis=1)).sum())
     out = dict(metric="particle-steps/s plain-FP64 vs batched-mixed at <=1e-3 <m> err; p50/p99 latency",
                value=value, unit="particle-steps/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=elapsed / args.steps * 1e3, higher_is_better=True, scaling="weak", vs_baseline=None,
                dtype="f64", data="synthetic",
-               config=dict(workload="config2: BrownAndNeel full model, uniaxial, Heun_Strong, FP64; 65536 particles x "
-                                    f"{e.total_steps} steps, dt 1e-11 s, 12 mT 100 kHz sinusoidal z-field, output every 50 steps",
+               config=dict(workload="BrownAndNeel full model, uniaxial, Heun_Strong, FP64: 65536 particles x "
+                                    f"{e.total_steps} steps, dt 1e-11 s, r 12 nm, r_h 25 nm, K 5e3, 12 mT 100 kHz "
+                                    "sinusoidal z-field, output every 50 steps",
                            particles=e.n, steps=e.total_steps, particle_steps_per_pass=ps),
                roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP64_VALU_TFLOPS, unit="TFLOP/s",
-                             frac=achieved / PEAK_FP64_VALU_TFLOPS, traffic=None,
+                             frac=achieved / PEAK_FP64_VALU_TFLOPS, traffic=traffic_per_launch(),
                              kernel="k_plain<BNCfg<uniaxial,Heun_Strong,double>>", kernel_avg_ms=kern_avg_ms,
-                             flops_per_particle_step=flops, vgprs=st["vgprs"]))
-    prep.release()
+                             launches_per_pass=launches // args.steps, flops_per_particle_step=FLOPS_PER_STEP,
+                             vgprs=st["vgprs"]),
+               reference_scheme=dict(nonfinite_particles=bad, finite_mean=bool(np.isfinite(ref["mean"]).all()),
+                                     note="Heun_Strong as the reference writes it does not keep |m| = 1 for this "
+                                          "problem; particles whose |m| runs away overflow, as in the reference"))
     if not args.no_mixed:
-        em = configs.config2(n_steps=n_steps, precision="mixed")
-        em.skip_check = True
-        pm = ctx.prepare(em)
-        pm.execute()
-        t0 = time.perf_counter()
-        mres = pm.execute()
-        t_mixed = time.perf_counter() - t0
-        pm.release()
-        err = float(np.max(np.abs(mres["mean"] - ref["mean"])))
-        out["mixed"] = dict(value=ps / t_mixed, unit="particle-steps/s", path="plain", err_vs_fp64=err,
-                            speedup_vs_fp64=(ps / t_mixed) / value)
-        log(f"bench: mixed {ps / t_mixed:.3e} particle-steps/s, max |<m>_mixed - <m>_fp64| = {err:.2e}")
+        out["mixed"] = mixed_leg(ctx, configs, n_steps)
+    if not args.no_serving:
+        out["serving"] = serving_leg(ctx, configs, args.serve_seconds, args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(e)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]

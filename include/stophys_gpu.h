@@ -46,7 +46,7 @@ typedef enum sp_status {
 /* Settings::IProblem (Settings/ProblemSettings.h:55-70) */
 enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_NEEL = 3 };
 /* Settings::ISolver (Settings/SolverSettings.h:34-47) */
-enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
+enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_EULER_MARUYAMA_NORMALIZED = 2, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
        SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7 };
 /* Properties::IField (Fields/FieldList.h:48-59) */
 enum { SP_FIELD_ZERO = 1, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4 };
@@ -66,7 +66,10 @@ enum {
     SP_FLAG_COORD_TRANSFORM = 1u << 1,   /* NeelSpherical Use_coordinate_transformation (ProblemSettings.h:192-205) */
     SP_FLAG_KEEP_FINAL_STATE = 1u << 2,  /* copy per-particle final state back */
     SP_FLAG_NO_FALLBACK = 1u << 3,       /* mixed precision: report the check error but never re-run in FP64 */
-    SP_FLAG_SKIP_CHECK = 1u << 4         /* mixed precision: no FP64 shadow check */
+    SP_FLAG_SKIP_CHECK = 1u << 4,        /* mixed precision: no FP64 shadow check */
+    SP_FLAG_RENORMALIZE = 1u << 5        /* apply the problem's normalize() (NeelRelaxation.h:204-208,
+                                            BrownAndNeelRelaxation.h:168-172) after every step of any
+                                            solver, in FP64; EulerMaruyama_Normalized always does */
 };
 
 /*
@@ -239,6 +242,10 @@ int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first
                    uint64_t count, int precision, double* out);
 /* host copy of the same generator (the library's own host path) */
 int sp_host_normals(uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, double* out);
+/* Box-Muller transform of caller-given Philox words: out[4i..4i+3] from words[4i..4i+3], in the
+ * FP64 (bit-identical to sp_host_box_muller) or the mixed-precision (FP32 hardware) form */
+int sp_dev_box_muller(sp_ctx* ctx, const uint32_t* words, uint64_t n_blocks, int precision, double* out);
+int sp_host_box_muller(const uint32_t* words, uint64_t n_blocks, double* out);
 
 #ifdef __cplusplus
 }

@@ -137,7 +137,7 @@ def newton_testfn(fn, x0, tol, max_iter):
 
 
 def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0, axes=None, field=None,
-             bn_simple=False, coord_transform=False, min_angle=0.0, newton_tol=1e-10, newton_max_iter=20,
+             bn_simple=False, coord_transform=False, min_angle=0.0, newton_tol=1e-10, newton_max_iter=20, renormalize=False,
              replay=None, threads=1, keep_final=False, per_particle=False, native=False):
     """Run the oracle ensemble.  props: [N,8] (radius, Ms, alpha, gamma, T, K, r_h, viscosity);
     y0: [N, dim_y]; axes: [N, 9] or None; field: dict(type, offset, amp, freq, phase) or None;
@@ -152,7 +152,7 @@ def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0,
     dspec[0], dspec[1], dspec[2], dspec[3] = dt, min_angle, newton_tol, f.get("type", 1)
     for j, key in enumerate(("offset", "amp", "freq", "phase")):
         dspec[4 + 3 * j:7 + 3 * j] = np.asarray(f.get(key, [0.0, 0.0, 0.0]), dtype=np.float64)
-    flags = (1 if bn_simple else 0) | (2 if coord_transform else 0)
+    flags = (1 if bn_simple else 0) | (2 if coord_transform else 0) | (4 if renormalize else 0)
     ispec = np.array([problem, solver, aniso, flags, newton_max_iter, threads], dtype=np.int64)
     rep = None if replay is None else np.ascontiguousarray(replay, dtype=np.float64)
     uspec = np.array([N, n_steps, oversampling, seed, 0 if rep is None else rep.shape[0]], dtype=np.uint64)

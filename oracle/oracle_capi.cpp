@@ -166,9 +166,12 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
                 double* per_particle, double* stats) {
     RunSpec rs{};
     rs.problem = (int)ispec[0]; rs.solver = (int)ispec[1]; rs.aniso = (int)ispec[2];
-    rs.bnSimple = (ispec[3] & 1) != 0; rs.coordTransform = (ispec[3] & 2) != 0;
+    rs.bnSimple = (ispec[3] & 1) != 0; rs.coordTransform = (ispec[3] & 2) != 0; rs.renormalize = (ispec[3] & 4) != 0;
     rs.newtonMaxIter = (int)ispec[4];
     const int nThreads = (int)ispec[5];
+    if (rs.solver != SOLVER_EM && rs.solver != SOLVER_EM_NORMALIZED && rs.solver != SOLVER_IM && rs.solver != SOLVER_MILSTEIN &&
+        rs.solver != SOLVER_HEUN)
+        return 3;
     rs.nParticles = uspec[0]; rs.nSteps = uspec[1]; rs.oversampling = uspec[2]; rs.seed = uspec[3];
     rs.replayCount = uspec[4]; rs.replay = replay;
     rs.dt = dspec[0]; rs.minAngle = dspec[1]; rs.newtonTol = dspec[2];

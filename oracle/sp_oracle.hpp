@@ -215,6 +215,15 @@ inline Axes axesFromEuler(double phi, double theta, double psi) {
 
 template <class Aniso, class S> Vec<S, 3> aniso_field(const Aniso& an, const Vec<S, 3>& m, const Axes& ax) { return an.field(m, ax); }
 
+// Eigen's MatrixBase::normalize() on a 3-block: v /= sqrt(v.v) when v.v > 0.
+template <int N> void normalize3(Vec<double, N>& y, int o) {
+    const double q = (y[o] * y[o] + y[o + 1] * y[o + 1]) + y[o + 2] * y[o + 2];
+    if (q > 0.0) {
+        const double r = std::sqrt(q);
+        y[o] = y[o] / r; y[o + 1] = y[o + 1] / r; y[o + 2] = y[o + 2] / r;
+    }
+}
+
 // ------------------------------------------------------------------ Neel problem
 // Problems/NeelRelaxation.h:27-309.  Stratonovich (NeelRelaxation_Definitions.h:105).
 template <class Aniso> struct NeelProblem {
@@ -225,6 +234,7 @@ template <class Aniso> struct NeelProblem {
 
     template <class S> void prepare(Vec<S, 3>&) {}                         // :195-196
     template <class S> void finish(Vec<S, 3>&) const {}                     // :201-202
+    void normalize(Vec<double, 3>& y) const { normalize3(y, 0); }           // :204-208
     template <class S> Mat<S, 3, 3> sto(const Vec<S, 3>& yi) const {        // :91-109
         const double pre2 = mParams.NeelFactor2 * mParams.NoisePrefactor;
         const double pre1 = mParams.NeelFactor1 * mParams.NoisePrefactor;
@@ -331,6 +341,7 @@ template <class Aniso> struct NeelSphericalProblem {
         Vec<S, 2> r; r[0] = -0.5 * mParams.DriftPrefactor * cos_t * c.one_div_sin_t; r[1] = S(0.0);
         return r;
     }
+    void normalize(Vec<double, 2>&) const {}                                                                 // :534-535
     template <class S> void finish(Vec<S, 2>& y, const SphericalCache<Aniso, S>& c) const {                  // :440-471
         if (c.isRotated) y = rotateBack(y);
         else { y[0] = fmod(y[0], TWO_PI); y[1] = fmod(y[1], TWO_PI); }
@@ -353,6 +364,7 @@ template <class Aniso, bool Simple> struct BrownAndNeelProblem {
 
     template <class S> void prepare(Vec<S, 6>&) {}
     template <class S> void finish(Vec<S, 6>&) const {}
+    void normalize(Vec<double, 6>& y) const { normalize3(y, 0); normalize3(y, 3); }   // BrownAndNeelRelaxation.h:168-172
     template <class S> static Vec<S, 3> head(const Vec<S, 6>& y) { return v3(y[0], y[1], y[2]); }
     template <class S> static Vec<S, 3> tail(const Vec<S, 6>& y) { return v3(y[3], y[4], y[5]); }
     template <class S> static Vec<S, 6> join(const Vec<S, 3>& a, const Vec<S, 3>& b) { Vec<S, 6> r; for (int i = 0; i < 3; ++i) { r[i] = a[i]; r[i + 3] = b[i]; } return r; }

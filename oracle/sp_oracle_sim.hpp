@@ -1,7 +1,7 @@
 // sp_oracle_sim.hpp -- solvers, simulator loop and ensemble mean of the CPU oracle.
 //
 // TEST INFRASTRUCTURE ONLY (see sp_oracle.hpp).  Follows
-//   SDEFramework/Solver/EulerMaruyama.inl:41-72, Heun_Strong.inl:36-47,
+//   SDEFramework/Solver/EulerMaruyama.inl:41-72, EulerMaruyama_Normalized.inl:40-55, Heun_Strong.inl:36-47,
 //   Millstein.inl:34-44, Implicit_Midpoint.inl:38-188,
 //   Simulator/SingleParticleSimulator.h:129-159 and :236-273,
 //   Results/MeanSimulationResult.h:63-119.
@@ -21,12 +21,13 @@
 
 namespace oracle {
 
-enum { SOLVER_EM = 1, SOLVER_IM = 3, SOLVER_MILSTEIN = 6, SOLVER_HEUN = 7 };
+enum { SOLVER_EM = 1, SOLVER_EM_NORMALIZED = 2, SOLVER_IM = 3, SOLVER_MILSTEIN = 6, SOLVER_HEUN = 7 };
 enum { PROB_NEEL = 1, PROB_SPHERICAL = 2, PROB_BN = 3 };
 
 struct RunSpec {
     int problem, solver, aniso;
     bool bnSimple, coordTransform;
+    bool renormalize;          // problem.normalize() after every step, any solver (engine option)
     double dt, minAngle, newtonTol;
     int newtonMaxIter;
     uint64_t nSteps, oversampling, seed, nParticles;
@@ -260,10 +261,15 @@ void simulateParticle(Prob& prob, Stepper& st, const RunSpec& rs, uint64_t p, co
             const double time = rs.dt * ((double)++counter);
             switch (rs.solver) {
                 case SOLVER_EM: yi = st.em(yi, time, ns); break;
+                case SOLVER_EM_NORMALIZED:                    // EulerMaruyama_Normalized.inl:40-55
+                    yi = st.em(yi, time, ns);
+                    prob.normalize(yi);
+                    break;
                 case SOLVER_HEUN: yi = st.heun(yi, time, ns); break;
                 case SOLVER_MILSTEIN: yi = st.milstein(yi, time, ns); break;
                 default: yi = st.implicitMidpoint(yi, time, ns); break;
             }
+            if (rs.renormalize) prob.normalize(yi);
             const Vec<double, DOUT> o = prob.output(yi);
             for (int c = 0; c < DOUT; ++c) {                  // Kahan summation (:148-153)
                 const double y = o[c] - comp[c];

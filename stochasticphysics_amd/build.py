@@ -29,7 +29,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
 
 SOURCES = [
     "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_common.hip",
-    "sp_kernels_multi.hip", "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
+    "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
     "frontend/ini.cpp", "frontend/settings.cpp",
 ]
 
@@ -55,9 +55,7 @@ def _compile(src):
 def build(verbose=False, jobs=None):
     """Compile all sources and link the shared library.  Always recompiles."""
     BUILD.mkdir(exist_ok=True)
-    srcs = [s for s in SOURCES if (CSRC / s).exists()]
-    if not (CSRC / "sp_server.cpp").exists():
-        srcs.append("sp_server_stub.cpp")
+    srcs = list(SOURCES)
     jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))

@@ -12,12 +12,13 @@ LIB_PATH = Path(__file__).resolve().parent / "libstophys_gpu.so"
 
 # enum values (identical to the reference's selector enums)
 PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3)
-SOLVER = dict(EulerMaruyama=1, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
+SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
 FIELD = dict(none=1, sinusoidal=3, lissajous=4)
 ANISO = dict(uniaxial=1, cubic=2)
 PREC = dict(fp64=0, mixed=1)
 NOISE = dict(philox=0, replay=1)
 FLAG_BN_SIMPLE, FLAG_COORD_TRANSFORM, FLAG_KEEP_FINAL, FLAG_NO_FALLBACK, FLAG_SKIP_CHECK = 1, 2, 4, 8, 16
+FLAG_RENORMALIZE = 32
 DIM_Y = {1: 3, 2: 2, 3: 6}
 DIM_W = {1: 3, 2: 3, 3: 6}
 DIM_OUT = {1: 3, 2: 3, 3: 6}
@@ -66,7 +67,7 @@ EXPORTS = [
     "sp_last_error", "sp_abi_version", "sp_parse_enum", "sp_enum_name", "sp_request_shape",
     "sp_compute_particle_params", "sp_ctx_create", "sp_ctx_destroy", "sp_ctx_set_tuning", "sp_ctx_get_tuning",
     "sp_ctx_set_memory_budget", "sp_ctx_last_run_stats", "sp_run", "sp_run_batch", "sp_submit", "sp_wait",
-    "sp_result_get", "sp_cancel", "sp_server_stats", "sp_autotune", "sp_dev_philox", "sp_dev_normals",
+    "sp_result_get", "sp_cancel", "sp_server_stats", "sp_autotune", "sp_dev_philox", "sp_dev_normals", "sp_dev_box_muller", "sp_host_box_muller",
     "sp_host_normals", "sp_prepare", "sp_execute", "sp_release", "sp_settings_load", "sp_settings_request",
     "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free",
 ]
@@ -116,6 +117,8 @@ def lib():
         L.sp_settings_free.argtypes = [C.c_void_p]
         L.sp_settings_free.restype = None
         L.sp_host_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d]
+        L.sp_dev_box_muller.argtypes = [C.c_void_p, P(C.c_uint32), C.c_uint64, C.c_int, d]
+        L.sp_host_box_muller.argtypes = [P(C.c_uint32), C.c_uint64, d]
         _LIB = L
     return _LIB
 

@@ -68,23 +68,31 @@ __device__ __forceinline__ V3<double> field_at(const DevRequest& rq, double t) {
 // Field values for 64 consecutive steps, one step per lane; step n (0-based) is
 // evaluated at t = dt (n+1) (+ dt/2 for ImplicitMidpoint), the reference's
 // time = _timestep*(++counter) (SingleParticleSimulator.h:145) and
-// Implicit_Midpoint.inl:59.
+// Implicit_Midpoint.inl:59.  The refill reads the field parameters with
+// volatile loads so that they are not kept in scalar registers across the loop.
 struct FieldCache {
     uint64_t base;
     double hx, hy, hz;
     bool active;
     __device__ __forceinline__ void init(const DevRequest& rq) { base = ~0ull; hx = hy = hz = 0.0; active = rq.field_type != 1; }
+    // make steps [n, n + count) available (count <= 64, 64 % count == 0, n aligned to count)
     template <bool MIDPOINT>
-    __device__ __forceinline__ V3<double> get(const DevRequest& rq, uint64_t n) {
+    __device__ __forceinline__ void ensure(const DevRequest& rq, uint64_t n, uint32_t count) {
+        if (!active) return;
+        if (base != ~0ull && n - base + count <= 64) return;
+        base = n;
+        const volatile DevRequest* v = &rq;
+        DevRequest f;
+        f.field_type = v->field_type;
+        for (int i = 0; i < 3; ++i) { f.f_off[i] = v->f_off[i]; f.f_amp[i] = v->f_amp[i]; f.f_w[i] = v->f_w[i]; f.f_ph[i] = v->f_ph[i]; }
+        const int lane = threadIdx.x & 63;
+        double t = rq.dt * (double)(n + lane + 1);
+        if (MIDPOINT) t = t + 0.5 * rq.dt;
+        const V3<double> h = field_at(f, t);
+        hx = h.x; hy = h.y; hz = h.z;
+    }
+    __device__ __forceinline__ V3<double> at(uint64_t n) const {
         if (!active) return {0.0, 0.0, 0.0};
-        if (n - base >= 64 || base == ~0ull) {
-            base = n;
-            const int lane = threadIdx.x & 63;
-            double t = rq.dt * (double)(n + lane + 1);
-            if (MIDPOINT) t = t + 0.5 * rq.dt;
-            const V3<double> h = field_at(rq, t);
-            hx = h.x; hy = h.y; hz = h.z;
-        }
         const int k = (int)(n - base);
         return {readlane_d(hx, k), readlane_d(hy, k), readlane_d(hz, k)};
     }
@@ -182,7 +190,7 @@ __device__ __forceinline__ V3<R> neel_s(const NeelLane<ANISO, R>& L, V3<R> m, V3
 }
 
 template <int ANISO, int SOLVER, class R> struct NeelCfg {
-    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4;
+    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4, NORM = 1;   // NORM: 3-blocks that normalize() rescales
     using Lane = NeelLane<ANISO, R>;
     struct Cache {};
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { neel_load<ANISO, R>(rq, p, L); }
@@ -271,7 +279,7 @@ __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
 }
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
-    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2;
+    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2, NORM = 2;
     using Lane = BNLane<ANISO, R>;
     struct Cache {};
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { bn_load<ANISO, R>(rq, p, L); }
@@ -365,7 +373,7 @@ __device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Tri
 }
 
 template <int ANISO, int SOLVER, class R> struct SphCfg {
-    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4;
+    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4, NORM = 0;
     using Lane = SphLane<ANISO, R>;
     using Cache = Trig<R>;          // trig of the current (unrotated) state
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) {
@@ -545,7 +553,7 @@ template <int N> __device__ __forceinline__ bool solve_lin(double A[N][N], doubl
 
 // residuals per problem -------------------------------------------------------------
 template <int ANISO> struct IMNeel {
-    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4;
+    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4, NORM = 1;
     double a, b, as, bs, dt; AnisoLane<double> A; double w;
     __device__ void load(const DevRequest& rq, uint64_t p) {
         a = ld(rq, P_A, p); b = ld(rq, P_B, p);
@@ -572,7 +580,7 @@ template <int ANISO> struct IMNeel {
     }
 };
 template <int ANISO> struct IMBN {
-    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2;
+    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2, NORM = 2;
     double pf, pt, a, b, c, d, cF, s, dt, sq; bool simple; double w;
     __device__ void load(const DevRequest& rq, uint64_t p) {
         a = ld(rq, P_A, p); b = ld(rq, P_B, p); s = ld(rq, P_SIG, p);
@@ -615,7 +623,7 @@ template <int ANISO> struct IMBN {
     }
 };
 template <int ANISO> struct IMSph {
-    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4;
+    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4, NORM = 0;
     double a, b, sg, dt, tmin, tmax; AnisoLane<double> A; double w;
     __device__ void load(const DevRequest& rq, uint64_t p) {
         a = ld(rq, P_A, p); b = ld(rq, P_B, p); sg = ld(rq, P_SIG, p) * rq.sqdt; dt = rq.dt;
@@ -666,7 +674,7 @@ template <int ANISO> struct IMSph {
 };
 
 template <class Prob> struct IMCfg {
-    static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G;
+    static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G, NORM = Prob::NORM;
     using Lane = Prob;
     struct Cache { unsigned fails; };
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { L.load(rq, p); }
@@ -694,6 +702,22 @@ template <class Prob> struct IMCfg {
         for (int i = 0; i < DY; ++i) y[i] = yj[i];
     }
 };
+
+// Eigen's normalize() on each 3-block of the state, in FP64: v /= |v| when |v| > 0
+// (NeelRelaxation.h:204-208, BrownAndNeelRelaxation.h:168-172; a no-op for
+// NeelSpherical, NeelRelaxationSpherical.h:534-535).
+template <int NORM>
+__device__ __forceinline__ void normalize_blocks(double* y) {
+#pragma unroll
+    for (int b = 0; b < NORM; ++b) {
+        double* v = y + 3 * b;
+        const double q = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+        if (q > 0.0) {
+            const double r = sqrt(q);
+            v[0] = v[0] / r; v[1] = v[1] / r; v[2] = v[2] / r;
+        }
+    }
+}
 
 // ================================================================ tile driver
 // Runs steps [tw.step_begin, tw.step_end) of one 64-particle tile
@@ -739,38 +763,66 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     }
     FieldCache fc;
     fc.init(rq);
+    const bool renorm = Cfg::NORM > 0 && (rq.flags & RQ_NORMALIZE) != 0;
     double acc[DOUT], comp[DOUT];
 #pragma unroll
     for (int c = 0; c < DOUT; ++c) { acc[c] = 0.0; comp[c] = 0.0; }
     uint32_t l = 0;
-    for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
-        R z[12];
-        normals12<R>(rq, p, n0 * DW, z);
+    auto accumulate = [&](const double* o) {
 #pragma unroll
-        for (int s = 0; s < G; ++s) {
-            const uint64_t n = n0 + s;
-            if (n < tw.step_end) {
-                const V3<double> h = fc.get<MIDPOINT>(rq, n);
-                Cfg::step(rq, L, y, cache, z + s * DW, h);
+        for (int c = 0; c < DOUT; ++c) {
+            if (KAHAN) {                      // Kahan summation (:148-153)
+                const double yk = o[c] - comp[c];
+                const double t = acc[c] + yk;
+                comp[c] = (t - acc[c]) - yk;
+                acc[c] = t;
+            } else {
+                acc[c] += o[c];
+            }
+        }
+    };
+    auto close_window = [&]() {
+        double mean[DOUT];
+#pragma unroll
+        for (int c = 0; c < DOUT; ++c) { mean[c] = acc[c] / (double)OS; acc[c] = 0.0; comp[c] = 0.0; }
+        flush(mean);
+    };
+    if (OS % G == 0) {
+        // Windows end on group boundaries: each group is straight-line code, and the
+        // next group's Philox + Box-Muller is issued alongside this group's physics.
+        R zc[12], zn[12];
+        if (tw.step_begin < tw.step_end) normals12<R>(rq, p, tw.step_begin * DW, zc);
+        for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
+            fc.ensure<MIDPOINT>(rq, n0, G);
+            normals12<R>(rq, p, (n0 + G) * DW, zn);
+#pragma unroll
+            for (int s = 0; s < G; ++s) {
+                Cfg::step(rq, L, y, cache, zc + s * DW, fc.at(n0 + s));
+                if (renorm) normalize_blocks<Cfg::NORM>(y);
                 double o[DOUT];
                 Cfg::output(L, y, cache, o);
+                accumulate(o);
+            }
+            l += G;
+            if (l == OS) { close_window(); l = 0; }
 #pragma unroll
-                for (int c = 0; c < DOUT; ++c) {
-                    if (KAHAN) {                      // Kahan summation (:148-153)
-                        const double yk = o[c] - comp[c];
-                        const double t = acc[c] + yk;
-                        comp[c] = (t - acc[c]) - yk;
-                        acc[c] = t;
-                    } else {
-                        acc[c] += o[c];
-                    }
-                }
-                if (++l == OS) {
-                    double mean[DOUT];
+            for (int j = 0; j < 12; ++j) zc[j] = zn[j];
+        }
+    } else {
+        for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
+            R z[12];
+            normals12<R>(rq, p, n0 * DW, z);
+            fc.ensure<MIDPOINT>(rq, n0, G);
 #pragma unroll
-                    for (int c = 0; c < DOUT; ++c) { mean[c] = acc[c] / (double)OS; acc[c] = 0.0; comp[c] = 0.0; }
-                    flush(mean);
-                    l = 0;
+            for (int s = 0; s < G; ++s) {
+                const uint64_t n = n0 + s;
+                if (n < tw.step_end) {
+                    Cfg::step(rq, L, y, cache, z + s * DW, fc.at(n));
+                    if (renorm) normalize_blocks<Cfg::NORM>(y);
+                    double o[DOUT];
+                    Cfg::output(L, y, cache, o);
+                    accumulate(o);
+                    if (++l == OS) { close_window(); l = 0; }
                 }
             }
         }

@@ -54,8 +54,8 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (!rq || !pt) return set_error(SP_ERR_INVALID_ARG, "request or particles is NULL");
     uint32_t dy, dw, dout;
     if (request_dims(rq->problem, &dy, &dw, &dout)) return set_error(SP_ERR_UNSUPPORTED, "unknown problem type");
-    if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN &&
-        rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT)
+    if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_EULER_MARUYAMA_NORMALIZED &&
+        rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT)
         return set_error(SP_ERR_UNSUPPORTED, "solver not available on the GPU backend");
     if (rq->field != SP_FIELD_ZERO && rq->field != SP_FIELD_SINUSOIDAL && rq->field != SP_FIELD_LISSAJOUS)
         return set_error(SP_ERR_UNSUPPORTED, "field type not available on the GPU backend");
@@ -89,7 +89,9 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.total_steps = (hr.n_out - 1) * rq->oversampling;
     hr.n_tiles = (uint32_t)((hr.N + 63) / 64);
     hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT;
-    hr.kind = kernel_kind(rq->problem, rq->solver, rq->anisotropy, hr.mixed ? 1 : 0);
+    // EulerMaruyama_Normalized runs the EulerMaruyama kernels with the per-step normalize switched on
+    const int ksolver = rq->solver == SP_SOLVER_EULER_MARUYAMA_NORMALIZED ? SP_SOLVER_EULER_MARUYAMA : rq->solver;
+    hr.kind = kernel_kind(rq->problem, ksolver, rq->anisotropy, hr.mixed ? 1 : 0);
     hr.params.assign((size_t)NPARAM * hr.N, 0.0);
     compute_params(*rq, *pt, hr.params.data(), hr.N);
     hr.y0.resize((size_t)hr.dy * hr.N);
@@ -123,11 +125,12 @@ struct ReqDev {
     double* out = nullptr;        // [n_out*dout] weighted sums
     double* check = nullptr;      // [n_out*dout] weighted sums over the first n_check tiles
     unsigned long long* newton_fail = nullptr;
+    DevRequest* dreq = nullptr;   // device copy of the request descriptor (plain path)
     size_t bytes = 0;
 };
 
 inline size_t request_bytes(const HostRequest& hr) {
-    return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 + 2 * hr.n_out * hr.dout * 8 + 64;
+    return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 + 2 * hr.n_out * hr.dout * 8 + 64 + sizeof(DevRequest);
 }
 
 inline void free_dev(ReqDev& d) {
@@ -137,6 +140,7 @@ inline void free_dev(ReqDev& d) {
     if (d.out) hipFree(d.out);
     if (d.check) hipFree(d.check);
     if (d.newton_fail) hipFree(d.newton_fail);
+    if (d.dreq) hipFree(d.dreq);
     d = ReqDev{};
 }
 
@@ -153,6 +157,7 @@ inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
     if ((e = hipMalloc(&d.out, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("out", e);
     if ((e = hipMalloc(&d.check, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("check", e);
     if ((e = hipMalloc(&d.newton_fail, 8)) != hipSuccess) return fail("counter", e);
+    if ((e = hipMalloc(&d.dreq, sizeof(DevRequest))) != hipSuccess) return fail("descriptor", e);
     d.bytes = request_bytes(hr);
     SP_HIP(hipMemcpyAsync(d.params, hr.params.data(), hr.params.size() * 8, hipMemcpyHostToDevice, s));
     SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
@@ -169,7 +174,7 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     dr.dt = r.dt;
     dr.sqdt = std::sqrt(r.dt);
     const bool rot = (r.flags & SP_FLAG_COORD_TRANSFORM) != 0;
-    dr.tmin = rot ? r.min_angle : -1.0;
+    dr.tmin = rot ? r.min_angle : -1e300;   // no chart rotation: theta may leave [0, pi] freely
     dr.tmax = rot ? (3.14159265358979323846 - r.min_angle) : 1e300;
     dr.newton_tol = r.newton_tol;
     for (int i = 0; i < 3; ++i) {
@@ -185,7 +190,9 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     dr.replay_count = hr.replay.empty() ? 0 : r.replay_count;
     dr.oversampling = (uint32_t)r.oversampling;
     dr.field_type = (uint32_t)r.field;
-    dr.flags = ((r.flags & SP_FLAG_BN_SIMPLE_MODEL) ? RQ_BN_SIMPLE : 0u) | (hr.replay.empty() ? 0u : RQ_REPLAY);
+    const bool norm = (r.flags & SP_FLAG_RENORMALIZE) || r.solver == SP_SOLVER_EULER_MARUYAMA_NORMALIZED;
+    dr.flags = ((r.flags & SP_FLAG_BN_SIMPLE_MODEL) ? RQ_BN_SIMPLE : 0u) | (hr.replay.empty() ? 0u : RQ_REPLAY) |
+               (norm ? RQ_NORMALIZE : 0u);
     dr.newton_max_iter = r.newton_max_iter;
     dr.n_tiles = hr.n_tiles;
     dr.kind = hr.kind;

@@ -15,6 +15,7 @@ const Entry kEntries[] = {
     {"solver", "undefined", 0}, {"solver", "EulerMaruyama", SP_SOLVER_EULER_MARUYAMA},
     {"solver", "EulerMaruyama_Normalized", 2}, {"solver", "Implicit_Midpoint", SP_SOLVER_IMPLICIT_MIDPOINT},
     {"solver", "Implicit_Midpoint_GSL", 4}, {"solver", "Implicit_Midpoint_GSL_Derivative_Free", 5},
+    {"solver", "EulerMaruyama_Normalized", SP_SOLVER_EULER_MARUYAMA_NORMALIZED},
     {"solver", "Millstein", SP_SOLVER_MILLSTEIN}, {"solver", "Heun_Strong", SP_SOLVER_HEUN_STRONG},
     {"solver", "Heun_NotConsistent", 8}, {"solver", "WeakTest", 9}, {"solver", "ExplicitStrong_1.0", 10},
     {"field", "none", SP_FIELD_ZERO}, {"field", "constant", 2}, {"field", "sinusoidal", SP_FIELD_SINUSOIDAL},

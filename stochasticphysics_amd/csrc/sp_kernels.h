@@ -12,7 +12,7 @@ struct KernelInfo {
 };
 
 // One request over its tiles (grid = one wave per 64-particle tile).
-bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq,
+bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq,
                   uint64_t step_begin, uint64_t step_end, double* partial, uint32_t pstride);
 // Persistent: waves pull global tile indices from an atomic counter until all tiles of the
 // launch's slots are taken.
@@ -21,9 +21,9 @@ bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
 bool kernel_info(uint32_t kind, KernelInfo* info);
 
 // per-problem registries (one translation unit each)
-bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
-bool launch_plain_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
-bool launch_plain_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest&, uint64_t, uint64_t, double*, uint32_t);
+bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
+bool launch_plain_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
+bool launch_plain_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool launch_batch_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool launch_batch_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
@@ -40,5 +40,6 @@ void launch_philox_test(hipStream_t s, uint32_t c0, uint32_t c1, uint32_t c2, ui
                         uint64_t count, uint32_t* out);
 void launch_normals_test(hipStream_t s, uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int mixed,
                          double* out);
+void launch_box_muller_test(hipStream_t s, const uint32_t* words, uint64_t n_blocks, int mixed, double* out);
 
 }  // namespace sp

@@ -21,7 +21,7 @@ namespace sp {
     X(kernel_kind(3, S_IM, 2, 0), IMCfg<IMBN<2>>, double, true, true)
 #define SP_COMMA ,
 
-bool launch_plain_bn(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq, uint64_t sb,
+bool launch_plain_bn(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                      uint64_t se, double* partial, uint32_t pstride) {
     switch (kind) { SP_CLASSES(SP_CASE_PLAIN) default: return false; }
 }

@@ -3,7 +3,7 @@
 
 namespace sp {
 
-bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq, uint64_t sb,
+bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                   uint64_t se, double* partial, uint32_t pstride) {
     switch (kind / 64) {
         case 1: return launch_plain_neel(kind, grid, block, s, rq, sb, se, partial, pstride);
@@ -110,6 +110,26 @@ void launch_normals_test(hipStream_t s, uint64_t seed, uint64_t particle, uint64
     if (!count) return;
     const uint64_t blocks = ((first + count - 1) >> 2) - (first >> 2) + 1;
     k_normals_test<<<dim3((unsigned)((blocks + 255) / 256)), dim3(256), 0, s>>>(seed, particle, first, count, mixed, out);
+}
+
+// Box-Muller of caller-given Philox words: out[4i..4i+3] from words[4i..4i+3]
+__global__ void k_box_muller_test(const uint32_t* words, uint64_t n_blocks, int mixed, double* out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_blocks) return;
+    const U4 w{words[4 * i], words[4 * i + 1], words[4 * i + 2], words[4 * i + 3]};
+    if (mixed) {
+        float zf[4];
+        box_muller4_f32(w, zf);
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = (double)zf[j];
+    } else {
+        double z[4];
+        box_muller4_f64(w, z);
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = z[j];
+    }
+}
+void launch_box_muller_test(hipStream_t s, const uint32_t* words, uint64_t n_blocks, int mixed, double* out) {
+    if (!n_blocks) return;
+    k_box_muller_test<<<dim3((unsigned)((n_blocks + 255) / 256)), dim3(256), 0, s>>>(words, n_blocks, mixed, out);
 }
 
 }  // namespace sp

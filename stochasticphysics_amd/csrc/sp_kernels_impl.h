@@ -7,8 +7,9 @@ namespace sp {
 
 // One wave per 64-particle tile of one request.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(256) void k_plain(DevRequest rq, uint64_t step_begin, uint64_t step_end,
+__global__ __launch_bounds__(256) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
                                                double* partial, uint32_t pstride) {
+    const DevRequest& rq = *rqp;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (wave >= rq.n_tiles) return;
     const TileWork tw{0u, wave, step_begin, step_end, partial, pstride, wave};
@@ -42,8 +43,7 @@ __global__ __launch_bounds__(256) void k_batch(const DevRequest* __restrict__ re
         const SlotWork sw = slots[find_slot(slots, n_slots, idx)];
         const uint32_t tile = idx - sw.tile_base;
         const TileWork tw{sw.req, tile, sw.step_begin, sw.step_end, sw.partial, sw.n_tiles, tile};
-        const DevRequest rq = reqs[sw.req];
-        run_tile<Cfg, R, KAHAN, MID>(rq, tw);
+        run_tile<Cfg, R, KAHAN, MID>(reqs[sw.req], tw);
     }
 }
 
@@ -61,7 +61,7 @@ bool fill_info(KernelInfo* info, const char* name) {
 // expands the per-class switch cases for one (problem, solver, aniso, mixed) class
 #define SP_CASE_PLAIN(KIND, CFG, R, KAHAN, MID)                                                        \
     case KIND:                                                                                           \
-        k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);               \
+        k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);              \
         return true;
 #define SP_CASE_BATCH(KIND, CFG, R, KAHAN, MID)                                                        \
     case KIND:                                                                                           \

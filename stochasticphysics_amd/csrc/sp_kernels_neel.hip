@@ -21,7 +21,7 @@ namespace sp {
     X(kernel_kind(1, S_IM, 2, 0), IMCfg<IMNeel<2>>, double, true, true)
 #define SP_COMMA ,
 
-bool launch_plain_neel(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest& rq, uint64_t sb,
+bool launch_plain_neel(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                      uint64_t se, double* partial, uint32_t pstride) {
     switch (kind) { SP_CLASSES(SP_CASE_PLAIN) default: return false; }
 }

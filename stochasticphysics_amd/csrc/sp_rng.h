@@ -150,9 +150,10 @@ __device__ __forceinline__ void box_muller4_f32(U4 w, float z[4]) {
     constexpr float SC = 2.3283064365386963e-10f, HALF = 1.1641532182693481e-10f;
     const float u0 = fmaf((float)w.x, SC, HALF), u1 = fmaf((float)w.y, SC, HALF);
     const float u2 = fmaf((float)w.z, SC, HALF), u3 = fmaf((float)w.w, SC, HALF);
-    // -2 ln u = -2 ln2 log2 u
-    const float r0 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u0));
-    const float r1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u2));
+    // -2 ln u = -2 ln2 log2 u.  u can round to 1 (w >= 2^32 - 128); log2 is clamped at 0 so that an
+    // approximation error of the hardware log at u -> 1 can never make the radicand negative.
+    const float r0 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * fminf(__builtin_amdgcn_logf(u0), 0.0f));
+    const float r1 = __builtin_amdgcn_sqrtf(-1.3862943611198906f * fminf(__builtin_amdgcn_logf(u2), 0.0f));
     z[0] = r0 * __builtin_amdgcn_cosf(u1);
     z[1] = r0 * __builtin_amdgcn_sinf(u1);
     z[2] = r1 * __builtin_amdgcn_cosf(u3);

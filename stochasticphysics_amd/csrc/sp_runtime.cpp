@@ -50,6 +50,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     SP_HIP(hipMemsetAsync(d.check, 0, hr.n_out * hr.dout * 8, s));
     SP_HIP(hipMemsetAsync(d.newton_fail, 0, 8, s));
     const DevRequest dr = make_dev_request(hr, d);
+    SP_HIP(hipMemcpyAsync(d.dreq, &dr, sizeof(DevRequest), hipMemcpyHostToDevice, s));
     const uint64_t OS = hr.rq.oversampling;
     const uint64_t align = lcm64(OS, 4);
     // chunk so that a launch's partial sums fit a 256 MiB buffer
@@ -77,7 +78,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
         const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
         const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
         SP_HIP(hipEventRecord(ev[2 * k], s));
-        if (!launch_plain(hr.kind, grid, dim3(block), s, dr, sb, se, ctx->partial, hr.n_tiles))
+        if (!launch_plain(hr.kind, grid, dim3(block), s, d.dreq, sb, se, ctx->partial, hr.n_tiles))
             return set_error(SP_ERR_UNSUPPORTED, "no kernel for this problem/solver/anisotropy/precision");
         SP_HIP(hipGetLastError());
         SP_HIP(hipEventRecord(ev[2 * k + 1], s));
@@ -301,6 +302,36 @@ int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     hipFree(d);
     if (e != hipSuccess) return set_error(SP_ERR_DEVICE, hipGetErrorString(e));
+    return SP_OK;
+}
+
+int sp_dev_box_muller(sp_ctx* ctx, const uint32_t* words, uint64_t n_blocks, int precision, double* out) {
+    if (!ctx || (n_blocks && (!words || !out))) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    if (precision != SP_PREC_FP64 && precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
+    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    SP_HIP(hipSetDevice(ctx->device));
+    const uint64_t nb = std::max<uint64_t>(1, n_blocks);
+    uint32_t* dw = nullptr;
+    double* d = nullptr;
+    SP_HIP(hipMalloc(&dw, nb * 16));
+    hipError_t e = hipMalloc(&d, nb * 32);
+    if (e == hipSuccess) e = hipMemcpyAsync(dw, words, n_blocks * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+        launch_box_muller_test(ctx->stream, dw, n_blocks, precision == SP_PREC_MIXED, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, n_blocks * 32, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(dw);
+    if (d) hipFree(d);
+    if (e != hipSuccess) return set_error(SP_ERR_DEVICE, hipGetErrorString(e));
+    return SP_OK;
+}
+
+int sp_host_box_muller(const uint32_t* words, uint64_t n_blocks, double* out) {
+    if (n_blocks && (!words || !out)) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    for (uint64_t i = 0; i < n_blocks; ++i)
+        box_muller4_f64(U4{words[4 * i], words[4 * i + 1], words[4 * i + 2], words[4 * i + 3]}, out + 4 * i);
     return SP_OK;
 }
 

@@ -183,7 +183,11 @@ static void make_shadow(const HostRequest& parent, HostRequest& sh) {
     for (uint32_t c = 0; c < sh.dout; ++c) sh.wsum[c] = parent.wsum_check[c];
 }
 
+// Max |<m>_mixed - <m>_FP64| over the shadow subset; a non-finite value anywhere in the mixed
+// result (FP32 overflow of a run-away particle) counts as an infinite error.
 static double check_error_of(const Outputs& parent, const Outputs& shadow) {
+    for (double v : parent.mean)
+        if (!std::isfinite(v)) return HUGE_VAL;
     double e = 0.0;
     for (size_t i = 0; i < parent.check_mean.size() && i < shadow.mean.size(); ++i)
         e = std::max(e, std::fabs(parent.check_mean[i] - shadow.mean[i]));
@@ -338,7 +342,7 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
         if (rc == SP_OK) {
             o.check_error = check_error_of(o, so);
             const double tol = sl->hr->rq.check_tol > 0.0 ? sl->hr->rq.check_tol : 1e-3;
-            if (o.check_error > tol && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
+            if (!(o.check_error <= tol) && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
         }
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -563,7 +567,7 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_p
     // per-request FP64 fallback for mixed results that failed the shadow check
     for (uint32_t i = 0; i < n_req; ++i) {
         const double tol = hrs[i].rq.check_tol > 0.0 ? hrs[i].rq.check_tol : 1e-3;
-        if (outs[i].check_error > tol && !(hrs[i].rq.flags & SP_FLAG_NO_FALLBACK)) {
+        if (outs[i].check_error >= 0.0 && !(outs[i].check_error <= tol) && !(hrs[i].rq.flags & SP_FLAG_NO_FALLBACK)) {
             HostRequest h2;
             sp_request r2 = reqs[i];
             r2.precision = SP_PREC_FP64;

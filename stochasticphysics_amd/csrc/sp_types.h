@@ -32,7 +32,7 @@ struct DevRequest {
     uint64_t replay_count;         // normals available per particle in replay mode
     uint32_t oversampling;
     uint32_t field_type;           // 1 zero, 3 sinusoidal, 4 lissajous
-    uint32_t flags;                // bit0 BN simple model, bit1 replay noise
+    uint32_t flags;                // RQ_BN_SIMPLE, RQ_REPLAY, RQ_NORMALIZE
     uint32_t newton_max_iter;
     uint32_t n_tiles;              // ceil(N / 64)
     uint32_t kind;                 // kernel class (see kernel_kind)
@@ -42,7 +42,7 @@ struct DevRequest {
     unsigned long long* newton_fail;
 };
 
-enum : uint32_t { RQ_BN_SIMPLE = 1u, RQ_REPLAY = 2u };
+enum : uint32_t { RQ_BN_SIMPLE = 1u, RQ_REPLAY = 2u, RQ_NORMALIZE = 4u };
 
 // One unit of work for one wave: 64 particles of one request for a step range.
 // Window rows: row r of the partial buffer holds window first_window + r;

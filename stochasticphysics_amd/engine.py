@@ -46,6 +46,7 @@ class Ensemble:
     check_tol: float = 1e-3
     no_fallback: bool = False
     skip_check: bool = False
+    renormalize: bool = False                  # problem.normalize() after every step (any solver)
     field_offset: tuple = (0.0, 0.0, 0.0)
     field_amplitude: tuple = (0.0, 0.0, 0.0)
     field_freq: tuple = (0.0, 0.0, 0.0)
@@ -120,7 +121,8 @@ class Ensemble:
         r.anisotropy, r.precision = ANISO[self.anisotropy], PREC[self.precision]
         r.noise = NOISE["replay"] if self.replay is not None else NOISE["philox"]
         r.flags = ((capi.FLAG_BN_SIMPLE if self.bn_simple else 0) | (capi.FLAG_COORD_TRANSFORM if self.coord_transform else 0)
-                   | (capi.FLAG_NO_FALLBACK if self.no_fallback else 0) | (capi.FLAG_SKIP_CHECK if self.skip_check else 0))
+                   | (capi.FLAG_NO_FALLBACK if self.no_fallback else 0) | (capi.FLAG_SKIP_CHECK if self.skip_check else 0)
+                   | (capi.FLAG_RENORMALIZE if self.renormalize else 0))
         r.newton_max_iter = self.newton_max_iter
         r.n_particles, r.n_steps, r.oversampling, r.seed = n, self.n_steps, self.oversampling, self.seed
         r.dt, r.newton_tol, r.min_angle, r.check_tol = self.dt, self.newton_tol, self.min_angle, self.check_tol
@@ -162,6 +164,14 @@ def _finish_result(res: Result, o):
 def host_normals(seed, particle, first, count):
     out = np.zeros(count)
     check(lib().sp_host_normals(seed, particle, first, count, dptr(out)))
+    return out
+
+
+def host_box_muller(words):
+    """FP64 Box-Muller of Philox words [n, 4] (uint32) -> [n, 4] standard normals."""
+    w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros(w.shape)
+    check(lib().sp_host_box_muller(w.ctypes.data_as(C.POINTER(C.c_uint32)), w.shape[0], dptr(out)))
     return out
 
 
@@ -304,4 +314,11 @@ class Context:
     def dev_normals(self, seed, particle, first, count, precision="fp64"):
         out = np.zeros(count)
         check(lib().sp_dev_normals(self._ctx, seed, particle, first, count, PREC[precision], dptr(out)))
+        return out
+
+    def dev_box_muller(self, words, precision="fp64"):
+        w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 4)
+        out = np.zeros(w.shape)
+        check(lib().sp_dev_box_muller(self._ctx, w.ctypes.data_as(C.POINTER(C.c_uint32)), w.shape[0], PREC[precision],
+                                      dptr(out)))
         return out

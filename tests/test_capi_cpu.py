@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle
-from stochasticphysics_amd import capi, configs, compute_params, host_normals
+from stochasticphysics_amd import capi, configs, compute_params, host_box_muller, host_normals
 from stochasticphysics_amd.capi import lib
 
 HEADER = Path(__file__).resolve().parent.parent / "include" / "stophys_gpu.h"
@@ -56,6 +56,32 @@ def test_host_normals_equal_oracle():
         assert np.array_equal(host_normals(seed, p, first, count), oracle.normals(seed, p, first, count))
 
 
+EXTREME_WORDS = np.array([[0, 0, 1, 1], [0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFF80, 0x7FFFFFFF],
+                          [0x80000000, 0x40000000, 0xC0000000, 0x3FFFFFFF], [0xFFFFFFFE, 0x1, 0xFFFFFF7F, 0xBFFFFFFF]],
+                         dtype=np.uint32)
+
+
+def test_host_box_muller_closed_form():
+    """Box-Muller of given words against the closed form sqrt(-2 ln u) (cos, sin)(2 pi u), u = (w + 1/2) 2^-32,
+    including the words that give the smallest and the largest u."""
+    w = np.concatenate([EXTREME_WORDS, np.random.default_rng(4).integers(0, 2 ** 32, (256, 4), dtype=np.uint64)
+                        .astype(np.uint32)])
+    z = host_box_muller(w)
+    u = (w.astype(np.float64) + 0.5) * 2.0 ** -32
+    r0, r1 = np.sqrt(-2 * np.log(u[:, 0])), np.sqrt(-2 * np.log(u[:, 2]))
+    ref = np.stack([r0 * np.cos(2 * np.pi * u[:, 1]), r0 * np.sin(2 * np.pi * u[:, 1]),
+                    r1 * np.cos(2 * np.pi * u[:, 3]), r1 * np.sin(2 * np.pi * u[:, 3])], 1)
+    assert np.all(np.isfinite(z))
+    assert np.max(np.abs(z - ref)) <= 1e-14 * np.maximum(1.0, np.abs(ref)).max()
+
+
+def test_host_box_muller_is_the_stream_transform():
+    """normals of a particle's stream = Box-Muller of its Philox blocks (counter (c, 0, particle, 0))."""
+    seed, part = 0x1234_5678_9ABC, 77
+    words = np.stack([oracle.philox([c, 0, part, 0], [seed & 0xFFFFFFFF, seed >> 32]) for c in range(64)])
+    assert np.array_equal(host_box_muller(words).ravel(), host_normals(seed, part, 0, 256))
+
+
 def test_particle_params_match_oracle():
     e = configs.config2(n=16, n_steps=200, oversampling=50)
     P = compute_params(e)
@@ -81,5 +107,5 @@ def test_request_validation_errors():
     assert (n_out.value, dy.value, dw.value, do.value) == (10, 3, 3, 3)
     out = np.zeros(capi.NPARAM * 8)
     n_par = C.c_uint32()
-    r.solver = 2      # EulerMaruyama_Normalized: not built for the GPU backend
+    r.solver = 8      # Heun_NotConsistent: not built for the GPU backend
     assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), C.byref(n_par)) == 2

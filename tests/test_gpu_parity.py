@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
-from stochasticphysics_amd import Context, Ensemble, configs
+from stochasticphysics_amd import Context, Ensemble, configs, host_box_muller
 from stochasticphysics_amd.capi import DIM_W
 
 pytestmark = pytest.mark.gpu
@@ -28,11 +28,13 @@ def ctx():
 
 
 def run_oracle(e: Ensemble, keep_final=True, threads=8):
-    return oracle.simulate(e.pid, {"EulerMaruyama": 1, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7}[e.solver],
+    return oracle.simulate(e.pid, {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint": 3, "Millstein": 6,
+                                   "Heun_Strong": 7}[e.solver],
                            {"uniaxial": 1, "cubic": 2}[e.anisotropy], e.n_steps, e.oversampling, e.dt, e.seed,
                            e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
                            coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
-                           newton_max_iter=e.newton_max_iter, replay=e.replay, threads=threads, keep_final=keep_final)
+                           newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, replay=e.replay, threads=threads,
+                           keep_final=keep_final)
 
 
 # ---------------------------------------------------------------- noise
@@ -73,6 +75,25 @@ def test_normals_fp32_close(ctx):
     # FP32 hardware log2/sqrt/sin/cos on the same uniforms: ~1e-6 relative, worst in the far tail
     assert np.max(np.abs(dev32 - ref)) < 5e-5
     assert abs(dev32.mean()) < 0.02 and abs(dev32.std() - 1) < 0.02
+
+
+def test_box_muller_extreme_words(ctx):
+    """Every word that puts u0 within 2^-16 of 1 (where u rounds to 1 in FP32) and of 0, plus random words:
+    the FP64 transform is bit-identical to the host copy, the FP32 one finite and close to it."""
+    hi = np.arange(0xFFFF0000, 0x1_0000_0000, dtype=np.uint64).astype(np.uint32)
+    lo = np.arange(0, 0x10000, dtype=np.uint32)
+    rnd = np.random.default_rng(8).integers(0, 2 ** 32, (4, 1 << 16), dtype=np.uint64).astype(np.uint32)
+    blocks = [np.stack([hi, rnd[0], hi[::-1], rnd[1]], 1), np.stack([lo, rnd[2], lo[::-1], rnd[3]], 1), rnd.T]
+    w = np.concatenate(blocks)
+    ref = host_box_muller(w)
+    assert np.array_equal(ctx.dev_box_muller(w, "fp64"), ref)
+    z32 = ctx.dev_box_muller(w, "mixed")
+    assert np.all(np.isfinite(z32))
+    err = np.abs(z32 - ref)
+    # FP32 u = (w + 1/2) 2^-32 carries an absolute rounding error <= 2^-25 near 1, so r^2 = -2 ln u ~ 2 (1 - u)
+    # is off by <= 2^-23 and r by <= 2^-11.5 when u -> 1 (the first 2^16 rows); elsewhere ~1e-6 relative
+    assert err[:0x10000].max() < 2.0 ** -11.5 + 5e-5
+    assert err[0x10000:].max() < 5e-5
 
 
 # ---------------------------------------------------------------- helpers to build small ensembles
@@ -170,12 +191,41 @@ def test_spherical_parity(ctx, solver, aniso):
 
 
 def test_spherical_no_rotation(ctx):
-    """Without chart rotation the reference's 1/sin(theta) terms blow up at the poles and amplify rounding
-    differences without bound, so this case starts away from them (theta in [pi/6, 5 pi/6])."""
-    e = sph_ensemble(solver="EulerMaruyama", rotate=False, aniso="uniaxial", field="none")
+    """Without chart rotation theta is never folded back: particles pass through the poles and theta leaves
+    [0, pi] (here one reaches theta < -1).  Near a pole the reference's 1/sin(theta) terms amplify rounding
+    differences, so T = 100 K keeps that amplification below 1e-13 (an FMA-contracted oracle build differs
+    from the plain one by 9e-14 on this case)."""
+    e = sph_ensemble(solver="EulerMaruyama", rotate=False, aniso="uniaxial", field="none", T=100.0)
     rng = np.random.default_rng(3)
     e.y0 = np.stack([rng.uniform(math.pi / 6, 5 * math.pi / 6, e.n), rng.uniform(0, 2 * math.pi, e.n)], 1)
-    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10, spherical=True)
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    assert (o["final_state"][:, 0] < -1.0).any()
+    compare(g, o, 1e-10, spherical=True)
+
+
+@pytest.mark.parametrize("make", [lambda: neel_ensemble(solver="EulerMaruyama_Normalized", field="sinusoidal", n=130),
+                                  lambda: bn_ensemble(solver="EulerMaruyama_Normalized"),
+                                  lambda: sph_ensemble(solver="EulerMaruyama_Normalized", n=64)])
+def test_euler_maruyama_normalized_parity(ctx, make):
+    """EulerMaruyama_Normalized (EulerMaruyama_Normalized.inl:40-55): EulerMaruyama plus normalize() each step;
+    the Cartesian states stay on the unit sphere."""
+    e = make()
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    compare(g, o, 1e-10, spherical=e.problem == "NeelSpherical")
+    if e.problem != "NeelSpherical":
+        for b in range(e.dim_y // 3):
+            assert np.allclose(np.linalg.norm(g["final_state"][:, 3 * b:3 * b + 3], axis=1), 1.0, atol=1e-14)
+
+
+@pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein", "Implicit_Midpoint"])
+def test_renormalize_flag_parity(ctx, solver):
+    """The renormalize option applies normalize() after every step of any solver, here on BrownAndNeel."""
+    e = bn_ensemble(solver=solver, n=72, n_steps=600 if solver != "Implicit_Midpoint" else 200)
+    e.renormalize = True
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    assert g["newton_failures"] == o["newton_failures"]
+    compare(g, o, 1e-10)
+    assert np.allclose(np.linalg.norm(g["final_state"][:, 3:], axis=1), 1.0, atol=1e-14)
 
 
 @pytest.mark.parametrize("problem", ["Neel", "NeelSpherical", "BrownAndNeel"])
@@ -226,8 +276,9 @@ def test_zero_temperature_fixed_point(ctx):
     e.T = np.zeros(64)
     g = ctx.run(e, keep_final=True)
     assert np.array_equal(g["final_state"], np.tile([0.0, 0.0, 1.0], (64, 1)))
-    # the mean is sum(mu m_z) / sum(mu): exact up to the rounding of the two sums
-    assert np.max(np.abs(g["mean"] - np.tile([0.0, 0.0, 1.0], (e.n_out, 1)))) <= 4e-16
+    # the mean is sum(mu m_z) / sum(mu) with the two sums taken in different orders: each carries a relative
+    # rounding error of at most (n - 1) u for n = 64 positive terms, so the ratio is 1 within 2 * 63 * 2^-53
+    assert np.max(np.abs(g["mean"] - np.tile([0.0, 0.0, 1.0], (e.n_out, 1)))) <= 2 * 63 * 2.0**-53
 
 
 # ---------------------------------------------------------------- mixed precision

@@ -1,0 +1,189 @@
+"""Pins of the oracle's solvers and simulator loop by what the mathematics fixes (CPU only).
+
+The reference's golden vectors pin the drift, diffusion and Jacobian terms of each problem
+(test_oracle_golden.py); these tests pin what they leave open -- the solver steps, the noise
+scaling, the Stratonovich drift corrections, the chart handling and the per-step normalisation:
+
+* deterministic LLG in a constant field has a closed form (tanh law for the polar angle, uniform
+  precession), which EulerMaruyama, Heun_Strong, Millstein and Implicit_Midpoint must approach with
+  their orders of convergence (a dropped or mis-signed term breaks the order, not only the value);
+* at finite temperature the stationary distribution on the sphere is the Boltzmann one
+  (Langevin function for a field, Boltzmann integral for uniaxial anisotropy);
+* free Brownian rotation of the particle axis decays as exp(-2 D_r t), D_r = kB T / (6 eta V_h).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from stochasticphysics_amd import configs
+from stochasticphysics_amd.engine import Ensemble
+
+KB = 1.3806485279e-23
+SOLVER = {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7}
+
+
+def run(e: Ensemble, threads=8):
+    return oracle.simulate(e.pid, SOLVER[e.solver], {"uniaxial": 1, "cubic": 2}[e.anisotropy], e.n_steps, e.oversampling,
+                           e.dt, e.seed, e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
+                           coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
+                           newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, threads=threads, keep_final=True)
+
+
+# ---------------------------------------------------------------- deterministic closed form
+B, ALPHA, R = 0.05, 0.1, 10e-9
+A_N = -configs.GAMMA / (1 + ALPHA ** 2)       # NeelFactor1
+B_N = configs.GAMMA * ALPHA / (1 + ALPHA ** 2)  # NeelFactor2
+
+
+def exact_m(m0, t, b=B_N):
+    """dm/dt = a m x H - b m x (m x H), H = B z, |m| = 1: m_z = tanh(b B t + atanh m_z0), phi = phi0 - a B t."""
+    mz = math.tanh(b * B * t + math.atanh(m0[2]))
+    phi = math.atan2(m0[1], m0[0]) - A_N * B * t
+    s = math.sqrt(1 - mz * mz)
+    return np.array([s * math.cos(phi), s * math.sin(phi), mz])
+
+
+def precession(problem, solver, dt, n_steps, os_=10, rotate=False, m0=(0.6, 0.0, -0.8)):
+    m0 = np.asarray(m0, dtype=np.float64)
+    n = 1
+    kw = dict(problem=problem, solver=solver, anisotropy="uniaxial", field="sinusoidal", n_steps=n_steps, oversampling=os_,
+              dt=dt, seed=1, radius=np.full(n, R), Ms=np.full(n, 4.8e5), alpha=np.full(n, ALPHA),
+              gamma=np.full(n, configs.GAMMA), T=np.zeros(n), K=np.zeros(n), field_offset=(0.0, 0.0, B),
+              axes=np.tile([1.0, 0, 0, 0, 1, 0, 0, 0, 1], (n, 1)), newton_tol=1e-13, newton_max_iter=50)
+    if problem == "NeelSpherical":
+        kw.update(y0=configs.to_spherical(m0[None, :]), coord_transform=rotate, min_angle=math.pi / 6)
+    elif problem == "BrownAndNeel":   # K = 0: the axis n is carried along, m follows the Neel law with b + d
+        kw.update(y0=np.concatenate([[0.0, 0.0, 1.0], m0])[None, :], r_h=np.full(n, 25e-9), viscosity=np.full(n, 1e-3))
+    else:
+        kw.update(y0=m0[None, :])
+    e = Ensemble(**kw)
+    o = run(e, threads=1)
+    y = o["final_state"][0]
+    if problem == "NeelSpherical":
+        m = np.array([math.sin(y[0]) * math.cos(y[1]), math.sin(y[0]) * math.sin(y[1]), math.cos(y[0])])
+    elif problem == "BrownAndNeel":
+        m = y[3:]
+    else:
+        m = y
+    t = dt * ((n_steps // os_) - 1) * os_
+    b = B_N
+    if problem == "BrownAndNeel":     # omega_N = -a H + (b + d) m x H: the Neel law with b -> b + d, d = c mu
+        mu = 4.8e5 * 4.0 / 3.0 * math.pi * R ** 3
+        b = B_N + mu / (6e-3 * 4.0 / 3.0 * math.pi * (25e-9) ** 3)
+    return np.max(np.abs(m - exact_m(m0, t, b)))
+
+
+ORDERS = [("Neel", "EulerMaruyama", 1), ("Neel", "Heun_Strong", 2), ("Neel", "Millstein", 1),
+          ("Neel", "Implicit_Midpoint", 2), ("NeelSpherical", "EulerMaruyama", 1), ("NeelSpherical", "Heun_Strong", 2),
+          ("NeelSpherical", "Implicit_Midpoint", 2), ("BrownAndNeel", "EulerMaruyama", 1),
+          ("BrownAndNeel", "Heun_Strong", 2)]
+
+
+@pytest.mark.parametrize("problem,solver,order", ORDERS)
+def test_zero_temperature_convergence_order(problem, solver, order):
+    """T = 0, K = 0, constant field: the final state converges to the closed form with the solver's order."""
+    dt = 4e-13 if problem != "BrownAndNeel" else 2e-13
+    e1 = precession(problem, solver, dt, 5000)
+    e2 = precession(problem, solver, dt / 2, 10000 - 10)   # same final time: (n_out - 1) OS steps
+    ratio = e1 / e2
+    assert e1 < (5e-2 if order == 1 else 2e-4), e1
+    assert 2 ** order * 0.8 < ratio < 2 ** order * 1.25, (e1, e2, ratio)
+
+
+def test_chart_rotation_tracks_the_closed_form():
+    """NeelSpherical with the chart rotation on (theta_min = pi/6): the orbit starts in the rotated chart
+    (m_z = -0.9, theta > 5 pi/6), crosses the unrotated band and ends in the rotated chart again near +z
+    (m_z ~ 0.965, theta < pi/6); Heun_Strong still converges with order 2."""
+    m0 = (math.sqrt(0.19), 0.0, -0.9)
+    e1 = precession("NeelSpherical", "Heun_Strong", 4e-13, 10000, rotate=True, m0=m0)
+    e2 = precession("NeelSpherical", "Heun_Strong", 2e-13, 19990, rotate=True, m0=m0)
+    assert e1 < 2e-4 and 3.2 < e1 / e2 < 5.0, (e1, e2)
+
+
+# ---------------------------------------------------------------- statistics
+def langevin(x):
+    return 1.0 / math.tanh(x) - 1.0 / x
+
+
+def thermal(problem, solver, n, n_steps, K=0.0, Bz=0.0, T=300.0, seed=7, dt=1e-12, os_=100, renormalize=False):
+    rng = np.random.default_rng(seed)
+    m0 = configs.unit_sphere(rng, n)
+    kw = dict(problem=problem, solver=solver, anisotropy="uniaxial", field="sinusoidal", n_steps=n_steps,
+              oversampling=os_, dt=dt, seed=seed, radius=np.full(n, R), Ms=np.full(n, 4.8e5), alpha=np.full(n, 0.3),
+              gamma=np.full(n, configs.GAMMA), T=np.full(n, T), K=np.full(n, K), field_offset=(0.0, 0.0, Bz),
+              axes=np.tile([1.0, 0, 0, 0, 1, 0, 0, 0, 1], (n, 1)), renormalize=renormalize)
+    if problem == "NeelSpherical":
+        kw.update(y0=configs.to_spherical(m0), coord_transform=True, min_angle=math.pi / 36)
+    else:
+        kw.update(y0=m0)
+    e = Ensemble(**kw)
+    o = run(e)
+    f = o["final_state"]
+    if problem == "NeelSpherical":
+        f = np.stack([np.sin(f[:, 0]) * np.cos(f[:, 1]), np.sin(f[:, 0]) * np.sin(f[:, 1]), np.cos(f[:, 0])], 1)
+    return e, o, f
+
+
+@pytest.mark.parametrize("problem,solver,renorm", [("NeelSpherical", "EulerMaruyama", False),
+                                                    ("NeelSpherical", "Heun_Strong", False),
+                                                    ("Neel", "EulerMaruyama_Normalized", False),
+                                                    ("Neel", "Heun_Strong", True)])
+def test_langevin_equilibrium(problem, solver, renorm):
+    """Isotropic particle in a constant field: <m_z> -> L(mu B / kB T) at equilibrium (3 sigma / sqrt(N))."""
+    n = 1500
+    mu = 4.8e5 * 4.0 / 3.0 * math.pi * R ** 3
+    Bz = 0.004
+    xi = mu * Bz / (KB * 300.0)
+    e, o, f = thermal(problem, solver, n, 40_000, Bz=Bz, renormalize=renorm)
+    tail = o["mean"][len(o["mean"]) // 2:, 2]        # time average over the equilibrated half
+    sig = math.sqrt(1.0 - 2.0 * langevin(xi) / xi - langevin(xi) ** 2)   # std of m_z under the Langevin law
+    assert abs(f[:, 2].mean() - langevin(xi)) < 3 * sig / math.sqrt(n), (f[:, 2].mean(), langevin(xi))
+    assert abs(tail.mean() - langevin(xi)) < 3 * sig / math.sqrt(n)
+    assert np.allclose(np.linalg.norm(f, axis=1), 1.0, atol=1e-12)
+
+
+def test_uniaxial_boltzmann_mz2():
+    """Zero field, uniaxial K: the reference's field H = (-2K/Ms)(m.n) n (UniaxialAnisotropy.h:71-77) is the
+    gradient of E = +K V (m.n)^2 (so K < 0 is an easy axis, as its NegativeKmeansUniaxial test states), hence
+    <m_z^2> = int x^2 e^{-s x^2} / int e^{-s x^2} with s = K V / kB T (by quadrature).  Both signs of K."""
+    n = 1500
+    V = 4.0 / 3.0 * math.pi * R ** 3
+    x = np.linspace(-1.0, 1.0, 200_001)
+    for K in (4e3, -4e3):
+        s = K * V / (KB * 300.0)
+        w = np.exp(-s * x * x)
+        mz2 = np.trapezoid(x * x * w, x) / np.trapezoid(w, x)
+        var = np.trapezoid(x ** 4 * w, x) / np.trapezoid(w, x) - mz2 ** 2
+        e, o, f = thermal("NeelSpherical", "Heun_Strong", n, 40_000, K=K)
+        assert abs((f[:, 2] ** 2).mean() - mz2) < 3 * math.sqrt(var / n), (K, (f[:, 2] ** 2).mean(), mz2)
+
+
+def test_brownian_rotation_decay():
+    """BrownAndNeel with K = 0 and no field: the axis n diffuses freely, <n(t).n(0)> = exp(-2 D_r t) with
+    D_r = kB T c, c = 1 / (6 eta V_h) (ParameterCalculatorBrown.h:43), in the simple model (the full model adds
+    Neel-noise terms d sigma to the axis); EulerMaruyama_Normalized keeps |n| = 1."""
+    n = 2000
+    r_h, eta, T = 12e-9, 2e-6, 300.0
+    Vh = 4.0 / 3.0 * math.pi * r_h ** 3
+    Dr = KB * T / (6 * eta * Vh)
+    dt, os_ = 1e-12, 50
+    steps = int(0.6 / (2 * Dr) / dt) // os_ * os_
+    rng = np.random.default_rng(3)
+    e = Ensemble(problem="BrownAndNeel", solver="EulerMaruyama_Normalized", anisotropy="uniaxial", field="none",
+                 n_steps=steps, oversampling=os_, dt=dt, seed=9, radius=np.full(n, R), Ms=np.full(n, 4.8e5),
+                 alpha=np.full(n, 0.1), gamma=np.full(n, configs.GAMMA), T=np.full(n, T), K=np.zeros(n),
+                 r_h=np.full(n, r_h), viscosity=np.full(n, eta), bn_simple=True,
+                 y0=np.concatenate([np.tile([0.0, 0.0, 1.0], (n, 1)), configs.unit_sphere(rng, n)], axis=1))
+    o = run(e)
+    t = o["times"]
+    nz = o["mean"][:, 2]                 # Brown block: plain (unweighted) mean of n_z
+    k = len(t) - 1
+    expect = math.exp(-2 * Dr * dt * k * os_)       # end of the last window (its mean is within O(OS dt) of this)
+    # per-particle n_z(t) has variance (1 + 2 e^{-6 D_r t}) / 3 - e^{-4 D_r t}
+    tt = dt * k * os_
+    var = (1 + 2 * math.exp(-6 * Dr * tt)) / 3 - math.exp(-4 * Dr * tt)
+    assert abs(o["final_state"][:, 2].mean() - expect) < 3 * math.sqrt(var / n), (o["final_state"][:, 2].mean(), expect)
+    assert np.allclose(np.linalg.norm(o["final_state"][:, :3], axis=1), 1.0, atol=1e-12)
+    assert np.allclose(np.linalg.norm(o["final_state"][:, 3:], axis=1), 1.0, atol=1e-12)

@@ -187,3 +187,35 @@ def test_uniaxial_sign_convention():
     props = oracle.props_array(10e-9, 477464.0, 0.1, 1.76e11, 295.0, 1e4)
     vals = [-oracle.aniso_field(1, props, e, axes)[0] @ e for e in np.eye(3)]
     assert vals[0] > vals[1] and vals[0] > vals[2] and vals[1] == vals[2]
+
+
+def test_cubic_field_is_energy_gradient():
+    """Cubic anisotropy has no reference test: pin H = K1_MS sum_i c_i (c_j^2 + c_k^2) e_i (CubicAnisotropy.h:109-146,
+    K1_MS = -2K/Ms) as -(1/Ms) grad of K sum_{i<j} c_i^2 c_j^2 (central differences) for random cube orientations,
+    plus its symmetries: zero along a cube axis, parallel to m along a body diagonal."""
+    rng = np.random.default_rng(12)
+    Ms, K = 4.8e5, -1.1e4
+    props = oracle.props_array(11e-9, Ms, 0.05, 1.76e11, 300.0, K)
+
+    def energy(m, ax):
+        c = ax @ m
+        return K * (c[0] ** 2 * c[1] ** 2 + c[1] ** 2 * c[2] ** 2 + c[0] ** 2 * c[2] ** 2)
+
+    for _ in range(8):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        ax = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y + w * z), 2 * (x * z - w * y)],
+                       [2 * (x * y - w * z), 1 - 2 * (x * x + z * z), 2 * (y * z + w * x)],
+                       [2 * (x * z + w * y), 2 * (y * z - w * x), 1 - 2 * (x * x + y * y)]])
+        m = rng.standard_normal(3)
+        m /= np.linalg.norm(m)
+        H, _ = oracle.aniso_field(2, props, m, ax)
+        h = 1e-6
+        grad = np.array([(energy(m + h * e, ax) - energy(m - h * e, ax)) / (2 * h) for e in np.eye(3)])
+        assert np.allclose(H, -grad / Ms, rtol=1e-7, atol=1e-12 * abs(K) / Ms)
+        for i in range(3):                                   # along a cube axis: c = (1, 0, 0) up to rounding -> H = 0
+            assert np.allclose(oracle.aniso_field(2, props, ax[i], ax)[0], 0.0, atol=1e-14 * abs(K) / Ms)
+        d = ax.sum(axis=0) / math.sqrt(3.0)                  # body diagonal: c_i = 1/sqrt(3) -> H = (-2K/Ms)(2/3) d
+        Hd = oracle.aniso_field(2, props, d, ax)[0]
+        assert np.allclose(Hd, (-2 * K / Ms) * (2.0 / 3.0) * d, rtol=1e-13, atol=1e-14 * abs(K) / Ms)
