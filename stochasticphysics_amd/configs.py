@@ -50,10 +50,10 @@ def full(n, v):
     return np.full(n, float(v))
 
 
-def config1(n=4096, n_steps=200_000, oversampling=100, seed=11):
+def config1(n=4096, n_steps=200_000, oversampling=100, seed=11, precision="fp64"):
     """Plain Neel relaxation, uniaxial, EulerMaruyama, FP64, zero field, easy axes and m0 along +z."""
     axes = np.tile(np.eye(3).ravel(), (n, 1))
-    return Ensemble(problem="Neel", solver="EulerMaruyama", field="none", anisotropy="uniaxial", precision="fp64",
+    return Ensemble(problem="Neel", solver="EulerMaruyama", field="none", anisotropy="uniaxial", precision=precision,
                     n_steps=n_steps, oversampling=oversampling, dt=1e-12, seed=seed,
                     radius=full(n, 10e-9), Ms=full(n, 4.8e5), alpha=full(n, 0.1), gamma=full(n, GAMMA),
                     T=full(n, 295.0), K=full(n, 1.0e4), axes=axes, y0=np.tile([0.0, 0.0, 1.0], (n, 1)))

@@ -266,7 +266,9 @@ def test_zero_temperature_parity(ctx, solver):
 
 
 def test_zero_temperature_brown_and_neel(ctx):
-    e = bn_ensemble(n=64, n_steps=5000, T=0.0)
+    """2000 steps: an FMA-contracted oracle build differs from the plain one by 1e-14 here (3.5e-13 at 5000
+    steps, where the driven orbit has started to amplify rounding)."""
+    e = bn_ensemble(n=64, n_steps=2000, T=0.0)
     compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-12)
 
 
