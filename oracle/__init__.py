@@ -37,9 +37,10 @@ def lib(native=False):
         d, u64, i64 = C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_int64)
         L.or_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
         L.or_normals.argtypes = [u64, u64, u64, u64, C.c_int, d]
-        L.or_bm_log.argtypes = [C.c_double]
+        L.or_bm_log.argtypes = [C.c_uint32]
         L.or_bm_log.restype = C.c_double
-        L.or_bm_sincos2pi.argtypes = [C.c_double, d, d]
+        L.or_bm_sincos2pi.argtypes = [C.c_uint32, d, d]
+        L.or_box_muller.argtypes = [C.POINTER(C.c_uint32), u64, d]
         L.or_neel_params.argtypes = [C.c_double] * 5 + [d]
         L.or_brown_params.argtypes = [C.c_double] * 3 + [d]
         L.or_aniso_field.argtypes = [C.c_int, d, d, d, d, d]
@@ -72,14 +73,23 @@ def normals(seed, particle, first, count, libm=False):
     return out
 
 
-def bm_log(u):
-    return lib().or_bm_log(float(u))
+def bm_log(w):
+    """ln u of the engine's Box-Muller, u = (w + 1/2) 2^-32, for a 32-bit Philox word w."""
+    return lib().or_bm_log(int(w))
 
 
-def bm_sincos2pi(u):
+def bm_sincos2pi(w):
     s, c = C.c_double(), C.c_double()
-    lib().or_bm_sincos2pi(float(u), C.byref(s), C.byref(c))
+    lib().or_bm_sincos2pi(int(w), C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def box_muller(words):
+    """[n, 4] Philox words -> [n, 4] standard normals (the engine's FP64 transform)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 4)
+    out = np.zeros(w.shape)
+    lib().or_box_muller(_p(w, C.c_uint32), w.shape[0], _p(out))
+    return out
 
 
 def neel_params(radius, Ms, alpha, gamma, T):

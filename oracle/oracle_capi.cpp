@@ -38,8 +38,11 @@ void or_normals(uint64_t seed, uint64_t particle, uint64_t first, uint64_t count
     }
 }
 
-double or_bm_log(double u) { return bm_log(u); }
-void or_bm_sincos2pi(double u, double* s, double* c) { bm_sincos2pi(u, s, c); }
+double or_bm_log(uint32_t w) { return bm_log(w); }
+void or_bm_sincos2pi(uint32_t w, double* s, double* c) { bm_sincos2pi(w, s, c); }
+void or_box_muller(const uint32_t* words, uint64_t n_blocks, double* out) {
+    for (uint64_t i = 0; i < n_blocks; ++i) box_muller4(words + 4 * i, out + 4 * i);
+}
 
 void or_neel_params(double r, double Ms, double alpha, double gamma, double T, double* out) {
     MagneticProperties mp{r, Ms, alpha, gamma, 0.0, 1};

@@ -454,35 +454,54 @@ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
     }
 }
 inline double u01(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }
-// log(u) by 2 atanh((m-1)/(m+1)) with coefficients 1/(2k+1); same operation sequence as the product
-inline double bm_log(double u) {
-    int e; double m = std::frexp(u, &e);
-    if (m < 0.7071067811865476) { m = m * 2.0; e -= 1; }
-    const double f = m - 1.0, s = f / (2.0 + f), t = s * s;
-    static const double A[11] = {0.043478260869565216, 0.047619047619047616, 0.05263157894736842, 0.058823529411764705,
-                                 0.06666666666666667, 0.07692307692307693, 0.09090909090909091, 0.1111111111111111,
-                                 0.14285714285714285, 0.2, 0.3333333333333333};
-    double P = A[0];
-    for (int i = 1; i < 11; ++i) P = std::fma(P, t, A[i]);
-    P = P * t;
-    const double lm = std::fma(-s, std::fma(-2.0, P, f), f);
-    const double de = (double)e;
-    return std::fma(de, 0.6931471805599453, std::fma(de, 2.3190468138462996e-17, lm));
+// The engine's FP64 Box-Muller (its own stream definition; the reference draws
+// pcg64 + Boost normals instead): ln u by a 97-entry table of ln(1 + j/128)
+// and 1/(1 + j/128) plus the series of ln(1 + r) to r^7; sin/cos(2 pi u) by a
+// 128-entry table of sin/cos((j + 1/2) 2 pi/128) and the angle-sum formulas
+// with the series of sin x (to x^7) and cos x - 1 (to x^6).  The tables are
+// computed here with the C library from the same double arguments.
+struct BoxMullerTables {
+    double sc[128][2], lg[97][2];
+    BoxMullerTables() {
+        const double step = TWO_PI / 128.0;
+        for (int j = 0; j < 128; ++j) { const double t = (j + 0.5) * step; sc[j][0] = std::sin(t); sc[j][1] = std::cos(t); }
+        for (int j = -32; j <= 64; ++j) { const double c = 1.0 + j / 128.0; lg[j + 32][0] = std::log(c); lg[j + 32][1] = 1.0 / c; }
+    }
+};
+inline const BoxMullerTables& bmTables() { static const BoxMullerTables t; return t; }
+inline double bm_log(uint32_t w) {
+    const double LN2_HI = 0.6931471803691238, LN2_LO = 1.9082149292705877e-10;
+    const double v = (double)w + 0.5;
+    int e; double m = std::frexp(v, &e) * 2.0; e -= 1;
+    if (m >= 1.5) { m = m * 0.5; e += 1; }
+    const double jd = std::rint((m - 1.0) * 128.0);
+    const double* T = bmTables().lg[(int)jd + 32];
+    const double c = std::fma(jd, 0.0078125, 1.0);
+    const double r = (m - c) * T[1];
+    static const double A[6] = {0.14285714285714285, -0.16666666666666666, 0.2, -0.25, 0.3333333333333333, -0.5};
+    double p = A[0];
+    for (int i = 1; i < 6; ++i) p = std::fma(p, r, A[i]);
+    const double l1p = std::fma(p, r * r, r);
+    const double E = (double)(e - 32);
+    return std::fma(E, LN2_HI, T[0] + std::fma(E, LN2_LO, l1p));
 }
-inline void bm_sincos2pi(double u, double* s, double* c) {
-    const double x = 4.0 * u, q = std::rint(x), r = x - q, r2 = r * r;
-    static const double SC[10] = {-4.377065467313742e-14, 6.0669357311061955e-12, -6.688035109811468e-10, 5.692172921967927e-08,
-                                  -3.598843235212085e-06, 0.00016044118478735983, -0.004681754135318688, 0.07969262624616705,
-                                  -0.6459640975062463, 1.5707963267948966};
-    static const double CC[11] = {3.437739179098607e-15, -5.294400200734623e-13, 6.565963114979473e-11, -6.386603083791852e-09,
-                                  4.710874778818172e-07, -2.5202042373060607e-05, 0.0009192602748394266, -0.02086348076335296,
-                                  0.25366950790104803, -1.2337005501361697, 1.0};
-    double S = SC[0]; for (int i = 1; i < 10; ++i) S = std::fma(S, r2, SC[i]);
-    S = S * r;
-    double C = CC[0]; for (int i = 1; i < 11; ++i) C = std::fma(C, r2, CC[i]);
-    const int iq = ((int)q) & 3;
-    if (iq == 0) { *s = S; *c = C; } else if (iq == 1) { *s = C; *c = -S; }
-    else if (iq == 2) { *s = -S; *c = -C; } else { *s = -C; *c = S; }
+inline void bm_sincos2pi(uint32_t w, double* s, double* c) {
+    const double* T = bmTables().sc[w >> 25];
+    const int d = (int)(w & 0x1FFFFFFu) - 0x1000000;
+    const double x = ((double)d + 0.5) * 1.4629180792671596e-09;
+    const double x2 = x * x;
+    double sp = std::fma(std::fma(-0.0001984126984126984, x2, 0.008333333333333333), x2, -0.16666666666666666);
+    const double sx = std::fma(sp * x2, x, x);
+    const double cp = std::fma(std::fma(-0.001388888888888889, x2, 0.041666666666666664), x2, -0.5);
+    const double cm1 = cp * x2;
+    *s = T[0] + std::fma(T[0], cm1, T[1] * sx);
+    *c = T[1] + std::fma(T[1], cm1, -(T[0] * sx));
+}
+inline void box_muller4(const uint32_t c[4], double z[4]) {
+    const double r0 = std::sqrt(-2.0 * bm_log(c[0])), r1 = std::sqrt(-2.0 * bm_log(c[2]));
+    double s0, c0, s1, c1;
+    bm_sincos2pi(c[1], &s0, &c0); bm_sincos2pi(c[3], &s1, &c1);
+    z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
 }
 inline void normals4(uint64_t seed, uint64_t particle, uint64_t call, uint32_t tag, double z[4], bool libm = false) {
     uint32_t c[4] = {(uint32_t)call, (uint32_t)(call >> 32), (uint32_t)particle, tag};
@@ -493,10 +512,7 @@ inline void normals4(uint64_t seed, uint64_t particle, uint64_t call, uint32_t t
         z[2] = r1 * std::cos(TWO_PI * u01(c[3])); z[3] = r1 * std::sin(TWO_PI * u01(c[3]));
         return;
     }
-    const double r0 = std::sqrt(-2.0 * bm_log(u01(c[0]))), r1 = std::sqrt(-2.0 * bm_log(u01(c[2])));
-    double s0, c0, s1, c1;
-    bm_sincos2pi(u01(c[1]), &s0, &c0); bm_sincos2pi(u01(c[3]), &s1, &c1);
-    z[0] = r0 * c0; z[1] = r0 * s0; z[2] = r1 * c1; z[3] = r1 * s1;
+    box_muller4(c, z);
 }
 
 // per-particle standard-normal source: Philox stream or replayed buffer

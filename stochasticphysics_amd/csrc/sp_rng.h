@@ -18,17 +18,22 @@
 // numbers: as easy as 1, 2, 3" (SC'11): round multipliers 0xD2511F53 and
 // 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85, ten rounds.
 //
-// The FP64 logarithm and sin/cos(2 pi u) below are written for this engine so
-// that host and device produce bit-identical normals: both only use IEEE
-// operations that are correctly rounded on x86-64 and on gfx950 (add, mul, fma,
-// div, sqrt, frexp, rint).  log uses m = 2^-e u in [sqrt(1/2), sqrt(2)),
-// s = (m-1)/(m+1) and log m = 2 atanh(s) summed as a series in s^2 with the
-// exact coefficients 1/(2k+1).  sin/cos reduce 4u exactly to a quadrant q and a
-// remainder r in [-1/2, 1/2] and sum the Taylor series of sin(pi r/2) and
-// cos(pi r/2); the coefficients are (pi/2)^n / n! rounded to double.
+// The FP64 ln u and sin/cos(2 pi u) are written for this engine so that host
+// and device produce bit-identical normals: they only use IEEE operations that
+// are correctly rounded on x86-64 and on gfx950 (add, mul, fma, sqrt, frexp,
+// rint, int<->double conversion) and the constant tables of sp_rng_tables.h,
+// and they work on the Philox word w itself, so the reduction is exact:
+//   ln u: v = w + 1/2 = m 2^e with m in [3/4, 3/2); j = rint(128 (m - 1)),
+//     c_j = 1 + j/128, r = (m - c_j) / c_j (|r| <= 1/192) and
+//     ln u = (e - 32) ln 2 + ln c_j + ln(1 + r), ln(1 + r) by its series to r^7;
+//   2 pi u = t_j + x with t_j = (j + 1/2) 2 pi / 128 for the top 7 bits j of w
+//     and |x| <= 2 pi / 256; sin x and cos x - 1 by their series (to x^7, x^6),
+//     combined with the tabulated sin t_j, cos t_j by the angle-sum formulas.
 #pragma once
 #include <stdint.h>
 #include <math.h>
+
+#include "sp_rng_tables.h"
 
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
@@ -65,77 +70,62 @@ SP_HD U4 noise_block(uint64_t seed, uint64_t particle, uint64_t call, uint32_t t
 
 SP_HD double u01_f64(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }  // (w+1/2) 2^-32
 
-// ---- FP64 log(u), u in (0, 1] ---------------------------------------------------
-SP_HD double log_f64(double u) {
+// ---- FP64 ln u, u = (w + 1/2) 2^-32 ----------------------------------------------
+SP_HD double log_u01_f64(uint32_t w) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    constexpr double LN2_HI = 0x1.62e42fee00000p-1;   // ln 2 to 32 significant bits: E * LN2_HI is exact
+    constexpr double LN2_LO = 0x1.a39ef35793c76p-33;  // ln 2 - LN2_HI
+    const double v = (double)w + 0.5;                 // exact
     int e;
-    double m = frexp(u, &e);                 // u = m 2^e, m in [1/2, 1)
-    if (m < 0.7071067811865476) { m = m * 2.0; e -= 1; }
-    const double f = m - 1.0;                // exact (Sterbenz)
-    const double s = f / (2.0 + f);
-    const double t = s * s;
-    double P = 0.043478260869565216;         // 1/23
-    P = fma(P, t, 0.047619047619047616);     // 1/21
-    P = fma(P, t, 0.05263157894736842);      // 1/19
-    P = fma(P, t, 0.058823529411764705);     // 1/17
-    P = fma(P, t, 0.06666666666666667);      // 1/15
-    P = fma(P, t, 0.07692307692307693);      // 1/13
-    P = fma(P, t, 0.09090909090909091);      // 1/11
-    P = fma(P, t, 0.1111111111111111);       // 1/9
-    P = fma(P, t, 0.14285714285714285);      // 1/7
-    P = fma(P, t, 0.2);                      // 1/5
-    P = fma(P, t, 0.3333333333333333);       // 1/3
-    P = P * t;                               // sum_{k>=1} t^k/(2k+1)
-    // log m = 2s(1+P) = f - s (f - 2P)   (using 2s = f - s f)
-    const double lm = fma(-s, fma(-2.0, P, f), f);
-    const double de = (double)e;
-    return fma(de, 0.6931471805599453, fma(de, 2.3190468138462996e-17, lm));
+    double m = frexp(v, &e) * 2.0;                    // v = m 2^(e-1), m in [1, 2)
+    e -= 1;
+    if (m >= 1.5) { m = m * 0.5; e += 1; }            // m in [3/4, 3/2)
+    const double jd = rint((m - 1.0) * 128.0);        // exact; in [-32, 64]
+    const double* T = SP_LOG_TAB[(int)jd + 32];       // ln c_j, 1 / c_j
+    const double c = fma(jd, 0.0078125, 1.0);         // c_j, exact
+    const double r = (m - c) * T[1];                  // m - c_j exact
+    double p = 0.14285714285714285;                   // 1/7
+    p = fma(p, r, -0.16666666666666666);
+    p = fma(p, r, 0.2);
+    p = fma(p, r, -0.25);
+    p = fma(p, r, 0.3333333333333333);
+    p = fma(p, r, -0.5);
+    const double l1p = fma(p, r * r, r);              // ln(1 + r)
+    const double E = (double)(e - 32);
+    return fma(E, LN2_HI, T[0] + fma(E, LN2_LO, l1p));
 }
 
-// ---- FP64 sin and cos of 2 pi u, u in [0, 1) -----------------------------------
-SP_HD void sincos2pi_f64(double u, double* s_out, double* c_out) {
-    const double x = 4.0 * u;                // exact
-    const double q = rint(x);                // quadrant, no ties possible for u = (w+1/2)2^-32
-    const double r = x - q;                  // exact, |r| <= 1/2
-    const double r2 = r * r;
-    double S = -4.377065467313742e-14;
-    S = fma(S, r2, 6.0669357311061955e-12);
-    S = fma(S, r2, -6.688035109811468e-10);
-    S = fma(S, r2, 5.692172921967927e-08);
-    S = fma(S, r2, -3.598843235212085e-06);
-    S = fma(S, r2, 0.00016044118478735983);
-    S = fma(S, r2, -0.004681754135318688);
-    S = fma(S, r2, 0.07969262624616705);
-    S = fma(S, r2, -0.6459640975062463);
-    S = fma(S, r2, 1.5707963267948966);
-    S = S * r;                               // sin(pi r / 2)
-    double C = 3.437739179098607e-15;
-    C = fma(C, r2, -5.294400200734623e-13);
-    C = fma(C, r2, 6.565963114979473e-11);
-    C = fma(C, r2, -6.386603083791852e-09);
-    C = fma(C, r2, 4.710874778818172e-07);
-    C = fma(C, r2, -2.5202042373060607e-05);
-    C = fma(C, r2, 0.0009192602748394266);
-    C = fma(C, r2, -0.02086348076335296);
-    C = fma(C, r2, 0.25366950790104803);
-    C = fma(C, r2, -1.2337005501361697);
-    C = fma(C, r2, 1.0);                     // cos(pi r / 2)
-    const int iq = ((int)q) & 3;
-    double s, c;
-    if (iq == 0)      { s = S;  c = C;  }
-    else if (iq == 1) { s = C;  c = -S; }
-    else if (iq == 2) { s = -S; c = -C; }
-    else              { s = -C; c = S;  }
-    *s_out = s;
-    *c_out = c;
+// ---- FP64 sin and cos of 2 pi u, u = (w + 1/2) 2^-32 ------------------------------
+SP_HD void sincos2pi_u01_f64(uint32_t w, double* s_out, double* c_out) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+    constexpr double TWO_PI_2M32 = 0x1.921fb54442d18p-30;   // (nearest double to 2 pi) 2^-32
+    const double* T = SP_SINCOS_TAB[w >> 25];              // sin t_j, cos t_j
+    const int d = (int)(w & 0x1FFFFFFu) - 0x1000000;       // u = (j + 1/2)/128 + (d + 1/2) 2^-32
+    const double x = ((double)d + 0.5) * TWO_PI_2M32;      // |x| <= 2 pi / 256
+    const double x2 = x * x;
+    double sp = -0.0001984126984126984;                    // -1/5040
+    sp = fma(sp, x2, 0.008333333333333333);
+    sp = fma(sp, x2, -0.16666666666666666);
+    const double sx = fma(sp * x2, x, x);                  // sin x
+    double cp = -0.001388888888888889;                     // -1/720
+    cp = fma(cp, x2, 0.041666666666666664);
+    cp = fma(cp, x2, -0.5);
+    const double cm1 = cp * x2;                            // cos x - 1
+    *s_out = T[0] + fma(T[0], cm1, T[1] * sx);
+    *c_out = T[1] + fma(T[1], cm1, -(T[0] * sx));
 }
 
 // four standard normals (FP64) from one Philox block
 SP_HD void box_muller4_f64(U4 w, double z[4]) {
-    const double r0 = sqrt(-2.0 * log_f64(u01_f64(w.x)));
-    const double r1 = sqrt(-2.0 * log_f64(u01_f64(w.z)));
+    const double r0 = sqrt(-2.0 * log_u01_f64(w.x));
+    const double r1 = sqrt(-2.0 * log_u01_f64(w.z));
     double s0, c0, s1, c1;
-    sincos2pi_f64(u01_f64(w.y), &s0, &c0);
-    sincos2pi_f64(u01_f64(w.w), &s1, &c1);
+    sincos2pi_u01_f64(w.y, &s0, &c0);
+    sincos2pi_u01_f64(w.w, &s1, &c1);
     z[0] = r0 * c0;
     z[1] = r0 * s0;
     z[2] = r1 * c1;

@@ -82,6 +82,51 @@ def test_host_box_muller_is_the_stream_transform():
     assert np.array_equal(host_box_muller(words).ravel(), host_normals(seed, part, 0, 256))
 
 
+def table_words():
+    """Words that visit every sin/cos table row (top 7 bits) and every ln table row (c_j = 1 + j/128,
+    j = -32..64, at several binades), plus both ends of each reduction interval."""
+    rng = np.random.default_rng(6)
+    top = (np.arange(128, dtype=np.uint64) << 25)
+    sc = np.concatenate([top, top | 0x1FFFFFF, top | 0x1000000, top | rng.integers(0, 1 << 25, 128, dtype=np.uint64)])
+    lg = []
+    for k in (1, 8, 20, 30, 31):
+        for j in range(-32, 65):
+            for off in (-0.49 / 128, 0.0, 0.49 / 128):
+                w = int(round(2.0 ** k * (1.0 + j / 128.0 + off)))
+                if 0 <= w < 2 ** 32:
+                    lg.append(w)
+    lg = np.array(lg + [0, 1, 2, 3, 2 ** 32 - 1, 2 ** 32 - 2, 2 ** 31 - 1, 2 ** 31], dtype=np.uint64)
+    n = max(len(sc), len(lg))
+    sc = np.resize(sc, n).astype(np.uint32)
+    lg = np.resize(lg, n).astype(np.uint32)
+    return np.stack([lg, sc, lg[::-1], sc[::-1]], 1)
+
+
+def test_host_box_muller_equals_oracle_on_every_table_row():
+    """The library's literal tables (sp_rng_tables.h) and the oracle's tables computed with the C library give
+    bit-identical normals on words that use every row of both tables."""
+    w = table_words()
+    assert np.array_equal(host_box_muller(w), oracle.box_muller(w))
+
+
+def test_box_muller_pieces_against_high_precision():
+    """ln u against a 40-digit decimal logarithm (relative 4e-16, including u -> 1 where ln u ~ -1e-10), and
+    sin/cos(2 pi u) against libm on the exactly reduced angle (absolute 1e-15)."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    w = table_words().ravel()
+    for x in np.unique(w):
+        u = (Decimal(int(x)) + Decimal("0.5")) / Decimal(2 ** 32)
+        ref = float(u.ln())
+        assert abs(oracle.bm_log(int(x)) - ref) <= 4e-16 * abs(ref), (x, oracle.bm_log(int(x)), ref)
+    for x in np.unique(w)[::7]:
+        k = int(x)
+        s, c = oracle.bm_sincos2pi(k)
+        frac = ((k + 0.5) / 2 ** 32) % 1.0              # exact
+        ang = 2 * math.pi * (frac if frac < 0.5 else frac - 1.0)
+        assert abs(s - math.sin(ang)) <= 1e-15 and abs(c - math.cos(ang)) <= 1e-15, (k, s, c)
+
+
 def test_particle_params_match_oracle():
     e = configs.config2(n=16, n_steps=200, oversampling=50)
     P = compute_params(e)
