@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 1
+#define SP_ABI_VERSION 2
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -93,7 +93,8 @@ typedef struct sp_request {
     double dt;                /* Timestep [s] */
     double newton_tol;        /* Implicit_Midpoint Accuracy_Goal (SolverSettings.h:122) */
     double min_angle;         /* NeelSpherical Min_angle_transformation [rad] */
-    double check_tol;         /* mixed precision: max |<m>_mixed - <m>_fp64| on the shadow subset */
+    double check_tol;         /* mixed precision: tolerated bias of <m>_mixed - <m>_fp64 (0: 1e-3; a negative
+                                 value fails every check, which forces the FP64 re-run) */
     /* field parameters (Properties/Fields/Sinusoidal.hpp:15-35, Lissajous.hpp:15-37):
      * sinusoidal: H = amplitude*sin(2 pi freq[0] t + phase[0]) + offset
      * lissajous:  H_i = offset_i + amplitude_i*sin(2 pi freq_i t + phase_i) */
@@ -143,7 +144,12 @@ typedef struct sp_result {
     double device_ms;          /* device time spent on this request's kernels (batch share) */
     double latency_ms;         /* submit -> result ready */
     double particle_steps;     /* particles * steps executed */
-    double check_error;        /* mixed: max abs error of the shadow-subset mean vs FP64 (-1 if not checked) */
+    double check_error;        /* mixed: max over outputs of |<m>_mixed - <m>_FP64| on the check subset, the
+                                  request's first 1/8 (512..4096 particles) re-run in FP64 (-1 if not checked) */
+    double check_noise;        /* mixed: max over outputs of 3 standard errors of that difference, from its
+                                  spread over the subset's 64-particle tiles.  The check fails, and the request
+                                  is re-run in FP64, where the time-averaged difference exceeds check_tol by
+                                  more than its 99.9% margin, or where the mixed result is not finite */
     uint32_t used_fallback;    /* 1 if the mixed result failed the check and was recomputed in FP64 */
     uint32_t precision_used;   /* SP_PREC_* of the returned result */
     uint64_t newton_failures;  /* ImplicitMidpoint steps that hit Max_Iterations */

@@ -54,7 +54,8 @@ class Result(C.Structure):
                 ("weight", C.POINTER(C.c_double)), ("final_state", C.POINTER(C.c_double)),
                 ("n_out", C.c_uint64), ("dim_out", C.c_uint32), ("dim_y", C.c_uint32),
                 ("device_ms", C.c_double), ("latency_ms", C.c_double), ("particle_steps", C.c_double),
-                ("check_error", C.c_double), ("used_fallback", C.c_uint32), ("precision_used", C.c_uint32),
+                ("check_error", C.c_double), ("check_noise", C.c_double), ("used_fallback", C.c_uint32),
+                ("precision_used", C.c_uint32),
                 ("newton_failures", C.c_uint64)]
 
 

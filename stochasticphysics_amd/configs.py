@@ -115,7 +115,7 @@ def config4_sets(n_sets=512, seed=44, precision="mixed", particle_range=(1e3, 1.
     return sets
 
 
-def serving_request(rng, idx, n_range=(256, 8192), step_range=(10_000, 50_000), oversampling=100, precision="mixed"):
+def serving_request(rng, idx, n_range=(256, 4096), step_range=(2_000, 10_000), oversampling=100, precision="mixed"):
     """One request of the serving loop: config-3 physics (NeelSpherical, cubic, Lissajous) with damping
     U[0.01, 0.5]; damping >= 0.3 switches to Implicit_Midpoint (FP64 Newton, tolerance 1e-10)."""
     n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))

@@ -80,6 +80,19 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     return SP_OK;
 }
 
+// kernel class of a request; EulerMaruyama_Normalized runs the EulerMaruyama kernels with the
+// per-step normalize switched on
+inline uint32_t kind_for(const sp_request& rq, bool mixed) {
+    const int ks = rq.solver == SP_SOLVER_EULER_MARUYAMA_NORMALIZED ? SP_SOLVER_EULER_MARUYAMA : rq.solver;
+    return kernel_kind(rq.problem, ks, rq.anisotropy, mixed ? 1 : 0);
+}
+
+// 64-particle tiles covered by the mixed-precision check: the request's first 1/8, at least 8 and
+// at most 64 tiles (512..4096 particles), never more than the request has
+inline uint32_t check_tiles_for(uint32_t n_tiles) {
+    return std::min<uint32_t>(n_tiles, std::max<uint32_t>(8u, std::min<uint32_t>(64u, n_tiles / 8)));
+}
+
 inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostRequest& hr) {
     if (int e = validate(rq, pt)) return e;
     hr.rq = *rq;
@@ -89,9 +102,7 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.total_steps = (hr.n_out - 1) * rq->oversampling;
     hr.n_tiles = (uint32_t)((hr.N + 63) / 64);
     hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT;
-    // EulerMaruyama_Normalized runs the EulerMaruyama kernels with the per-step normalize switched on
-    const int ksolver = rq->solver == SP_SOLVER_EULER_MARUYAMA_NORMALIZED ? SP_SOLVER_EULER_MARUYAMA : rq->solver;
-    hr.kind = kernel_kind(rq->problem, ksolver, rq->anisotropy, hr.mixed ? 1 : 0);
+    hr.kind = kind_for(*rq, hr.mixed);
     hr.params.assign((size_t)NPARAM * hr.N, 0.0);
     compute_params(*rq, *pt, hr.params.data(), hr.N);
     hr.y0.resize((size_t)hr.dy * hr.N);
@@ -103,7 +114,7 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
         hr.rq.replay_count = K;
     }
     hr.rq.replay_normals = nullptr;
-    hr.n_check_tiles = std::min<uint32_t>(hr.n_tiles, 4);
+    hr.n_check_tiles = check_tiles_for(hr.n_tiles);
     const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
     for (uint32_t c = 0; c < hr.dout; ++c) { hr.wsum[c] = 0.0; hr.wsum_check[c] = 0.0; }
     for (uint64_t p = 0; p < hr.N; ++p) {
@@ -123,14 +134,19 @@ struct ReqDev {
     double* state = nullptr;
     double* replay = nullptr;
     double* out = nullptr;        // [n_out*dout] weighted sums
-    double* check = nullptr;      // [n_out*dout] weighted sums over the first n_check tiles
+    double* check = nullptr;      // [n_out*dout][n_check_tiles] weighted sums of each of the first tiles
     unsigned long long* newton_fail = nullptr;
     DevRequest* dreq = nullptr;   // device copy of the request descriptor (plain path)
     size_t bytes = 0;
 };
 
 inline size_t request_bytes(const HostRequest& hr) {
-    return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 + 2 * hr.n_out * hr.dout * 8 + 64 + sizeof(DevRequest);
+    return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 +
+           (1 + (size_t)std::max<uint32_t>(1, hr.n_check_tiles)) * hr.n_out * hr.dout * 8 + 64 + sizeof(DevRequest);
+}
+
+inline size_t check_bytes(const HostRequest& hr) {
+    return (size_t)hr.n_out * hr.dout * std::max<uint32_t>(1, hr.n_check_tiles) * 8;
 }
 
 inline void free_dev(ReqDev& d) {
@@ -155,7 +171,7 @@ inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
     if ((e = hipMalloc(&d.state, hr.y0.size() * 8)) != hipSuccess) return fail("state", e);
     if (!hr.replay.empty() && (e = hipMalloc(&d.replay, hr.replay.size() * 8)) != hipSuccess) return fail("replay", e);
     if ((e = hipMalloc(&d.out, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("out", e);
-    if ((e = hipMalloc(&d.check, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("check", e);
+    if ((e = hipMalloc(&d.check, check_bytes(hr))) != hipSuccess) return fail("check", e);
     if ((e = hipMalloc(&d.newton_fail, 8)) != hipSuccess) return fail("counter", e);
     if ((e = hipMalloc(&d.dreq, sizeof(DevRequest))) != hipSuccess) return fail("descriptor", e);
     d.bytes = request_bytes(hr);
@@ -163,7 +179,7 @@ inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
     SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
     if (!hr.replay.empty()) SP_HIP(hipMemcpyAsync(d.replay, hr.replay.data(), hr.replay.size() * 8, hipMemcpyHostToDevice, s));
     SP_HIP(hipMemsetAsync(d.out, 0, hr.n_out * hr.dout * 8, s));
-    SP_HIP(hipMemsetAsync(d.check, 0, hr.n_out * hr.dout * 8, s));
+    SP_HIP(hipMemsetAsync(d.check, 0, check_bytes(hr), s));
     SP_HIP(hipMemsetAsync(d.newton_fail, 0, 8, s));
     return SP_OK;
 }
@@ -205,17 +221,22 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
 
 // ------------------------------------------------------------------ results
 struct Outputs {
-    std::vector<double> time, field, mean, weight, final_state, check_mean;
-    double device_ms = 0.0, latency_ms = 0.0, particle_steps = 0.0, check_error = -1.0;
+    std::vector<double> time, field, mean, weight, final_state;
+    std::vector<double> check_tiles;   // [n_out*dout][n_check_tiles] raw weighted tile sums
+    double device_ms = 0.0, latency_ms = 0.0, particle_steps = 0.0, check_error = -1.0, check_noise = 0.0;
     uint32_t used_fallback = 0, precision_used = SP_PREC_FP64;
     uint64_t newton_failures = 0;
+    bool check_failed = false;
 };
 
 inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t s, bool keep_final, bool want_check, Outputs& o) {
     const uint64_t M = hr.n_out * hr.dout;
-    std::vector<double> sums(M), csum;
+    std::vector<double> sums(M);
     SP_HIP(hipMemcpyAsync(sums.data(), d.out, M * 8, hipMemcpyDeviceToHost, s));
-    if (want_check) { csum.resize(M); SP_HIP(hipMemcpyAsync(csum.data(), d.check, M * 8, hipMemcpyDeviceToHost, s)); }
+    if (want_check) {
+        o.check_tiles.resize(check_bytes(hr) / 8);
+        SP_HIP(hipMemcpyAsync(o.check_tiles.data(), d.check, check_bytes(hr), hipMemcpyDeviceToHost, s));
+    }
     unsigned long long nf = 0;
     SP_HIP(hipMemcpyAsync(&nf, d.newton_fail, 8, hipMemcpyDeviceToHost, s));
     if (keep_final) {
@@ -234,12 +255,6 @@ inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t 
     o.mean.assign(M, 0.0);
     for (uint64_t k = 0; k < hr.n_out; ++k)
         for (uint32_t c = 0; c < hr.dout; ++c) o.mean[k * hr.dout + c] = zero ? sums[k * hr.dout + c] : sums[k * hr.dout + c] / hr.wsum[c];
-    if (want_check) {
-        o.check_mean.assign(M, 0.0);
-        for (uint64_t k = 0; k < hr.n_out; ++k)
-            for (uint32_t c = 0; c < hr.dout; ++c)
-                o.check_mean[k * hr.dout + c] = hr.wsum_check[c] != 0.0 ? csum[k * hr.dout + c] / hr.wsum_check[c] : 0.0;
-    }
     o.weight.assign(hr.wsum, hr.wsum + hr.dout);
     o.time.resize(hr.n_out);
     o.field.resize(hr.n_out * 3);
@@ -262,6 +277,7 @@ inline void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
     r->latency_ms = o.latency_ms;
     r->particle_steps = o.particle_steps;
     r->check_error = o.check_error;
+    r->check_noise = o.check_noise;
     r->used_fallback = o.used_fallback;
     r->precision_used = o.precision_used;
     r->newton_failures = o.newton_failures;
@@ -301,6 +317,12 @@ struct sp_ctx_impl;
 struct sp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // kernel classes of one scheduler iteration run concurrently: class k > 0 goes to side[(k-1) % 3],
+    // forked from and joined back into `stream` with events (all streams at the default priority)
+    static constexpr int N_SIDE = 3;
+    hipStream_t side[N_SIDE] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[N_SIDE] = {nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     std::mutex mu;                          // protects everything below
     std::map<uint64_t, sp_tuning> tuning;   // by (problem, solver, precision)

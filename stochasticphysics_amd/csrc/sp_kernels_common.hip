@@ -38,25 +38,19 @@ __global__ __launch_bounds__(256) void k_reduce(const ReduceWork* __restrict__ w
     const uint32_t r = blockIdx.x;
     if (r >= rw.n_rows) return;
     __shared__ double sh[256];
-    __shared__ double shc[256];
     const double* row = rw.partial + (uint64_t)r * rw.pstride;
-    double s = 0.0, sc = 0.0;
-    for (uint32_t i = threadIdx.x; i < rw.pstride; i += 256) {
-        const double v = row[i];
-        s += v;
-        if (i < rw.n_check) sc += v;
-    }
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < rw.pstride; i += 256) s += row[i];
+    // the check tiles are kept one by one (n_check <= 64 <= pstride)
+    if (rw.check_out && threadIdx.x < rw.n_check)
+        rw.check_out[(uint64_t)(rw.row_base + r) * rw.n_check + threadIdx.x] = row[threadIdx.x];
     sh[threadIdx.x] = s;
-    shc[threadIdx.x] = sc;
     __syncthreads();
     for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) { sh[threadIdx.x] += sh[threadIdx.x + w]; shc[threadIdx.x] += shc[threadIdx.x + w]; }
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        rw.out[rw.row_base + r] = sh[0];
-        if (rw.check_out) rw.check_out[rw.row_base + r] = shc[0];
-    }
+    if (threadIdx.x == 0) rw.out[rw.row_base + r] = sh[0];
 }
 
 void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows) {

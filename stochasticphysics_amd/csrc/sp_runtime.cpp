@@ -47,7 +47,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     const auto t0 = Clock::now();
     SP_HIP(hipMemcpyAsync(d.state, pr.state0, hr.y0.size() * 8, hipMemcpyDeviceToDevice, s));
     SP_HIP(hipMemsetAsync(d.out, 0, hr.n_out * hr.dout * 8, s));
-    SP_HIP(hipMemsetAsync(d.check, 0, hr.n_out * hr.dout * 8, s));
+    SP_HIP(hipMemsetAsync(d.check, 0, check_bytes(hr), s));
     SP_HIP(hipMemsetAsync(d.newton_fail, 0, 8, s));
     const DevRequest dr = make_dev_request(hr, d);
     SP_HIP(hipMemcpyAsync(d.dreq, &dr, sizeof(DevRequest), hipMemcpyHostToDevice, s));
@@ -180,6 +180,11 @@ int sp_ctx_create(int device, sp_ctx** out) {
     SP_HIP(hipSetDevice(device));
     SP_HIP(hipGetDeviceProperties(&c->prop, device));
     SP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int i = 0; i < sp_ctx::N_SIDE; ++i) {
+        SP_HIP(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+        SP_HIP(hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming));
+    }
+    SP_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     size_t free_b = 0, total_b = 0;
     SP_HIP(hipMemGetInfo(&free_b, &total_b));
     c->mem_budget = (uint64_t)(0.5 * (double)free_b);
@@ -346,6 +351,11 @@ extern "C" void sp_ctx_destroy(sp_ctx* ctx) {
     if (ctx->table) hipFree(ctx->table);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->counters) hipFree(ctx->counters);
+    for (int i = 0; i < sp_ctx::N_SIDE; ++i) {
+        if (ctx->side[i]) { hipStreamSynchronize(ctx->side[i]); hipStreamDestroy(ctx->side[i]); }
+        if (ctx->join_ev[i]) hipEventDestroy(ctx->join_ev[i]);
+    }
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

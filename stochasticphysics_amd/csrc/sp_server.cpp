@@ -13,9 +13,16 @@
 //  * sp_submit/sp_wait/sp_result_get: a worker thread admits requests by device
 //    memory footprint as slots free up and runs iterations back to back; new
 //    requests join at the next iteration boundary.
-//  * mixed precision: each mixed request gets an FP64 shadow over its first 256
-//    particles on the same Philox stream; the request's own mean over those
-//    particles is compared with it, and a failed check re-runs it in FP64.
+//  * mixed precision: each mixed request gets an FP64 shadow over its first
+//    1/8 (512..4096 particles) on the same Philox stream.  Per output and
+//    component, the difference of the two means over those particles and its
+//    standard error (from the spread over the 64-particle tiles) are formed;
+//    a request whose time-averaged difference exceeds check_tol by more than
+//    its 99.9% confidence margin (or that holds a non-finite value) is re-run
+//    in FP64.  Trajectories that pass a barrier or a pole can decorrelate
+//    between FP32 and FP64 under the same noise, so the difference is partly
+//    Monte-Carlo noise: the test asks for a bias beyond the tolerance, not for
+//    trajectory agreement.
 #include "sp_engine.h"
 
 namespace sp {
@@ -110,13 +117,25 @@ static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning&
 static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const SlotWork* d_work,
                             const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun) {
     if (!P.kinds.empty()) launch_zero_u32(s, d_counters, (uint32_t)P.kinds.size());
+    // fork: the classes of this iteration run side by side (the persistent launches of classes with few
+    // waves, such as FP64 shadows, would otherwise serialise behind the others)
+    const int n_side = (int)std::min<size_t>(P.kinds.size() > 0 ? P.kinds.size() - 1 : 0, (size_t)sp_ctx::N_SIDE);
+    if (n_side > 0) {
+        SP_HIP(hipEventRecord(ctx->fork_ev, s));
+        for (int i = 0; i < n_side; ++i) SP_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0));
+    }
     for (size_t k = 0; k < P.kinds.size(); ++k) {
         const sp_tuning& t = tun[k];
         const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(1024, t.block_size / 64 * 64));
         const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
-        if (!launch_batch(P.kinds[k], grid, dim3(block), s, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
+        hipStream_t sk = k == 0 ? s : ctx->side[(k - 1) % sp_ctx::N_SIDE];
+        if (!launch_batch(P.kinds[k], grid, dim3(block), sk, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
                           d_counters + k))
             return set_error(SP_ERR_UNSUPPORTED, "no batch kernel for a requested class");
+    }
+    for (int i = 0; i < n_side; ++i) {           // join
+        SP_HIP(hipEventRecord(ctx->join_ev[i], ctx->side[i]));
+        SP_HIP(hipStreamWaitEvent(s, ctx->join_ev[i], 0));
     }
     if (!P.reduce.empty()) launch_reduce(s, d_reduce, (uint32_t)P.reduce.size(), P.max_rows);
     return SP_OK;
@@ -172,26 +191,74 @@ static void make_shadow(const HostRequest& parent, HostRequest& sh) {
     sh.total_steps = parent.total_steps;
     sh.n_tiles = (uint32_t)((sh.N + 63) / 64);
     sh.mixed = false;
-    sh.kind = kernel_kind(parent.rq.problem, parent.rq.solver, parent.rq.anisotropy, 0);
+    sh.kind = kind_for(parent.rq, false);
     sh.params.resize((size_t)NPARAM * sh.N);
     for (uint32_t k = 0; k < NPARAM; ++k)
         for (uint64_t p = 0; p < sh.N; ++p) sh.params[k * sh.N + p] = parent.params[k * parent.N + p];
     sh.y0.resize((size_t)sh.dy * sh.N);
     for (uint32_t i = 0; i < sh.dy; ++i)
         for (uint64_t p = 0; p < sh.N; ++p) sh.y0[i * sh.N + p] = parent.y0[i * parent.N + p];
-    sh.n_check_tiles = 0;
-    for (uint32_t c = 0; c < sh.dout; ++c) sh.wsum[c] = parent.wsum_check[c];
+    sh.n_check_tiles = sh.n_tiles;              // the same tiles, kept one by one
+    for (uint32_t c = 0; c < sh.dout; ++c) { sh.wsum[c] = parent.wsum_check[c]; sh.wsum_check[c] = parent.wsum_check[c]; }
 }
 
-// Max |<m>_mixed - <m>_FP64| over the shadow subset; a non-finite value anywhere in the mixed
-// result (FP32 overflow of a run-away particle) counts as an infinite error.
-static double check_error_of(const Outputs& parent, const Outputs& shadow) {
-    for (double v : parent.mean)
-        if (!std::isfinite(v)) return HUGE_VAL;
-    double e = 0.0;
-    for (size_t i = 0; i < parent.check_mean.size() && i < shadow.mean.size(); ++i)
-        e = std::max(e, std::fabs(parent.check_mean[i] - shadow.mean[i]));
-    return e;
+// Two-sided 99.9% quantiles of Student's t with dof = 1, 2, ... (dof > 120: the normal quantile).
+static double t_quantile_999(uint32_t dof) {
+    static const double q[] = {636.62, 31.599, 12.924, 8.610, 6.869, 5.959, 5.408, 5.041, 4.781, 4.587,
+                               4.437, 4.318, 4.221, 4.140, 4.073, 4.015, 3.965, 3.922, 3.883, 3.850};
+    if (dof == 0) return HUGE_VAL;
+    if (dof <= 20) return q[dof - 1];
+    if (dof <= 30) return 3.646;
+    if (dof <= 40) return 3.551;
+    if (dof <= 60) return 3.460;
+    if (dof <= 120) return 3.373;
+    return 3.291;
+}
+
+// Check of a mixed result against its FP64 shadow over the T check tiles.  Per output row k and
+// component c, d_t = (mixed - fp64 weighted sum of tile t) / W and D = sum_t d_t (W = weight of the check
+// particles); its standard error SE = sqrt(T/(T-1) sum_t (d_t - D/T)^2).  Reported: error = max |D|,
+// noise = max 3 SE.  The test is on the time average: per component, the bias B = mean_k D over the
+// rows k >= 1 and its standard error from the per-tile time averages; the result fails where
+// |B| - t_999(T-1) SE(B) > tol (|B| > tol when T < 4), or where the mixed result is not finite (FP32
+// overflow of a run-away particle).  A negative tol fails every check.
+struct CheckResult { double error = 0.0, noise = 0.0; bool fail = false; };
+static CheckResult check_of(const HostRequest& hr, const Outputs& mixed, const Outputs& shadow) {
+    CheckResult r;
+    const double tol = hr.rq.check_tol != 0.0 ? hr.rq.check_tol : 1e-3;
+    for (double v : mixed.mean)
+        if (!std::isfinite(v)) { r.error = HUGE_VAL; r.fail = true; return r; }
+    const uint32_t T = hr.n_check_tiles;
+    if (T == 0 || mixed.check_tiles.size() < (size_t)hr.n_out * hr.dout * T ||
+        shadow.check_tiles.size() < (size_t)hr.n_out * hr.dout * T) { r.error = HUGE_VAL; r.fail = true; return r; }
+    std::vector<double> d(T), avg(T);
+    for (uint32_t c = 0; c < hr.dout; ++c) {
+        const double W = hr.wsum_check[c];
+        if (W == 0.0) continue;
+        std::fill(avg.begin(), avg.end(), 0.0);
+        for (uint64_t k = 0; k < hr.n_out; ++k) {
+            const double* a = &mixed.check_tiles[(k * hr.dout + c) * T];
+            const double* b = &shadow.check_tiles[(k * hr.dout + c) * T];
+            double D = 0.0;
+            for (uint32_t t = 0; t < T; ++t) { d[t] = (a[t] - b[t]) / W; D += d[t]; }
+            double ss = 0.0;
+            for (uint32_t t = 0; t < T; ++t) { const double x = d[t] - D / T; ss += x * x; }
+            const double se = T > 1 ? std::sqrt(ss * T / (T - 1)) : 0.0;
+            if (!std::isfinite(D) || !std::isfinite(se)) { r.error = HUGE_VAL; r.fail = true; return r; }
+            r.error = std::max(r.error, std::fabs(D));
+            r.noise = std::max(r.noise, 3.0 * se);
+            if (k >= 1) for (uint32_t t = 0; t < T; ++t) avg[t] += d[t];
+        }
+        if (hr.n_out < 2) continue;
+        double B = 0.0;
+        for (uint32_t t = 0; t < T; ++t) { avg[t] /= (double)(hr.n_out - 1); B += avg[t]; }
+        double ss = 0.0;
+        for (uint32_t t = 0; t < T; ++t) { const double x = avg[t] - B / T; ss += x * x; }
+        const double seB = T > 1 ? std::sqrt(ss * T / (T - 1)) : 0.0;
+        const double margin = T >= 4 ? t_quantile_999(T - 1) * seB : 0.0;
+        if (std::fabs(B) - margin > tol || tol < 0.0) r.fail = true;
+    }
+    return r;
 }
 
 static uint64_t chunk_for(const HostRequest& hr, const sp_tuning& t) {
@@ -322,8 +389,11 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
         outs[i].latency_ms = wall;
         if (sl->shadow) {
             Outputs so;
-            if (int e = download_outputs(*sl->shadow->hr, sl->shadow->dev, s, false, false, so)) return e;
-            outs[i].check_error = check_error_of(outs[i], so);
+            if (int e = download_outputs(*sl->shadow->hr, sl->shadow->dev, s, false, true, so)) return e;
+            const CheckResult cr = check_of(reqs[i], outs[i], so);
+            outs[i].check_error = cr.error;
+            outs[i].check_noise = cr.noise;
+            outs[i].check_failed = cr.fail;
         }
     }
     return SP_OK;
@@ -338,30 +408,32 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
     bool requeue = false;
     if (rc == SP_OK && sl->shadow) {
         Outputs so;
-        rc = download_outputs(*sl->shadow->hr, sl->shadow->dev, ctx->stream, false, false, so);
+        rc = download_outputs(*sl->shadow->hr, sl->shadow->dev, ctx->stream, false, true, so);
         if (rc == SP_OK) {
-            o.check_error = check_error_of(o, so);
-            const double tol = sl->hr->rq.check_tol > 0.0 ? sl->hr->rq.check_tol : 1e-3;
-            if (!(o.check_error <= tol) && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
+            const CheckResult cr = check_of(*sl->hr, o, so);
+            o.check_error = cr.error;
+            o.check_noise = cr.noise;
+            if (cr.fail && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
         }
     }
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (requeue) {
         // per-request FP64 fallback: run the request again in FP64 ahead of the queue
         job->out.check_error = o.check_error;
+        job->out.check_noise = o.check_noise;
         job->out.used_fallback = 1;
         job->hr.rq.precision = SP_PREC_FP64;
         job->hr.mixed = false;
-        job->hr.kind = kernel_kind(job->hr.rq.problem, job->hr.rq.solver, job->hr.rq.anisotropy, 0);
+        job->hr.kind = kind_for(job->hr.rq, false);
         ctx->pending.push_front(job);
         ctx->stat_fallbacks += 1;
         return;
     }
     if (rc == SP_OK) {
-        const double fallback_err = job->out.check_error;
+        const double fallback_err = job->out.check_error, fallback_noise = job->out.check_noise;
         const uint32_t fb = job->out.used_fallback;
         job->out = std::move(o);
-        if (fb) { job->out.used_fallback = 1; job->out.check_error = fallback_err; }
+        if (fb) { job->out.used_fallback = 1; job->out.check_error = fallback_err; job->out.check_noise = fallback_noise; }
         job->out.device_ms = job->device_ms_acc;
         job->out.latency_ms = ms_since(job->t_submit, Clock::now());
         job->status = SP_OK;
@@ -405,7 +477,8 @@ static void worker_main(sp_ctx* ctx) {
             while (!ctx->pending.empty()) {
                 auto& j = ctx->pending.front();
                 if (j->cancel) { j->status = SP_ERR_CANCELLED; ctx->pending.pop_front(); ctx->done_cv.notify_all(); continue; }
-                size_t need = request_bytes(j->hr) + (wants_shadow(j->hr) ? request_bytes(j->hr) / std::max<uint64_t>(1, j->hr.n_tiles) * 4 : 0);
+                size_t need = request_bytes(j->hr) +
+                              (wants_shadow(j->hr) ? request_bytes(j->hr) / std::max<uint64_t>(1, j->hr.n_tiles) * j->hr.n_check_tiles * 2 : 0);
                 size_t n_active = admitted.size();
                 for (auto& a : active) if (!a->is_shadow && !a->done) ++n_active;
                 if (n_active > 0 && (ctx->mem_used + need > ctx->mem_budget || n_active >= ctx->max_active)) break;
@@ -566,8 +639,7 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_p
     if (int e = run_batch_static(ctx, hrs, outs, keep_final, use_graph)) return e;
     // per-request FP64 fallback for mixed results that failed the shadow check
     for (uint32_t i = 0; i < n_req; ++i) {
-        const double tol = hrs[i].rq.check_tol > 0.0 ? hrs[i].rq.check_tol : 1e-3;
-        if (outs[i].check_error >= 0.0 && !(outs[i].check_error <= tol) && !(hrs[i].rq.flags & SP_FLAG_NO_FALLBACK)) {
+        if (outs[i].check_failed && !(hrs[i].rq.flags & SP_FLAG_NO_FALLBACK)) {
             HostRequest h2;
             sp_request r2 = reqs[i];
             r2.precision = SP_PREC_FP64;
@@ -580,6 +652,7 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_p
             }
             if (int e = run_plain(ctx, h2, o2, keep_final, t)) return e;
             o2.check_error = outs[i].check_error;
+            o2.check_noise = outs[i].check_noise;
             o2.used_fallback = 1;
             outs[i] = std::move(o2);
         }

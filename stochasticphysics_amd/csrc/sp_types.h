@@ -77,11 +77,11 @@ inline constexpr uint32_t kernel_kind(int problem, int solver, int aniso, int mi
 struct ReduceWork {
     const double* partial;     // rows of pstride doubles
     uint32_t pstride;
-    uint32_t n_check;          // leading slots also summed into check_out (FP64 shadow comparison)
+    uint32_t n_check;          // leading slots also copied one by one into check_out (mixed-precision check)
     uint32_t n_rows;           // rows (windows * DOUT)
     uint32_t row_base;         // first output row
     double* out;               // out[row_base + r]
-    double* check_out;         // nullable
+    double* check_out;         // nullable; check_out[(row_base + r) * n_check + slot]
 };
 
 }  // namespace sp

@@ -156,7 +156,8 @@ def _alloc_result(e: Ensemble, keep_final):
 
 def _finish_result(res: Result, o):
     o.update(device_ms=res.device_ms, latency_ms=res.latency_ms, particle_steps=res.particle_steps,
-             check_error=res.check_error, used_fallback=res.used_fallback, precision_used=res.precision_used,
+             check_error=res.check_error, check_noise=res.check_noise, used_fallback=res.used_fallback,
+             precision_used=res.precision_used,
              newton_failures=res.newton_failures)
     return o
 

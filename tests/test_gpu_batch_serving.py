@@ -68,15 +68,29 @@ def test_batch_mixed_precision_against_oracle(ctx):
         err = float(np.max(np.abs(r["mean"] - o["mean"])))
         assert r["precision_used"] == 1 and not r["used_fallback"]
         assert err <= 1e-3, err
-        # the FP64 shadow check measured the same kind of error on its 256-particle subset
-        assert 0.0 <= r["check_error"] <= 1e-3
+        # the FP64 shadow check measured the same kind of error on its subset (here the whole request), and
+        # its statistical resolution (3 standard errors over the 64-particle tiles) is stated beside it
+        assert 0.0 <= r["check_error"] <= 1e-3 and r["check_noise"] >= 0.0
+
+
+def test_check_accepts_decorrelated_but_unbiased_mixed(ctx):
+    """Moments released on an unstable point (in the reference's convention K > 0 makes the +z axis hard, so the
+    config-1 start is a maximum of the energy) fall off it differently in FP32 and FP64 under the same noise: the
+    two means differ by Monte-Carlo noise well above 1e-3 per output, with no bias.  The check states that error
+    and its noise level and keeps the mixed result instead of re-running it in FP64."""
+    e = configs.config1(n=8192, n_steps=200_000, oversampling=100, precision="mixed")
+    r = ctx.run_batch([e])[0]
+    assert r["used_fallback"] == 0 and r["precision_used"] == 1
+    assert r["check_error"] > 1e-3 and r["check_noise"] > 0.0
+    f = ctx.run(configs.config1(n=8192, n_steps=200_000, oversampling=100))
+    assert np.all(np.isfinite(r["mean"])) and np.abs(r["mean"] - f["mean"]).max() > 1e-3
 
 
 def test_forced_fp64_fallback(ctx):
     e = neel_ensemble(n=256, n_steps=2000, solver="Heun_Strong", field="sinusoidal", seed=41)
     ref = ctx.run(e)                       # FP64 plain
     e.precision = "mixed"
-    e.check_tol = 1e-300                   # every mixed result fails the check
+    e.check_tol = -1.0                     # a negative tolerance fails every check
     r = ctx.run_batch([e])[0]
     assert r["used_fallback"] == 1 and r["precision_used"] == 0 and r["check_error"] > 0
     same(r, ref)
