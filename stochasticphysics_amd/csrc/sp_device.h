@@ -74,7 +74,10 @@ struct FieldCache {
     uint64_t base;
     double hx, hy, hz;
     bool active;
-    __device__ __forceinline__ void init(const DevRequest& rq) { base = ~0ull; hx = hy = hz = 0.0; active = rq.field_type != 1; }
+    __device__ __forceinline__ void init(const DevRequest& rq) {
+        base = ~0ull; hx = hy = hz = 0.0;
+        active = __builtin_amdgcn_readfirstlane(rq.field_type) != 1;   // uniform
+    }
     // make steps [n, n + count) available (count <= 64, 64 % count == 0, n aligned to count)
     template <bool MIDPOINT>
     __device__ __forceinline__ void ensure(const DevRequest& rq, uint64_t n, uint32_t count) {
@@ -225,9 +228,9 @@ template <int ANISO, class R> struct BNLane {
     R pf, pt;                 // -2K/Ms, -2K V
     R a, bpd, c, d;           // a dt, (b+d) dt, c dt, d dt
     R cFs, dsig, bpds, as;    // cF sqrt(dt), d sigma sqrt(dt), (b+d|b) sigma sqrt(dt), a sigma sqrt(dt)
+    R cFm;                    // cF sqrt(dt) in the full model, 0 in the simple one
     R kn1, kn2, kn3, km;      // drift coefficients times dt
     double w;
-    bool simple;
 };
 template <int ANISO, class R>
 __device__ __forceinline__ void bn_load(const DevRequest& rq, uint64_t p, BNLane<ANISO, R>& L) {
@@ -235,10 +238,12 @@ __device__ __forceinline__ void bn_load(const DevRequest& rq, uint64_t p, BNLane
     const double c = ld(rq, P_C, p), cF = ld(rq, P_CF, p), d = ld(rq, P_DD, p);
     const bool simple = (rq.flags & RQ_BN_SIMPLE) != 0;
     const double dt = rq.dt, sq = rq.sqdt, s2 = s * s, cF2 = cF * cF, bpd = b + d;
-    L.simple = simple;
     L.pf = (R)ld(rq, P_K1, p); L.pt = (R)ld(rq, P_PT, p);
     L.a = (R)(a * dt); L.bpd = (R)(bpd * dt); L.c = (R)(c * dt); L.d = (R)(d * dt);
-    L.cFs = (R)(cF * sq); L.dsig = (R)(d * s * sq); L.as = (R)(a * s * sq);
+    // the simple model drops the coupling terms (.inl:114-170, :179-198); its coefficients are zero here so
+    // that the step has no branch: the full model's extra terms then add exact zeros
+    L.cFs = (R)(cF * sq); L.dsig = (R)(simple ? 0.0 : d * s * sq); L.as = (R)(a * s * sq);
+    L.cFm = (R)(simple ? 0.0 : cF * sq);
     L.bpds = (R)((simple ? b : bpd) * s * sq);
     L.kn1 = (R)(-cF2 * dt);
     L.kn2 = (R)(simple ? 0.0 : s2 * d * a * dt);
@@ -263,19 +268,14 @@ __device__ __forceinline__ void bn_f(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
         fm = {(R)0, (R)0, (R)0};
     }
     if (with_drift) {
-        fn = fn + L.kn1 * n;
-        if (!L.simple) fn = fn + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
+        fn = fn + L.kn1 * n + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
         fm = fm + L.km * m;
     }
 }
 template <int ANISO, class R>
 __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m, V3<R> zF, V3<R> zH, V3<R>& sn, V3<R>& sm) {
-    sn = L.cFs * cross(zF, n);
-    sm = L.bpds * (zH - dot(m, zH) * m) + L.as * cross(m, zH);
-    if (!L.simple) {
-        sn = sn + L.dsig * (dot(n, m) * zH - dot(n, zH) * m);
-        sm = sm + L.cFs * cross(zF, m);
-    }
+    sn = L.cFs * cross(zF, n) + L.dsig * (dot(n, m) * zH - dot(n, zH) * m);
+    sm = L.bpds * (zH - dot(m, zH) * m) + L.as * cross(m, zH) + L.cFm * cross(zF, m);
 }
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
@@ -311,8 +311,8 @@ template <int ANISO, int SOLVER, class R> struct BNCfg {
             bn_s<ANISO>(L, n, m, zF, zH, sn, sm);
             // drift (dW.dW) = drift dt (z.z) and the drift coefficients already carry dt
             const R zz = dot(zF, zF) + dot(zH, zH);
-            dn = L.kn1 * n; dm = L.km * m;
-            if (!L.simple) dn = dn + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
+            dn = L.kn1 * n + L.kn2 * cross(n, m) + (L.kn3 * (dot(m, m) - (R)1)) * n;
+            dm = L.km * m;
             in = fn + sn + zz * dn;
             im = fm + sm + zz * dm;
         }
@@ -719,6 +719,8 @@ __device__ __forceinline__ void normalize_blocks(double* y) {
     }
 }
 
+template <bool B> struct BoolTag { static constexpr bool value = B; };
+
 // ================================================================ tile driver
 // Runs steps [tw.step_begin, tw.step_end) of one 64-particle tile
 // (SingleParticleSimulator::startSimulation, :129-159, for 64 particles at
@@ -749,10 +751,12 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     const uint32_t OS = rq.oversampling;
     uint32_t row = 0;
     auto flush = [&](const double* v) {
+        double sums[DOUT];
 #pragma unroll
-        for (int c = 0; c < DOUT; ++c) {
-            const double s = wave_sum(v[c] * w[c]);
-            if (lane == 0) tw.partial[((uint64_t)row * DOUT + c) * tw.pstride + tw.pslot] = s;
+        for (int c = 0; c < DOUT; ++c) sums[c] = wave_sum(v[c] * w[c]);
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < DOUT; ++c) tw.partial[((uint64_t)row * DOUT + c) * tw.pstride + tw.pslot] = sums[c];
         }
         ++row;
     };
@@ -763,7 +767,9 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     }
     FieldCache fc;
     fc.init(rq);
-    const bool renorm = Cfg::NORM > 0 && (rq.flags & RQ_NORMALIZE) != 0;
+    const uint32_t flags = __builtin_amdgcn_readfirstlane(rq.flags);    // uniform: scalar branches only
+    const bool renorm = Cfg::NORM > 0 && (flags & RQ_NORMALIZE) != 0;
+    const bool replay = (flags & RQ_REPLAY) != 0;
     double acc[DOUT], comp[DOUT];
 #pragma unroll
     for (int c = 0; c < DOUT; ++c) { acc[c] = 0.0; comp[c] = 0.0; }
@@ -787,18 +793,21 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
         for (int c = 0; c < DOUT; ++c) { mean[c] = acc[c] / (double)OS; acc[c] = 0.0; comp[c] = 0.0; }
         flush(mean);
     };
-    if (OS % G == 0) {
-        // Windows end on group boundaries: each group is straight-line code, and the
-        // next group's Philox + Box-Muller is issued alongside this group's physics.
+    // Fast path (Philox noise, windows ending on group boundaries): each group is straight-line code, the
+    // next group's Philox + Box-Muller is issued alongside this group's physics, and the per-step
+    // normalisation is decided outside the loop.
+    auto fast = [&](auto renorm_tag) {
+        constexpr bool RN = decltype(renorm_tag)::value;
+        const uint64_t seed = rq.seed;
         R zc[12], zn[12];
-        if (tw.step_begin < tw.step_end) normals12<R>(rq, p, tw.step_begin * DW, zc);
+        if (tw.step_begin < tw.step_end) NormalsT<R>::philox(seed, p, (tw.step_begin * DW) >> 2, zc);
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             fc.ensure<MIDPOINT>(rq, n0, G);
-            normals12<R>(rq, p, (n0 + G) * DW, zn);
+            NormalsT<R>::philox(seed, p, ((n0 + G) * DW) >> 2, zn);
 #pragma unroll
             for (int s = 0; s < G; ++s) {
                 Cfg::step(rq, L, y, cache, zc + s * DW, fc.at(n0 + s));
-                if (renorm) normalize_blocks<Cfg::NORM>(y);
+                if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
                 double o[DOUT];
                 Cfg::output(L, y, cache, o);
                 accumulate(o);
@@ -808,6 +817,10 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
 #pragma unroll
             for (int j = 0; j < 12; ++j) zc[j] = zn[j];
         }
+    };
+    if (OS % G == 0 && !replay) {
+        if (renorm) fast(BoolTag<true>{});
+        else fast(BoolTag<false>{});
     } else {
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             R z[12];
