@@ -173,6 +173,60 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
                               "mixed (Heun_Strong) or FP64 Implicit_Midpoint for damping >= 0.3")
 
 
+def precision_study(ctx, configs, n, n_steps):
+    """NeelSpherical, cubic K1 = -1.1e4, log-normal radii, Lissajous field (the precision-study ensemble):
+    FP64 and mixed on the same Philox stream, Heun_Strong and Millstein; throughput and <m> error."""
+    out = {}
+    for solver in ("Heun_Strong", "Millstein"):
+        row = {}
+        res = {}
+        for prec in ("fp64", "mixed"):
+            e = configs.config3(n=n, n_steps=n_steps, solver=solver, precision=prec)
+            e.skip_check = True
+            p = ctx.prepare(e)
+            t0 = time.perf_counter()
+            res[prec] = p.execute()
+            dt = time.perf_counter() - t0
+            p.release()
+            row[prec] = e.particle_steps() / dt
+            log(f"bench: precision study {solver} {prec}: {row[prec]:.3e} particle-steps/s ({dt:.1f} s)")
+        d = res["mixed"]["mean"] - res["fp64"]["mean"]
+        row.update(err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
+                   speedup=row["mixed"] / row["fp64"], monte_carlo_sigma_over_sqrt_n=float(0.58 / math.sqrt(n)))
+        log(f"bench: precision study {solver}: mixed - FP64 <m> max {row['err_max']:.2e} rms {row['err_rms']:.2e}")
+        out[solver] = row
+    return dict(workload=f"NeelSpherical cubic, {n} particles x {(n_steps // 100 - 1) * 100} steps, dt 2e-12 s, "
+                         "Lissajous 12 mT at 25.25/24.5 kHz, output every 100 steps", unit="particle-steps/s",
+                results=out)
+
+
+def neel_relaxation(ctx, configs):
+    """Plain Neel relaxation, 4,096 uniaxial particles, EulerMaruyama, 2e5 steps (the reference's CPU-sized
+    case): FP64 and mixed throughput beside the CPU oracle on the full ensemble."""
+    e = configs.config1()
+    p = ctx.prepare(e)
+    p.execute()
+    t0 = time.perf_counter()
+    r64 = p.execute()
+    t64 = time.perf_counter() - t0
+    p.release()
+    em = configs.config1(precision="mixed")
+    em.skip_check = True
+    pm = ctx.prepare(em)
+    pm.execute()
+    t0 = time.perf_counter()
+    rm = pm.execute()
+    tm = time.perf_counter() - t0
+    pm.release()
+    cpu = cpu_baseline(e, budget_s=60.0)
+    ps = e.particle_steps()
+    d = rm["mean"] - r64["mean"]
+    return dict(workload="Neel uniaxial EulerMaruyama, 4096 particles x 199,900 steps, dt 1e-12 s, zero field",
+                fp64=ps / t64, mixed=ps / tm, unit="particle-steps/s", cpu_baseline=cpu, fp64_vs_cpu=ps / t64 / cpu["value"],
+                err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
+                note="the GPU is latency bound here: 4,096 particles are 64 waves on 1,024 SIMDs")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +239,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-sets", type=int, default=48, help="uneven sets in the mixed/batched leg")
+    ap.add_argument("--workload", choices=("headline", "precision", "neel"), default="headline",
+                    help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation "
+                         "(each prints its own JSON line instead of the headline one)")
+    ap.add_argument("--particles", type=int, default=1_048_576, help="precision study: particles")
+    ap.add_argument("--study-steps", type=int, default=500_000, help="precision study: steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the reference kernel that calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -206,6 +265,15 @@ def main():
         sys.exit(2)
 
     from stochasticphysics_amd import Context, configs
+    if args.workload != "headline":
+        ctx = Context(0)
+        if args.workload == "precision":
+            res = precision_study(ctx, configs, args.particles, args.study_steps)
+        else:
+            res = neel_relaxation(ctx, configs)
+        ctx.close()
+        print(json.dumps(res), flush=True)
+        return
     n_steps = 20_000 if args.quick else 200_000
     e = configs.config2(n_steps=n_steps)
     ctx = Context(0)

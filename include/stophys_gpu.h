@@ -157,7 +157,7 @@ typedef struct sp_result {
 
 /* execution knobs chosen by the autotuner (or by hand) */
 typedef struct sp_tuning {
-    uint32_t block_size;       /* work-group size: 64..1024, multiple of 64 */
+    uint32_t block_size;       /* work-group size: 64, 128, 192 or 256 */
     uint32_t particles_per_lane; /* 1 or 2 */
     uint32_t chunk_steps;      /* steps advanced per request per scheduler iteration / launch */
     uint32_t use_graph;        /* capture chunk launches in a hipGraph (static batches) */

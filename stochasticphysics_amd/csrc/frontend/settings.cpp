@@ -293,8 +293,10 @@ int write_result(const Settings& s, const sp_result& res, const std::string& pat
                   "\"Hydrodynamic_Radius\":%.17g,\"Viscosity\":%.17g}}",
                   mean(s.T), mean(s.radius), mean(s.Ms), mean(s.alpha), mean(s.gamma), mean(s.K), mean(s.r_h), mean(s.visc));
     o << buf;
-    std::snprintf(buf, sizeof buf, ",\"Engine\":{\"device_ms\":%.6f,\"particle_steps\":%.17g,\"precision_used\":%u,\"check_error\":%.6g}}",
-                  res.device_ms, res.particle_steps, res.precision_used, res.check_error);
+    std::snprintf(buf, sizeof buf,
+                  ",\"Engine\":{\"device_ms\":%.6f,\"particle_steps\":%.17g,\"precision_used\":%u,\"check_error\":%.6g,"
+                  "\"check_noise\":%.6g,\"used_fallback\":%u}}",
+                  res.device_ms, res.particle_steps, res.precision_used, res.check_error, res.check_noise, res.used_fallback);
     o << buf << '\n';
     return o ? SP_OK : set_error(SP_ERR_INVALID_ARG, "write failed: " + path);
 }

@@ -326,10 +326,47 @@ template <int ANISO, int SOLVER, class R> struct BNCfg {
 // vectors and projection P (:162-236), det = P H (:239-265), S = sigma P
 // (:267-293), drift (-D/2 cos(theta)/sin(theta), 0) (:295-309), finish (:440-471),
 // output (sin t cos p, sin t sin p, cos t) (:499-516).
+// sin and cos of a double, |x| < 2^19 pi/2, to about one ulp: Cody-Waite reduction x = k pi/2 + r with
+// pi/2 in three parts (k * part exact for |k| < 2^20), Taylor series of sin r to r^17 and cos r to r^16 on
+// |r| <= pi/4 (the next terms are below 1e-16 relative), quadrant from the low bits of k without branches.
+// Larger arguments (never reached by angles that stay near [0, 2 pi]) take the library routine.
+__device__ __forceinline__ void sincos_fast(double x, double* s_out, double* c_out) {
+    if (!(fabs(x) < 823549.6)) { sincos(x, s_out, c_out); return; }
+    const double k = rint(x * 0x1.45f306dc9c883p-1);          // x 2/pi
+    double r = fma(-k, 0x1.921fb54400000p+0, x);              // pi/2 = P1 + P2 + P3
+    r = fma(-k, 0x1.0b4611a600000p-34, r);
+    r = fma(-k, 0x1.3198a2e037073p-69, r);
+    const double r2 = r * r;
+    double sp = 2.8114572543455206e-15;                       // (-1)^n / (2n+1)!, n = 8 .. 1
+    sp = fma(sp, r2, -7.647163731819816e-13);
+    sp = fma(sp, r2, 1.6059043836821613e-10);
+    sp = fma(sp, r2, -2.505210838544172e-08);
+    sp = fma(sp, r2, 2.7557319223985893e-06);
+    sp = fma(sp, r2, -0.0001984126984126984);
+    sp = fma(sp, r2, 0.008333333333333333);
+    sp = fma(sp, r2, -0.16666666666666666);
+    const double sn = fma(r * r2, sp, r);
+    double cp = 4.779477332387385e-14;                        // (-1)^n / (2n)!, n = 8 .. 1
+    cp = fma(cp, r2, -1.1470745597729725e-11);
+    cp = fma(cp, r2, 2.08767569878681e-09);
+    cp = fma(cp, r2, -2.755731922398589e-07);
+    cp = fma(cp, r2, 2.48015873015873e-05);
+    cp = fma(cp, r2, -0.001388888888888889);
+    cp = fma(cp, r2, 0.041666666666666664);
+    cp = fma(cp, r2, -0.5);
+    const double cs = fma(r2, cp, 1.0);
+    const int q = (int)k & 3;
+    double ss = (q & 1) ? cs : sn, cc = (q & 1) ? sn : cs;
+    ss = (q & 2) ? -ss : ss;
+    cc = ((q + 1) & 2) ? -cc : cc;
+    *s_out = ss;
+    *c_out = cc;
+}
+
 template <class R> struct Trig { R st, ct, sp, cp; };
 __device__ __forceinline__ void trig_of(double th, double ph, Trig<double>& T) {
-    sincos(th, &T.st, &T.ct);
-    sincos(ph, &T.sp, &T.cp);
+    sincos_fast(th, &T.st, &T.ct);
+    sincos_fast(ph, &T.sp, &T.cp);
 }
 __device__ __forceinline__ void trig_of(double th, double ph, Trig<float>& T) {
     // hardware sin/cos take revolutions; the reduction to one turn is done in double
@@ -394,7 +431,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         const bool rot = (y[0] < rq.tmin) || (y[0] > rq.tmax);
         Y[0] = y[0]; Y[1] = y[1];
         if (rot) {
-            const double st = sin(y[0]), ct = cos(y[0]), sp = sin(y[1]), cp = cos(y[1]);
+            double st, ct, sp, cp;
+            sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
             Y[0] = acos(-cp * st);
             Y[1] = atan2(sp * st, ct);
             Trig<double> Td;
@@ -406,7 +444,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
     // finishCalculations (:440-471), then the trig of the new state
     __device__ __forceinline__ static void finish(const double r[2], bool rot, double* y, Trig<R>& T) {
         if (rot) {
-            const double st = sin(r[0]), ct = cos(r[0]), sp = sin(r[1]), cp = cos(r[1]);
+            double st, ct, sp, cp;
+            sincos_fast(r[0], &st, &ct); sincos_fast(r[1], &sp, &cp);
             y[0] = acos(cp * st);
             y[1] = atan2(sp * st, -ct);
         } else {
@@ -486,7 +525,7 @@ template <int N> __device__ __forceinline__ Dual<N> chain(const Dual<N>& a, doub
 #pragma unroll
     for (int i = 0; i < N; ++i) r.d[i] = df * a.d[i]; return r; }
 template <int N> __device__ __forceinline__ void dsincos(const Dual<N>& a, Dual<N>& s, Dual<N>& c) {
-    double sv, cv; sincos(a.v, &sv, &cv); s = chain(a, sv, cv); c = chain(a, cv, -sv);
+    double sv, cv; sincos_fast(a.v, &sv, &cv); s = chain(a, sv, cv); c = chain(a, cv, -sv);
 }
 template <int N> __device__ __forceinline__ Dual<N> dacos(const Dual<N>& a) { return chain(a, acos(a.v), -1.0 / sqrt(1.0 - a.v * a.v)); }
 template <int N> __device__ __forceinline__ Dual<N> datan2(const Dual<N>& y, const Dual<N>& x) {
@@ -632,7 +671,7 @@ template <int ANISO> struct IMSph {
     }
     __device__ void weights(double wv[3]) const { wv[0] = wv[1] = wv[2] = w; }
     __device__ void output(const double* y, double o[3]) const {
-        double st, ct, sp, cp; sincos(y[0], &st, &ct); sincos(y[1], &sp, &cp);
+        double st, ct, sp, cp; sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
         o[0] = st * cp; o[1] = st * sp; o[2] = ct;
     }
     __device__ void residual(const double* yv, const double* yi, V3<double> h, const double* z, double F[2], double J[2][2]) const {

@@ -7,7 +7,7 @@ namespace sp {
 
 // One wave per 64-particle tile of one request.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(256) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
+__global__ __launch_bounds__(MAX_BLOCK) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
                                                double* partial, uint32_t pstride) {
     const DevRequest& rq = *rqp;
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -35,7 +35,7 @@ __device__ __forceinline__ uint32_t find_slot(const SlotWork* __restrict__ slots
 // Persistent: every wave pulls global tile indices from the counter until all
 // n_tiles tiles of the launch are taken; every wave reaches that exit.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(256) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
+__global__ __launch_bounds__(MAX_BLOCK) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
                                                uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
     for (;;) {
         const uint32_t idx = next_work(counter);

@@ -65,7 +65,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     if (int e = ensure_table(ctx, n_chunks * sizeof(ReduceWork) + 256)) return e;
     ReduceWork* rw_host = (ReduceWork*)ctx->pinned;
     ReduceWork* rw_dev = (ReduceWork*)ctx->table;
-    const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(1024, tun.block_size ? tun.block_size / 64 * 64 : 256));
+    const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, tun.block_size ? tun.block_size / 64 * 64 : MAX_BLOCK));
     const uint32_t waves_per_block = block / 64;
     const dim3 grid((hr.n_tiles + waves_per_block - 1) / waves_per_block);
     std::vector<hipEvent_t> ev(2 * n_chunks + 2);
@@ -194,6 +194,10 @@ int sp_ctx_create(int device, sp_ctx** out) {
 
 int sp_ctx_set_tuning(sp_ctx* ctx, int problem, int solver, int precision, const sp_tuning* t) {
     if (!ctx || !t) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    if (t->block_size < 64 || t->block_size > MAX_BLOCK || t->block_size % 64)
+        return set_error(SP_ERR_INVALID_ARG, "block_size must be 64, 128, 192 or 256");
+    if (t->chunk_steps == 0 || t->persistent_waves_per_simd == 0 || t->persistent_waves_per_simd > 8)
+        return set_error(SP_ERR_INVALID_ARG, "chunk_steps must be > 0 and persistent_waves_per_simd in 1..8");
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->tuning[tuning_key(problem, solver, precision)] = *t;
     return SP_OK;

@@ -106,7 +106,7 @@ static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
 
 // persistent grid for one kind
 static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning& t) {
-    const uint32_t wpb = std::max<uint32_t>(1, t.block_size / 64);
+    const uint32_t wpb = std::max<uint32_t>(1, std::min<uint32_t>(MAX_BLOCK, t.block_size) / 64);
     const uint32_t per_simd = std::max<uint32_t>(1, t.persistent_waves_per_simd);
     const uint32_t resident = (uint32_t)ctx->prop.multiProcessorCount * 4u * per_simd;   // waves
     const uint32_t waves = std::min(n_work, resident);
@@ -126,7 +126,7 @@ static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const
     }
     for (size_t k = 0; k < P.kinds.size(); ++k) {
         const sp_tuning& t = tun[k];
-        const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(1024, t.block_size / 64 * 64));
+        const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, t.block_size / 64 * 64));
         const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
         hipStream_t sk = k == 0 ? s : ctx->side[(k - 1) % sp_ctx::N_SIDE];
         if (!launch_batch(P.kinds[k], grid, dim3(block), sk, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
@@ -753,21 +753,22 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
     HostRequest probe;
     if (int e = prepare_request(&r, parts, probe)) return e;
     const int prec = probe.mixed ? SP_PREC_MIXED : SP_PREC_FP64;
-    const uint32_t blocks[] = {64, 128, 256, 512};
+    const uint32_t blocks[] = {64, 128, 256};          // the kernels' __launch_bounds__ is MAX_BLOCK = 256
     const uint32_t waves[] = {1, 2, 4};
     const uint32_t chunks[] = {2000, 8000, 32000};
     double best_r = -1.0;
     sp_tuning bt{};
+    int err = SP_OK;
     auto trial = [&](const sp_tuning& t) -> double {
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             ctx->tuning[tuning_key(r.problem, r.solver, prec)] = t;
         }
         std::vector<HostRequest> one(1);
-        if (prepare_request(&r, parts, one[0])) return -1.0;
+        if ((err = prepare_request(&r, parts, one[0]))) return -1.0;
         std::vector<Outputs> outs;
         const auto t0 = Clock::now();
-        if (run_batch_static(ctx, one, outs, false, t.use_graph != 0)) return -1.0;
+        if ((err = run_batch_static(ctx, one, outs, false, t.use_graph != 0))) return -1.0;
         const double ms = ms_since(t0, Clock::now());
         return (double)probe.N * (double)probe.total_steps / (ms * 1e-3);
     };
@@ -776,7 +777,12 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
             for (uint32_t w : waves) {
                 sp_tuning t{b, 1, c, 1, w};
                 trial(t);                        // warm (allocations, code load)
-                const double rate = trial(t);
+                const double rate = err == SP_OK ? trial(t) : -1.0;
+                if (err != SP_OK) {               // a failed trial ends the search with its error
+                    std::lock_guard<std::mutex> lk(ctx->mu);
+                    ctx->tuning.erase(tuning_key(r.problem, r.solver, prec));
+                    return err;
+                }
                 if (rate > best_r) { best_r = rate; bt = t; }
             }
     {

@@ -4,6 +4,9 @@
 
 namespace sp {
 
+// largest work-group of the simulation kernels (their __launch_bounds__)
+constexpr uint32_t MAX_BLOCK = 256;
+
 // Per-particle parameter block, param-major in HBM: params[k * N + p].  Filled on
 // the host by the reference's parameter calculators (sp_params.cpp).
 enum ParamIndex : int {
