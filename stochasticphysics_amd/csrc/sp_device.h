@@ -94,9 +94,9 @@ struct FieldCache {
         const V3<double> h = field_at(f, t);
         hx = h.x; hy = h.y; hz = h.z;
     }
+    // branch-free: an inactive (zero) field keeps hx = hy = hz = 0 in every lane
     __device__ __forceinline__ V3<double> at(uint64_t n) const {
-        if (!active) return {0.0, 0.0, 0.0};
-        const int k = (int)(n - base);
+        const int k = (int)(n - base) & 63;
         return {readlane_d(hx, k), readlane_d(hy, k), readlane_d(hz, k)};
     }
 };
