@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 2
+#define SP_ABI_VERSION 3
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -49,7 +49,8 @@ enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_
 enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_EULER_MARUYAMA_NORMALIZED = 2, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
        SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7 };
 /* Properties::IField (Fields/FieldList.h:48-59) */
-enum { SP_FIELD_ZERO = 1, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4 };
+enum { SP_FIELD_ZERO = 1, SP_FIELD_CONSTANT = 2, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4,
+       SP_FIELD_TRIANGULAR = 5, SP_FIELD_RECTANGULAR = 6, SP_FIELD_SINC = 7 };
 /* Properties::IAnisotropy (Problems/Anisotropy/AnisotropyList.h:45-54) */
 enum { SP_ANISO_UNIAXIAL = 1, SP_ANISO_CUBIC = 2 };
 
@@ -95,9 +96,10 @@ typedef struct sp_request {
     double min_angle;         /* NeelSpherical Min_angle_transformation [rad] */
     double check_tol;         /* mixed precision: tolerated bias of <m>_mixed - <m>_fp64 (0: 1e-3; a negative
                                  value fails every check, which forces the FP64 re-run) */
-    /* field parameters (Properties/Fields/Sinusoidal.hpp:15-35, Lissajous.hpp:15-37):
+    /* field parameters (Properties/Fields/*.hpp): Offset_Field, Amplitude for every field but zero;
      * sinusoidal: H = amplitude*sin(2 pi freq[0] t + phase[0]) + offset
-     * lissajous:  H_i = offset_i + amplitude_i*sin(2 pi freq_i t + phase_i) */
+     * lissajous:  H_i = offset_i + amplitude_i*sin(2 pi freq_i t + phase_i)
+     * constant: H = offset; triangular, rectangular, sinc: see the period/time-offset/tau fields below */
     double field_offset[3];
     double field_amplitude[3];
     double field_freq[3];
@@ -105,6 +107,11 @@ typedef struct sp_request {
     /* SP_NOISE_REPLAY only: standard normals z[k*n_particles + p], k = step*dim_W + j */
     const double* replay_normals;
     uint64_t replay_count;    /* number of k values available */
+    double field_period;      /* triangular, rectangular, sinc: Periode [s] */
+    double field_time_offset; /* triangular, rectangular: TimeOffset [s] */
+    double field_tau;         /* rectangular: Tau [s], time constant of the edges */
+    uint32_t field_alternating; /* rectangular: Alternating (0/1) */
+    uint32_t reserved;
 } sp_request;
 
 /*

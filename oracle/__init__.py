@@ -45,6 +45,7 @@ def lib(native=False):
         L.or_brown_params.argtypes = [C.c_double] * 3 + [d]
         L.or_aniso_field.argtypes = [C.c_int, d, d, d, d, d]
         L.or_euler_axes.argtypes = [C.c_double] * 3 + [d]
+        L.or_field.argtypes = [d, C.c_double, d]
         L.or_eval.argtypes = [C.c_int, C.c_int, d, d, C.c_int, C.c_double, d, d, C.c_double, C.c_int, d]
         L.or_eval.restype = C.c_int
         L.or_newton_testfn.argtypes = [C.c_int, d, C.c_double, C.c_int, d]
@@ -89,6 +90,19 @@ def box_muller(words):
     w = np.ascontiguousarray(words, dtype=np.uint32).reshape(-1, 4)
     out = np.zeros(w.shape)
     lib().or_box_muller(_p(w, C.c_uint32), w.shape[0], _p(out))
+    return out
+
+
+def field_at(field, t):
+    """The oracle's applied field H(t) [T] for a field dict as simulate() takes it."""
+    d = np.zeros(20)
+    d[3] = field.get("type", 1)
+    for j, key in enumerate(("offset", "amp", "freq", "phase")):
+        d[4 + 3 * j:7 + 3 * j] = np.asarray(field.get(key, [0.0, 0.0, 0.0]), dtype=np.float64)
+    d[16], d[17], d[18] = field.get("period", 0.0), field.get("time_offset", 0.0), field.get("tau", 0.0)
+    d[19] = 1.0 if field.get("alternating", False) else 0.0
+    out = np.zeros(3)
+    lib().or_field(_p(d), float(t), _p(out))
     return out
 
 
@@ -158,10 +172,12 @@ def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0,
     dy, dout = DIM_Y[problem], DIM_OUT[problem]
     n_out = n_steps // oversampling
     f = field or dict(type=1)
-    dspec = np.zeros(16)
+    dspec = np.zeros(20)
     dspec[0], dspec[1], dspec[2], dspec[3] = dt, min_angle, newton_tol, f.get("type", 1)
     for j, key in enumerate(("offset", "amp", "freq", "phase")):
         dspec[4 + 3 * j:7 + 3 * j] = np.asarray(f.get(key, [0.0, 0.0, 0.0]), dtype=np.float64)
+    dspec[16], dspec[17], dspec[18] = f.get("period", 0.0), f.get("time_offset", 0.0), f.get("tau", 0.0)
+    dspec[19] = 1.0 if f.get("alternating", False) else 0.0
     flags = (1 if bn_simple else 0) | (2 if coord_transform else 0) | (4 if renormalize else 0)
     ispec = np.array([problem, solver, aniso, flags, newton_max_iter, threads], dtype=np.int64)
     rep = None if replay is None else np.ascontiguousarray(replay, dtype=np.float64)

@@ -67,6 +67,21 @@ void or_aniso_field(int aniso, const double* props, const double* m, const doubl
     if (jac) for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) jac[i * 3 + j] = J(i, j);
 }
 
+// dspec[3] type, [4..15] offset, amplitude, frequency, phase, [16] Periode, [17] TimeOffset, [18] Tau, [19] Alternating
+static FieldSpec fieldFrom(const double* dspec) {
+    FieldSpec f;
+    f.type = (int)dspec[3];
+    for (int i = 0; i < 3; ++i) { f.offset[i] = dspec[4 + i]; f.amp[i] = dspec[7 + i]; f.freq[i] = dspec[10 + i]; f.phase[i] = dspec[13 + i]; }
+    f.periode = dspec[16]; f.timeOffset = dspec[17]; f.tau = dspec[18]; f.alternating = dspec[19] != 0.0;
+    f.prepare();
+    return f;
+}
+
+void or_field(const double* dspec, double t, double* out) {
+    const Vec<double, 3> h = fieldFrom(dspec).at(t);
+    for (int i = 0; i < 3; ++i) out[i] = h[i];
+}
+
 void or_euler_axes(double phi, double theta, double psi, double* out) {
     const Axes a = axesFromEuler(phi, theta, psi);
     for (int i = 0; i < 3; ++i) { out[i] = a.x[i]; out[3 + i] = a.y[i]; out[6 + i] = a.z[i]; }
@@ -179,7 +194,7 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
     rs.replayCount = uspec[4]; rs.replay = replay;
     rs.dt = dspec[0]; rs.minAngle = dspec[1]; rs.newtonTol = dspec[2];
     rs.field.type = (int)dspec[3];
-    for (int i = 0; i < 3; ++i) { rs.field.offset[i] = dspec[4 + i]; rs.field.amp[i] = dspec[7 + i]; rs.field.freq[i] = dspec[10 + i]; rs.field.phase[i] = dspec[13 + i]; }
+    rs.field = fieldFrom(dspec);
     const uint64_t N = rs.nParticles;
     const int dy = rs.problem == PROB_NEEL ? 3 : (rs.problem == PROB_SPHERICAL ? 2 : 6);
     std::vector<ParticleIn> parts(N);

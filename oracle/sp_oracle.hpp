@@ -20,6 +20,7 @@
 //     formulas against a direct evaluation, equilibrium statistics.
 #pragma once
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -422,11 +423,29 @@ template <class Aniso, bool Simple> struct BrownAndNeelProblem {
 };
 
 // ------------------------------------------------------------------ fields
-// Fields/ZeroField.h:39, SinusoidalField.h:38-51, LissajousField.h:50-101
+// Fields/ZeroField.h:39, ConstantField.h:43, SinusoidalField.h:38-51, LissajousField.h:50-101,
+// TriangularField.h:42-67, RectangularField.h:26-63, SincField.h:34-45
 struct FieldSpec {
-    int type = 1;                              // 1 zero, 3 sinusoidal, 4 lissajous
+    int type = 1;     // 1 zero, 2 constant, 3 sinusoidal, 4 lissajous, 5 triangular, 6 rectangular, 7 sinc
     double offset[3] = {0, 0, 0}, amp[3] = {0, 0, 0}, freq[3] = {0, 0, 0}, phase[3] = {0, 0, 0};
+    double periode = 0, timeOffset = 0, tau = 0;
+    bool alternating = false;
+    // RectangularField constructor: derived constants (RectangularField.h:26-34)
+    double halfP = 0, tauInv = 0, newAmp[3] = {0, 0, 0}, maxField[3] = {0, 0, 0}, newTimeOffset = 0, newOffset[3] = {0, 0, 0};
+    void prepare() {
+        halfP = periode / 2.0;
+        tauInv = std::abs(tau) <= std::numeric_limits<double>::min() ? std::numeric_limits<double>::max() : 1.0 / tau;
+        double na = 0, mf = 0;
+        for (int i = 0; i < 3; ++i) {
+            newAmp[i] = alternating ? 2 * amp[i] : amp[i];
+            maxField[i] = newAmp[i] * (-std::expm1(-halfP * tauInv));
+            na += newAmp[i] * newAmp[i]; mf += maxField[i] * maxField[i];
+        }
+        newTimeOffset = alternating ? timeOffset - std::log(1 - std::sqrt(mf) / (2 * std::sqrt(na))) * tau : timeOffset;
+        for (int i = 0; i < 3; ++i) newOffset[i] = alternating ? newAmp[i] * std::expm1(-newTimeOffset * tauInv) + offset[i] : offset[i];
+    }
     Vec<double, 3> at(double t) const {
+        if (type == 2) return v3(offset[0], offset[1], offset[2]);
         if (type == 3) {
             const double w = TWO_PI * freq[0];
             const double s = std::sin(w * t + phase[0]);
@@ -436,6 +455,27 @@ struct FieldSpec {
             Vec<double, 3> r;
             for (int i = 0; i < 3; ++i) r[i] = offset[i] + amp[i] * std::sin(TWO_PI * freq[i] * t + phase[i]);
             return r;
+        }
+        if (type == 5) {
+            const double nt = t + timeOffset, half = periode / 2.0;
+            const double pos = nt < 0 ? periode + std::fmod(nt, periode) : std::fmod(nt, periode);
+            const double f = pos > half ? (2.0 * pos / half - 1.0) : (-2.0 * pos / half + 3.0);
+            return v3(f * amp[0] + offset[0], f * amp[1] + offset[1], f * amp[2] + offset[2]);
+        }
+        if (type == 6) {
+            const double nt = t + newTimeOffset;
+            const double pos = nt < 0 ? std::fmod(nt + periode, periode) : std::fmod(nt, periode);
+            Vec<double, 3> r;
+            if (pos <= halfP) { const double e = -std::expm1(-pos * tauInv); for (int i = 0; i < 3; ++i) r[i] = newAmp[i] * e + newOffset[i]; }
+            else { const double e = std::exp(-(pos - halfP) * tauInv); for (int i = 0; i < 3; ++i) r[i] = newAmp[i] * e + newOffset[i] - (newAmp[i] - maxField[i]); }
+            return r;
+        }
+        if (type == 7) {
+            const double half = periode * 0.5, a = 11.0 / half;
+            const double pos = t < 0 ? std::fmod(t + periode, periode) - half : std::fmod(t, periode) - half;
+            const double x = 3.14159265358979323846 * a * pos;
+            const double sc = x == 0.0 ? 1.0 : std::sin(x) / x;
+            return v3(amp[0] * sc * sc + offset[0], amp[1] * sc * sc + offset[1], amp[2] * sc * sc + offset[2]);
         }
         return Vec<double, 3>::zero();
     }

@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libstophys_gpu.so"
 # enum values (identical to the reference's selector enums)
 PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3)
 SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
-FIELD = dict(none=1, sinusoidal=3, lissajous=4)
+FIELD = dict(none=1, constant=2, sinusoidal=3, lissajous=4, triangular=5, rectangular=6, sinc=7)
 ANISO = dict(uniaxial=1, cubic=2)
 PREC = dict(fp64=0, mixed=1)
 NOISE = dict(philox=0, replay=1)
@@ -40,7 +40,9 @@ class Request(C.Structure):
                 ("seed", C.c_uint64), ("dt", C.c_double), ("newton_tol", C.c_double), ("min_angle", C.c_double),
                 ("check_tol", C.c_double), ("field_offset", C.c_double * 3), ("field_amplitude", C.c_double * 3),
                 ("field_freq", C.c_double * 3), ("field_phase", C.c_double * 3),
-                ("replay_normals", C.POINTER(C.c_double)), ("replay_count", C.c_uint64)]
+                ("replay_normals", C.POINTER(C.c_double)), ("replay_count", C.c_uint64),
+                ("field_period", C.c_double), ("field_time_offset", C.c_double), ("field_tau", C.c_double),
+                ("field_alternating", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class Particles(C.Structure):

@@ -189,6 +189,18 @@ Settings* load(const std::string& path) {
             if (!(ph.size() == 1 || ph.size() == 3)) throw std::runtime_error("LissajousField: Phases needs 1 or 3 values");
             fill3(r.field_phase, ph, true);
         }
+    } else if (r.field == SP_FIELD_CONSTANT) {                                   // Properties/Fields/Constant.hpp:29-32
+        fill3(r.field_offset, vec(f, FS, {"Offset_Field"}), false);
+    } else if (r.field == SP_FIELD_TRIANGULAR || r.field == SP_FIELD_RECTANGULAR || r.field == SP_FIELD_SINC) {
+        // Triangular.hpp:30-36, Rectangular.hpp:32-40, Sinc.hpp:30-36
+        fill3(r.field_offset, vec(f, FS, {"Offset_Field"}), false);
+        fill3(r.field_amplitude, vec(f, FS, {"Amplitude"}), false);
+        r.field_period = num(f, FS, {"Periode"}, 0.0);
+        r.field_time_offset = num(f, FS, {"TimeOffset"}, 0.0);
+        if (r.field == SP_FIELD_RECTANGULAR) {
+            r.field_tau = num(f, FS, {"Tau"}, 0.0);
+            r.field_alternating = flag(f, FS, {"Alternating"}, false) ? 1u : 0u;
+        }
     }
     // [Result_Settings] (ResultSettings.h:90-118)
     if (const std::string* sf = opt(f, "Result_Settings", {"Savefile"})) s->savefile = *sf;

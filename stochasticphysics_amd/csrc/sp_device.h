@@ -50,33 +50,19 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 }
 
 // ---------------------------------------------------------------- field
-// Fields/ZeroField.h:39, SinusoidalField.h:47-51 (A sin(w t + phi) + offset),
-// LissajousField.h:98-101 (offset + A o sin(w o t + phi)).
-__device__ __forceinline__ V3<double> field_at(const DevRequest& rq, double t) {
-    if (rq.field_type == 3) {
-        const double s = sin(rq.f_w[0] * t + rq.f_ph[0]);
-        return {rq.f_amp[0] * s + rq.f_off[0], rq.f_amp[1] * s + rq.f_off[1], rq.f_amp[2] * s + rq.f_off[2]};
-    }
-    if (rq.field_type == 4) {
-        return {rq.f_off[0] + rq.f_amp[0] * sin(rq.f_w[0] * t + rq.f_ph[0]),
-                rq.f_off[1] + rq.f_amp[1] * sin(rq.f_w[1] * t + rq.f_ph[1]),
-                rq.f_off[2] + rq.f_amp[2] * sin(rq.f_w[2] * t + rq.f_ph[2])};
-    }
-    return {0.0, 0.0, 0.0};
-}
-
 // Field values for 64 consecutive steps, one step per lane; step n (0-based) is
 // evaluated at t = dt (n+1) (+ dt/2 for ImplicitMidpoint), the reference's
 // time = _timestep*(++counter) (SingleParticleSimulator.h:145) and
-// Implicit_Midpoint.inl:59.  The refill reads the field parameters with
-// volatile loads so that they are not kept in scalar registers across the loop.
+// Implicit_Midpoint.inl:59; the field itself is field_eval (sp_field.h).  The
+// refill copies the parameters with volatile loads so that they are not kept in
+// scalar registers across the loop.
 struct FieldCache {
     uint64_t base;
     double hx, hy, hz;
     bool active;
     __device__ __forceinline__ void init(const DevRequest& rq) {
         base = ~0ull; hx = hy = hz = 0.0;
-        active = __builtin_amdgcn_readfirstlane(rq.field_type) != 1;   // uniform
+        active = __builtin_amdgcn_readfirstlane((int)rq.field.type) != FIELD_ZERO;   // uniform
     }
     // make steps [n, n + count) available (count <= 64, 64 % count == 0, n aligned to count)
     template <bool MIDPOINT>
@@ -84,15 +70,16 @@ struct FieldCache {
         if (!active) return;
         if (base != ~0ull && n - base + count <= 64) return;
         base = n;
-        const volatile DevRequest* v = &rq;
-        DevRequest f;
-        f.field_type = v->field_type;
-        for (int i = 0; i < 3; ++i) { f.f_off[i] = v->f_off[i]; f.f_amp[i] = v->f_amp[i]; f.f_w[i] = v->f_w[i]; f.f_ph[i] = v->f_ph[i]; }
+        FieldParams f;
+        const volatile double* src = reinterpret_cast<const volatile double*>(&rq.field);
+        double* dst = reinterpret_cast<double*>(&f);
+        for (int i = 0; i < (int)(sizeof(FieldParams) / 8); ++i) dst[i] = src[i];
         const int lane = threadIdx.x & 63;
         double t = rq.dt * (double)(n + lane + 1);
         if (MIDPOINT) t = t + 0.5 * rq.dt;
-        const V3<double> h = field_at(f, t);
-        hx = h.x; hy = h.y; hz = h.z;
+        double h[3];
+        field_eval(f, t, h);
+        hx = h[0]; hy = h[1]; hz = h[2];
     }
     // branch-free: an inactive (zero) field keeps hx = hy = hz = 0 in every lane
     __device__ __forceinline__ V3<double> at(uint64_t n) const {
@@ -742,7 +729,7 @@ template <class Prob> struct IMCfg {
     }
 };
 
-// Eigen's normalize() on each 3-block of the state, in FP64: v /= |v| when |v| > 0
+// Eigen's normalize() on each 3-block of the state, in FP64: v / |v| when |v| > 0
 // (NeelRelaxation.h:204-208, BrownAndNeelRelaxation.h:168-172; a no-op for
 // NeelSpherical, NeelRelaxationSpherical.h:534-535).
 template <int NORM>
@@ -751,9 +738,9 @@ __device__ __forceinline__ void normalize_blocks(double* y) {
     for (int b = 0; b < NORM; ++b) {
         double* v = y + 3 * b;
         const double q = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
-        if (q > 0.0) {
-            const double r = sqrt(q);
-            v[0] = v[0] / r; v[1] = v[1] / r; v[2] = v[2] / r;
+        if (q > 0.0) {                       // one division: v * (1 / |v|), within an ulp of v / |v|
+            const double inv = 1.0 / sqrt(q);
+            v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
         }
     }
 }

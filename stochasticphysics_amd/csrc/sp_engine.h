@@ -57,8 +57,11 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_EULER_MARUYAMA_NORMALIZED &&
         rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT)
         return set_error(SP_ERR_UNSUPPORTED, "solver not available on the GPU backend");
-    if (rq->field != SP_FIELD_ZERO && rq->field != SP_FIELD_SINUSOIDAL && rq->field != SP_FIELD_LISSAJOUS)
+    if (rq->field < SP_FIELD_ZERO || rq->field > SP_FIELD_SINC)
         return set_error(SP_ERR_UNSUPPORTED, "field type not available on the GPU backend");
+    if ((rq->field == SP_FIELD_TRIANGULAR || rq->field == SP_FIELD_RECTANGULAR || rq->field == SP_FIELD_SINC) &&
+        !(rq->field_period > 0.0))
+        return set_error(SP_ERR_INVALID_ARG, "the field's Periode must be > 0");
     if (rq->anisotropy != SP_ANISO_UNIAXIAL && rq->anisotropy != SP_ANISO_CUBIC)
         return set_error(SP_ERR_UNSUPPORTED, "anisotropy type not available on the GPU backend");
     if (rq->precision != SP_PREC_FP64 && rq->precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
@@ -193,19 +196,11 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     dr.tmin = rot ? r.min_angle : -1e300;   // no chart rotation: theta may leave [0, pi] freely
     dr.tmax = rot ? (3.14159265358979323846 - r.min_angle) : 1e300;
     dr.newton_tol = r.newton_tol;
-    for (int i = 0; i < 3; ++i) {
-        dr.f_off[i] = r.field_offset[i];
-        dr.f_amp[i] = r.field_amplitude[i];
-        dr.f_ph[i] = r.field_phase[i];
-        dr.f_w[i] = 2.0 * 3.14159265358979323846 * r.field_freq[i];   // SinusoidalField.h:39, LissajousField.h:50-64
-    }
-    if (r.field == SP_FIELD_SINUSOIDAL)
-        for (int i = 1; i < 3; ++i) { dr.f_w[i] = dr.f_w[0]; dr.f_ph[i] = dr.f_ph[0]; }
+    dr.field = field_params(r);
     dr.n_particles = hr.N;
     dr.seed = r.seed;
     dr.replay_count = hr.replay.empty() ? 0 : r.replay_count;
     dr.oversampling = (uint32_t)r.oversampling;
-    dr.field_type = (uint32_t)r.field;
     const bool norm = (r.flags & SP_FLAG_RENORMALIZE) || r.solver == SP_SOLVER_EULER_MARUYAMA_NORMALIZED;
     dr.flags = ((r.flags & SP_FLAG_BN_SIMPLE_MODEL) ? RQ_BN_SIMPLE : 0u) | (hr.replay.empty() ? 0u : RQ_REPLAY) |
                (norm ? RQ_NORMALIZE : 0u);

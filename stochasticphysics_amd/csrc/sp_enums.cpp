@@ -18,9 +18,9 @@ const Entry kEntries[] = {
     {"solver", "EulerMaruyama_Normalized", SP_SOLVER_EULER_MARUYAMA_NORMALIZED},
     {"solver", "Millstein", SP_SOLVER_MILLSTEIN}, {"solver", "Heun_Strong", SP_SOLVER_HEUN_STRONG},
     {"solver", "Heun_NotConsistent", 8}, {"solver", "WeakTest", 9}, {"solver", "ExplicitStrong_1.0", 10},
-    {"field", "none", SP_FIELD_ZERO}, {"field", "constant", 2}, {"field", "sinusoidal", SP_FIELD_SINUSOIDAL},
-    {"field", "lissajous", SP_FIELD_LISSAJOUS}, {"field", "triangular", 5}, {"field", "rectangular", 6},
-    {"field", "sinc", 7},
+    {"field", "none", SP_FIELD_ZERO}, {"field", "constant", SP_FIELD_CONSTANT}, {"field", "sinusoidal", SP_FIELD_SINUSOIDAL},
+    {"field", "lissajous", SP_FIELD_LISSAJOUS}, {"field", "triangular", SP_FIELD_TRIANGULAR}, {"field", "rectangular", SP_FIELD_RECTANGULAR},
+    {"field", "sinc", SP_FIELD_SINC},
     {"anisotropy", "undefined", 0}, {"anisotropy", "uniaxial", SP_ANISO_UNIAXIAL}, {"anisotropy", "cubic", SP_ANISO_CUBIC},
     {"anisotropy", "mixed", 3}, {"anisotropy", "uniaxialcubic", 4},
 };

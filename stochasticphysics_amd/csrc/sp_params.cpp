@@ -70,32 +70,41 @@ void compute_params(const sp_request& rq, const sp_particles& pt, double* out, u
     }
 }
 
+FieldParams field_params(const sp_request& rq) {
+    FieldParams f{};
+    f.type = (double)rq.field;
+    for (int i = 0; i < 3; ++i) {
+        f.off[i] = rq.field_offset[i];
+        f.amp[i] = rq.field_amplitude[i];
+        f.ph[i] = rq.field_phase[i];
+        f.w[i] = 2.0 * kPi * rq.field_freq[i];   // SinusoidalField.h:39, LissajousField.h:50-64
+    }
+    if (rq.field == SP_FIELD_SINUSOIDAL)
+        for (int i = 1; i < 3; ++i) { f.w[i] = f.w[0]; f.ph[i] = f.ph[0]; }
+    f.period = rq.field_period;
+    f.half = rq.field_period * 0.5;                                         // Periode / 2 (exact)
+    f.toff = rq.field_time_offset;
+    f.sinc_a = rq.field == SP_FIELD_SINC ? 11.0 / f.half : 0.0;            // SincField.h:36
+    if (rq.field == SP_FIELD_RECTANGULAR) field_rectangular_constants(f, rq.field_tau, rq.field_alternating != 0);
+    return f;
+}
+
 // SingleParticleSimulator::calculateTimeAndField (SingleParticleSimulator.h:236-273)
 void time_and_field(const sp_request& rq, double* times, double* fields) {
     const uint64_t points = rq.n_steps / rq.oversampling;
     if (points == 0) return;
     const uint64_t OS = rq.oversampling;
-    auto field = [&](double t, double h[3]) {
-        if (rq.field == SP_FIELD_SINUSOIDAL) {
-            const double w = 2.0 * kPi * rq.field_freq[0];
-            const double s = std::sin(w * t + rq.field_phase[0]);
-            for (int i = 0; i < 3; ++i) h[i] = rq.field_amplitude[i] * s + rq.field_offset[i];
-        } else if (rq.field == SP_FIELD_LISSAJOUS) {
-            for (int i = 0; i < 3; ++i) h[i] = rq.field_offset[i] + rq.field_amplitude[i] * std::sin(2.0 * kPi * rq.field_freq[i] * t + rq.field_phase[i]);
-        } else {
-            h[0] = h[1] = h[2] = 0.0;
-        }
-    };
+    const FieldParams f = field_params(rq);
     double start = 0.0, end = 0.0;
     const double timeshift = rq.dt * ((double)OS) * 0.5;
     times[0] = start;
-    field(start, fields);
+    field_eval(f, start, fields);
     uint64_t counter = 0;
     for (uint64_t i = 1; i < points; ++i) {
         double tmp[3] = {0.0, 0.0, 0.0}, h[3];
         start = end + rq.dt;
         for (uint64_t l = OS; l--;) {
-            field(rq.dt * ((double)++counter), h);
+            field_eval(f, rq.dt * ((double)++counter), h);
             for (int c = 0; c < 3; ++c) tmp[c] += h[c];
         }
         end = rq.dt * ((double)counter) + start;

@@ -2,6 +2,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "sp_field.h"
+
 namespace sp {
 
 // largest work-group of the simulation kernels (their __launch_bounds__)
@@ -29,12 +31,11 @@ struct DevRequest {
     double dt, sqdt;
     double tmin, tmax;             // NeelSpherical chart rotation window (rotation when theta outside)
     double newton_tol;
-    double f_off[3], f_amp[3], f_w[3], f_ph[3];   // field: offset, amplitude, angular frequency, phase
+    FieldParams field;             // applied field (sp_field.h)
     uint64_t n_particles;
     uint64_t seed;
     uint64_t replay_count;         // normals available per particle in replay mode
     uint32_t oversampling;
-    uint32_t field_type;           // 1 zero, 3 sinusoidal, 4 lissajous
     uint32_t flags;                // RQ_BN_SIMPLE, RQ_REPLAY, RQ_NORMALIZE
     uint32_t newton_max_iter;
     uint32_t n_tiles;              // ceil(N / 64)

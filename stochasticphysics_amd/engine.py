@@ -51,6 +51,10 @@ class Ensemble:
     field_amplitude: tuple = (0.0, 0.0, 0.0)
     field_freq: tuple = (0.0, 0.0, 0.0)
     field_phase: tuple = (0.0, 0.0, 0.0)
+    field_period: float = 0.0                  # triangular, rectangular, sinc: Periode [s]
+    field_time_offset: float = 0.0             # triangular, rectangular: TimeOffset [s]
+    field_tau: float = 0.0                     # rectangular: Tau [s]
+    field_alternating: bool = False            # rectangular: Alternating
     replay: Optional[np.ndarray] = None        # [K, n] standard normals
 
     @property
@@ -102,7 +106,8 @@ class Ensemble:
 
     def field_dict(self):
         return dict(type=FIELD[self.field], offset=self.field_offset, amp=self.field_amplitude,
-                    freq=self.field_freq, phase=self.field_phase)
+                    freq=self.field_freq, phase=self.field_phase, period=self.field_period,
+                    time_offset=self.field_time_offset, tau=self.field_tau, alternating=self.field_alternating)
 
     # ---- C structs
     def to_c(self):
@@ -128,6 +133,8 @@ class Ensemble:
         r.dt, r.newton_tol, r.min_angle, r.check_tol = self.dt, self.newton_tol, self.min_angle, self.check_tol
         for name in ("field_offset", "field_amplitude", "field_freq", "field_phase"):
             getattr(r, name)[:] = [float(x) for x in getattr(self, name)]
+        r.field_period, r.field_time_offset, r.field_tau = self.field_period, self.field_time_offset, self.field_tau
+        r.field_alternating = 1 if self.field_alternating else 0
         if self.replay is not None:
             rep = np.ascontiguousarray(self.replay, dtype=np.float64)
             keep.append(rep)

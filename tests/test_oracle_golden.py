@@ -219,3 +219,34 @@ def test_cubic_field_is_energy_gradient():
         d = ax.sum(axis=0) / math.sqrt(3.0)                  # body diagonal: c_i = 1/sqrt(3) -> H = (-2K/Ms)(2/3) d
         Hd = oracle.aniso_field(2, props, d, ax)[0]
         assert np.allclose(Hd, (-2 * K / Ms) * (2.0 / 3.0) * d, rtol=1e-13, atol=1e-14 * abs(K) / Ms)
+
+
+def test_fields_closed_forms():
+    """The oracle's fields at points where the reference's formulas have closed values (ConstantField.h:43,
+    TriangularField.h:51-67, RectangularField.h:26-63, SincField.h:41-45)."""
+    A, off, P = np.array([0.0, 0.01, 0.02]), np.array([0.001, 0.0, -0.002]), 1e-6
+    h = lambda f, t: oracle.field_at(f, t)
+    const = dict(type=2, offset=off)
+    assert np.array_equal(h(const, 0.3), off)
+    tri = dict(type=5, offset=off, amp=A, period=P)
+    for t, s in [(0.0, 3.0), (P / 4, 2.0), (P / 2, 1.0), (3 * P / 4, 2.0), (-P / 4, 2.0), (P + P / 4, 2.0)]:
+        assert np.allclose(h(tri, t), s * A + off, rtol=1e-12, atol=1e-18), (t, s)
+    tau = P / 10
+    rect = dict(type=6, offset=off, amp=A, period=P, tau=tau)
+    for pos in (0.1 * P, 0.3 * P, 0.5 * P):
+        assert np.allclose(h(rect, pos), A * (1 - math.exp(-pos / tau)) + off, rtol=1e-12, atol=1e-18)
+    mx = A * (1 - math.exp(-0.5 * P / tau))
+    for pos in (0.6 * P, 0.9 * P):
+        assert np.allclose(h(rect, pos), A * math.exp(-(pos - 0.5 * P) / tau) + off - (A - mx), rtol=1e-12, atol=1e-18)
+    assert np.allclose(h(rect, 0.5 * P), h(rect, 0.5 * P * (1 + 1e-12)), rtol=1e-9, atol=1e-15)   # continuous
+    alt = dict(type=6, offset=off, amp=A, period=P, tau=tau, alternating=True)
+    ts = np.linspace(0, P, 20001)
+    vals = np.array([h(alt, t) for t in ts])
+    # the alternating wave is symmetric about the offset: max - offset = offset - min
+    assert np.allclose(vals.max(0) - off, off - vals.min(0), rtol=1e-3, atol=1e-12)
+    sinc = dict(type=7, offset=off, amp=A, period=P)
+    half = P / 2
+    assert np.allclose(h(sinc, half), A + off, rtol=1e-15)
+    assert np.allclose(h(sinc, half + half / 11), off, atol=1e-18)            # sinc(pi) = 0
+    x = math.pi * 0.5
+    assert np.allclose(h(sinc, half + half / 22), A * (math.sin(x) / x) ** 2 + off, rtol=1e-12)

@@ -76,3 +76,23 @@ def test_app_rejects_other_instruction_sets():
     r = subprocess.run([str(app), "--parameter_file", str(INI / "neel_config1.ini"), "--instruction_set", "AVX2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "GFX950" in r.stderr
+
+
+def test_rectangular_field_ini(tmp_path):
+    """[Field_Parameters] of a rectangular field (Properties/Fields/Rectangular.hpp:32-40)."""
+    text = (INI / "neel_config1.ini").read_text().replace(
+        "Type_of_field = 'none'",
+        "Type_of_field = 'rectangular'\nOffset_Field = {0.0, 0.0, 0.001}\nAmplitude = {0.005, 0.0, 0.008}\n"
+        "Tau = 6e-11\nTimeOffset = 5e-11\nPeriode = 8e-10\nAlternating = TRUE")
+    f = tmp_path / "rect.ini"
+    f.write_text(text)
+    (tmp_path / "neel_particle.ini").write_text((INI / "neel_particle.ini").read_text())
+    h = C.c_void_p()
+    assert lib().sp_settings_load(str(f).encode(), C.byref(h)) == 0, lib().sp_last_error()
+    try:
+        r, p = Request(), Particles()
+        assert lib().sp_settings_request(h, C.byref(r), C.byref(p)) == 0
+        assert r.field == 6 and list(r.field_offset) == [0.0, 0.0, 0.001] and list(r.field_amplitude) == [0.005, 0.0, 0.008]
+        assert (r.field_tau, r.field_time_offset, r.field_period, r.field_alternating) == (6e-11, 5e-11, 8e-10, 1)
+    finally:
+        lib().sp_settings_free(h)
