@@ -7,7 +7,8 @@ is committed and the build never regenerates it.  Every entry is the double
 nearest to the stated value as Python's math module (the C library)
 computes it from the given double arguments:
   * SP_SINCOS_TAB[j] = (sin t_j, cos t_j), t_j = (j + 1/2) * (2 pi / 128), j = 0..127
-  * SP_LOG_TAB[j + 32] = (log c_j, 1 / c_j), c_j = 1 + j / 128, j = -32..64
+  * SP_LOG_TAB[j + 32] = (log c_j, 2^-7 / c_j), c_j = 1 + j / 128, j = -32..64
+    (2^-7 / c_j is the double nearest to 1 / c_j, scaled exactly by 2^-7)
 """
 import math
 from pathlib import Path
@@ -29,11 +30,11 @@ def main():
     for j in range(128):
         t = (j + 0.5) * step
         lines.append(f"    {{{hexd(math.sin(t))}, {hexd(math.cos(t))}}},")
-    lines += ["};", "", "// (log c_j, 1 / c_j), c_j = 1 + j / 128, row j + 32 for j = -32..64",
+    lines += ["};", "", "// (log c_j, 2^-7 / c_j), c_j = 1 + j / 128, row j + 32 for j = -32..64",
               "static constexpr double SP_LOG_TAB[97][2] = {"]
     for j in range(-32, 65):
         c = 1.0 + j / 128.0                     # exact
-        lines.append(f"    {{{hexd(math.log(c))}, {hexd(1.0 / c)}}},")
+        lines.append(f"    {{{hexd(math.log(c))}, {hexd((1.0 / c) * 2.0 ** -7)}}},")
     lines += ["};", "", "}  // namespace sp", ""]
     out = Path(__file__).resolve().parent / "sp_rng_tables.h"
     out.write_text("\n".join(lines))

@@ -20,8 +20,8 @@
 //
 // The FP64 ln u and sin/cos(2 pi u) are written for this engine so that host
 // and device produce bit-identical normals: they only use IEEE operations that
-// are correctly rounded on x86-64 and on gfx950 (add, mul, fma, sqrt, frexp,
-// rint, int<->double conversion) and the constant tables of sp_rng_tables.h,
+// are correctly rounded on x86-64 and on gfx950 (add, mul, fma, sqrt, ldexp,
+// int<->double conversion) and the constant tables of sp_rng_tables.h,
 // and they work on the Philox word w itself, so the reduction is exact:
 //   ln u: v = w + 1/2 = m 2^e with m in [3/4, 3/2); j = rint(128 (m - 1)),
 //     c_j = 1 + j/128, r = (m - c_j) / c_j (|r| <= 1/192) and
@@ -46,6 +46,15 @@ namespace sp {
 
 struct U4 { uint32_t x, y, z, w; };
 
+// a ^ b ^ c; on the device one gfx950 three-input logic instruction (truth table 0x96)
+SP_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 SP_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
@@ -55,7 +64,7 @@ SP_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
         const uint64_t p1 = (uint64_t)M1 * c.z;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        c = U4{xor3(hi1, c.y, k0), lo1, xor3(hi0, c.w, k1), lo0};
         k0 += W0;
         k1 += W1;
     }
@@ -71,21 +80,25 @@ SP_HD U4 noise_block(uint64_t seed, uint64_t particle, uint64_t call, uint32_t t
 SP_HD double u01_f64(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963e-10; }  // (w+1/2) 2^-32
 
 // ---- FP64 ln u, u = (w + 1/2) 2^-32 ----------------------------------------------
+// The reduction is read off the bits of v = w + 1/2: adding half an exponent unit to the high word
+// carries into the exponent exactly when the mantissa is >= 3/2, which gives the exponent e of
+// v = m 2^e with m in [3/4, 3/2); j = rint(128 (m - 1)) comes from the round-to-integer constant
+// 1.5 * 2^52, whose sum with 128 (m - 1) holds j in its low word.
 SP_HD double log_u01_f64(uint32_t w) {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
     constexpr double LN2_HI = 0x1.62e42fee00000p-1;   // ln 2 to 32 significant bits: E * LN2_HI is exact
     constexpr double LN2_LO = 0x1.a39ef35793c76p-33;  // ln 2 - LN2_HI
+    constexpr double RND = 0x1.8p52;                  // x + RND - RND = rint(x) for |x| < 2^51
     const double v = (double)w + 0.5;                 // exact
-    int e;
-    double m = frexp(v, &e) * 2.0;                    // v = m 2^(e-1), m in [1, 2)
-    e -= 1;
-    if (m >= 1.5) { m = m * 0.5; e += 1; }            // m in [3/4, 3/2)
-    const double jd = rint((m - 1.0) * 128.0);        // exact; in [-32, 64]
-    const double* T = SP_LOG_TAB[(int)jd + 32];       // ln c_j, 1 / c_j
-    const double c = fma(jd, 0.0078125, 1.0);         // c_j, exact
-    const double r = (m - c) * T[1];                  // m - c_j exact
+    const uint32_t be = ((uint32_t)(__builtin_bit_cast(uint64_t, v) >> 32) + 0x80000u) >> 20;   // e + 1023
+    const double u = ldexp(v, 1030 - (int)be) - 128.0;   // 128 (m - 1), exact, in [-32, 64)
+    const double t = u + RND;
+    const double jd = t - RND;                        // j = rint(128 (m - 1)), exact
+    const uint32_t row = (uint32_t)__builtin_bit_cast(uint64_t, t) + 32u;   // j + 32 from the low word
+    const double* T = SP_LOG_TAB[row];                // ln c_j, 2^-7 / c_j
+    const double r = (u - jd) * T[1];                 // (m - c_j) / c_j: 128 (m - c_j) is exact
     double p = 0.14285714285714285;                   // 1/7
     p = fma(p, r, -0.16666666666666666);
     p = fma(p, r, 0.2);
@@ -93,8 +106,29 @@ SP_HD double log_u01_f64(uint32_t w) {
     p = fma(p, r, 0.3333333333333333);
     p = fma(p, r, -0.5);
     const double l1p = fma(p, r * r, r);              // ln(1 + r)
-    const double E = (double)(e - 32);
+    const double E = (double)((int)be - 1055);        // e - 32
     return fma(E, LN2_HI, T[0] + fma(E, LN2_LO, l1p));
+}
+
+// sqrt(x) for the Box-Muller radius, x = -2 ln u in [4e-10, 45]: on the device the correctly rounded
+// square root without the rescaling of tiny arguments or the zero/infinity checks (v_rsq_f64 and two
+// Newton corrections of the root and the half reciprocal), on the host the C library's.
+SP_HD double sqrt_radius(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma clang fp contract(off)
+    double h = __builtin_amdgcn_rsq(x);
+    double s = x * h;
+    h = h * 0.5;
+    const double e = fma(-h, s, 0.5);
+    s = fma(s, e, s);
+    h = fma(h, e, h);
+    double d = fma(-s, s, x);
+    s = fma(d, h, s);
+    d = fma(-s, s, x);
+    return fma(d, h, s);
+#else
+    return sqrt(x);
+#endif
 }
 
 // ---- FP64 sin and cos of 2 pi u, u = (w + 1/2) 2^-32 ------------------------------
@@ -121,8 +155,8 @@ SP_HD void sincos2pi_u01_f64(uint32_t w, double* s_out, double* c_out) {
 
 // four standard normals (FP64) from one Philox block
 SP_HD void box_muller4_f64(U4 w, double z[4]) {
-    const double r0 = sqrt(-2.0 * log_u01_f64(w.x));
-    const double r1 = sqrt(-2.0 * log_u01_f64(w.z));
+    const double r0 = sqrt_radius(-2.0 * log_u01_f64(w.x));
+    const double r1 = sqrt_radius(-2.0 * log_u01_f64(w.z));
     double s0, c0, s1, c1;
     sincos2pi_u01_f64(w.y, &s0, &c0);
     sincos2pi_u01_f64(w.w, &s1, &c1);
