@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 3
+#define SP_ABI_VERSION 4
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -51,8 +51,9 @@ enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_EULER_MARUYAMA_NORMALIZED = 2, SP
 /* Properties::IField (Fields/FieldList.h:48-59) */
 enum { SP_FIELD_ZERO = 1, SP_FIELD_CONSTANT = 2, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4,
        SP_FIELD_TRIANGULAR = 5, SP_FIELD_RECTANGULAR = 6, SP_FIELD_SINC = 7 };
-/* Properties::IAnisotropy (Problems/Anisotropy/AnisotropyList.h:45-54) */
-enum { SP_ANISO_UNIAXIAL = 1, SP_ANISO_CUBIC = 2 };
+/* Properties::IAnisotropy (Problems/Anisotropy/AnisotropyList.h:45-54); the reference's app dispatches
+ * uniaxial, cubic and uniaxialcubic (Simulator/SimulationManager.h:454-476), not mixed (3) */
+enum { SP_ANISO_UNIAXIAL = 1, SP_ANISO_CUBIC = 2, SP_ANISO_UNIAXIALCUBIC = 4 };
 
 /* precision of the bulk arithmetic (no reference counterpart: PREC double,
  * General/GeneralDefines.h:3) */
@@ -111,7 +112,10 @@ typedef struct sp_request {
     double field_time_offset; /* triangular, rectangular: TimeOffset [s] */
     double field_tau;         /* rectangular: Tau [s], time constant of the edges */
     uint32_t field_alternating; /* rectangular: Alternating (0/1) */
-    uint32_t reserved;
+    uint32_t save_interval;   /* Save_single_simulations with Save_intervall (Settings/ResultSettings.h:93-94,
+                                 Results/SimulationResultManager.h:115-149): 0 = off; k > 0 keeps the window
+                                 means of every particle p with (p + 1) % k == 0 (the reference saves every k-th
+                                 finished simulation, in completion order) */
 } sp_request;
 
 /*
@@ -126,11 +130,15 @@ typedef struct sp_particles {
     const double* damping;                   /* [n] */
     const double* gyromagnetic_ratio;        /* [n] rad/(s T) */
     const double* temperature;               /* [n] K */
-    const double* anisotropy_constant;       /* [n] K_uniaxial or K_cubic, J/m^3 */
+    const double* anisotropy_constant;       /* [n] K_uniaxial (uniaxial, uniaxialcubic) or K_cubic (cubic), J/m^3 */
     const double* hydrodynamic_radius;       /* [n] m (BrownAndNeel only, else NULL) */
     const double* viscosity;                 /* [n] Pa s (BrownAndNeel only, else NULL) */
     const double* axes;                      /* [n][9] body axes x,y,z (Neel/NeelSpherical; NULL = identity) */
     const double* initial_state;             /* [n][dim_y]: m (Neel), (theta,phi) (NeelSpherical), (n,m) (BrownAndNeel) */
+    /* uniaxialcubic only (Properties/Anisotropy/UniaxialCubic.hpp:20-51), else NULL */
+    const double* anisotropy_constant_cubic; /* [n] K_cubic of the cubic part, J/m^3 */
+    const double* uniaxial_rotation;         /* [n][3] Euler angles of the uniaxial part (Uniaxial_Rotation): its easy
+                                                axis is R^T z, R the rotation of UniaxialCubicAnisotropy.hpp:61-90, :114 */
 } sp_particles;
 
 /*
@@ -144,7 +152,11 @@ typedef struct sp_result {
     double* mean;         /* [n_out*dim_out] weighted ensemble mean */
     double* weight;       /* [dim_out] sum of weights */
     double* final_state;  /* [n_particles*dim_y] or NULL */
+    double* trajectories; /* save_interval > 0: [n_out][dim_out][n_saved] unweighted window means of the saved
+                             particles (SingleSimulationResult "Results", Results/SingleSimulationResult.h:96-104),
+                             n_saved = n_particles / save_interval; or NULL */
     /* filled by the library */
+    uint64_t n_saved;     /* particles whose trajectories are kept */
     uint64_t n_out;
     uint32_t dim_out;
     uint32_t dim_y;
@@ -241,6 +253,12 @@ const char* sp_settings_savefile(const sp_settings* s);
 /* writes Mean_Results {Time, Field, Results, Weight, Mean_Properties} as JSON
  * (replaces the SerAr MAT/HDF5 writer, Results/SimulationResultManager.h:97-113) */
 int sp_settings_write_result(const sp_settings* s, const sp_result* res, const char* path);
+/* Single_simulations_savefile of [Result_Settings] ("" when unset) */
+const char* sp_settings_single_savefile(const sp_settings* s);
+/* writes the single-simulation results of a request run with save_interval > 0 as JSON, one named entry
+ * per kept particle {UsedProperties, Time, Weight, Field, Results} (replaces the SerAr writer of
+ * SimulationResultManager::addSingleResult, Results/SimulationResultManager.h:115-149) */
+int sp_settings_write_single_results(const sp_settings* s, const sp_result* res, const char* path);
 void sp_settings_free(sp_settings* s);
 
 /* ---- autotuner: times candidate tunings on a short synthetic run ---- */

@@ -77,6 +77,8 @@ public:
     const std::vector<double>& time() const { return time_; }
     const std::vector<double>& mean() const { return mean_; }
     const std::string& result_file() const { return result_file_; }
+    const std::string& single_result_file() const { return single_file_; }   // empty unless singles were saved
+    const std::vector<double>& trajectories() const { return traj_; }        // [n_out][dim_out][n_saved]
 
 private:
     static void check(int rc) {
@@ -95,8 +97,23 @@ private:
         res_.field = field_.data();
         res_.mean = mean_.data();
         res_.weight = weight_.data();
+        if (req_.save_interval) {   // Save_single_simulations: every Save_intervall-th particle's own trajectory
+            traj_.assign(n_out * dout * (req_.n_particles / req_.save_interval), 0.0);
+            res_.trajectories = traj_.data();
+        }
         check(sp_result_get(ctx_, id_, &res_));
         check(sp_settings_write_result(settings_, &res_, result_file_.c_str()));
+        if (res_.n_saved) {
+            const char* sf = sp_settings_single_savefile(settings_);
+            std::string path = sf && *sf ? std::string(sf) : std::string();
+            if (path.empty() || path == result_file_) {   // the reference refuses one file for both
+                path = result_file_;
+                const auto dot = path.find_last_of('.');
+                path = (dot == std::string::npos ? path : path.substr(0, dot)) + "_single.json";
+            }
+            check(sp_settings_write_single_results(settings_, &res_, path.c_str()));
+            single_file_ = path;
+        }
         ProgressCache().store(100);
         collected_ = true;
     }
@@ -107,9 +124,9 @@ private:
     sp_particles parts_{};
     uint64_t id_ = 0;
     bool started_ = false, collected_ = false;
-    std::string result_file_;
+    std::string result_file_, single_file_;
     sp_result res_{};
-    std::vector<double> time_, field_, mean_, weight_;
+    std::vector<double> time_, field_, mean_, weight_, traj_;
 };
 
 }  // namespace SimulationApplication

@@ -9,10 +9,12 @@ using namespace oracle;
 
 namespace {
 MagneticProperties magFrom(const double* pr, int aniso) {
-    // pr = [radius, Ms, alpha, gamma, T, K, r_h, viscosity]
+    // pr = [radius, Ms, alpha, gamma, T, K, r_h, viscosity] (+ [K_cubic, rotation(3)] for uniaxialcubic)
     MagneticProperties mp{pr[0], pr[1], pr[2], pr[3], pr[5], aniso};
+    if (aniso == 4) { mp.K2 = pr[8]; mp.urot[0] = pr[9]; mp.urot[1] = pr[10]; mp.urot[2] = pr[11]; }
     return mp;
 }
+int propsStride(int aniso) { return aniso == 4 ? 12 : 8; }
 Axes axesFrom(const double* a) {
     Axes ax;
     if (!a) { ax.x = v3(1.0, 0.0, 0.0); ax.y = v3(0.0, 1.0, 0.0); ax.z = v3(0.0, 0.0, 1.0); return ax; }
@@ -62,6 +64,7 @@ void or_aniso_field(int aniso, const double* props, const double* m, const doubl
     Vec<double, 3> H;
     Mat<double, 3, 3> J;
     if (aniso == 1) { Uniaxial u(mp); H = u.field(mv, ax); J = u.jacobian(ax); }
+    else if (aniso == 4) { UniaxialCubic uc(mp); H = uc.field(mv, ax); J = Mat<double, 3, 3>::zero(); }
     else { Cubic c(mp); H = c.field(mv, ax); J = c.jacobian(mv, ax); }
     for (int i = 0; i < 3; ++i) out[i] = H[i];
     if (jac) for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) jac[i * 3 + j] = J(i, j);
@@ -132,8 +135,8 @@ int or_eval(int problem, int aniso, const double* props, const double* axes, int
         if (what == 2) { auto M = P.sto(yv); for (int i = 0; i < 6; ++i) for (int j = 0; j < 6; ++j) out[i * 6 + j] = M(i, j); return 0; }
         return 1;
     };
-    if (problem == PROB_NEEL) return aniso == 1 ? neel(Uniaxial(mp)) : neel(Cubic(mp));
-    if (problem == PROB_SPHERICAL) return aniso == 1 ? sph(Uniaxial(mp)) : sph(Cubic(mp));
+    if (problem == PROB_NEEL) return aniso == 1 ? neel(Uniaxial(mp)) : (aniso == 4 ? neel(UniaxialCubic(mp)) : neel(Cubic(mp)));
+    if (problem == PROB_SPHERICAL) return aniso == 1 ? sph(Uniaxial(mp)) : (aniso == 4 ? sph(UniaxialCubic(mp)) : sph(Cubic(mp)));
     if (problem == PROB_BN) {
         const bool simple = flags & 1;
         if (aniso == 1) return simple ? bn(Uniaxial(mp), std::true_type{}) : bn(Uniaxial(mp), std::false_type{});
@@ -198,9 +201,10 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
     const uint64_t N = rs.nParticles;
     const int dy = rs.problem == PROB_NEEL ? 3 : (rs.problem == PROB_SPHERICAL ? 2 : 6);
     std::vector<ParticleIn> parts(N);
+    const int ps = propsStride(rs.aniso);
     for (uint64_t p = 0; p < N; ++p) {
-        parts[p].mp = magFrom(props + p * 8, rs.aniso);
-        parts[p].T = props[p * 8 + 4]; parts[p].r_h = props[p * 8 + 6]; parts[p].visc = props[p * 8 + 7];
+        parts[p].mp = magFrom(props + p * ps, rs.aniso);
+        parts[p].T = props[p * ps + 4]; parts[p].r_h = props[p * ps + 6]; parts[p].visc = props[p * ps + 7];
         parts[p].ax = axesFrom(axes ? axes + p * 9 : nullptr);
         for (int i = 0; i < dy; ++i) parts[p].y0[i] = y0[p * dy + i];
     }
@@ -209,18 +213,21 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
     const auto t0 = std::chrono::steady_clock::now();
     const bool keep = final_state != nullptr;
     std::vector<double>* ppo = per_particle ? &pp : nullptr;
+#define NEEL_RUN(A) runEnsemble<NeelProblem<A>, CartesianStepper<NeelProblem<A>>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelProblem<A>(q.mp, q.T, q.ax); }, rr, nThreads, keep, ppo)
+#define SPH_RUN(A) runEnsemble<NeelSphericalProblem<A>, SphericalStepper<A>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelSphericalProblem<A>(q.mp, q.T, q.ax, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo)
     if (rs.problem == PROB_NEEL) {
-        if (rs.aniso == 1) runEnsemble<NeelProblem<Uniaxial>, CartesianStepper<NeelProblem<Uniaxial>>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelProblem<Uniaxial>(q.mp, q.T, q.ax); }, rr, nThreads, keep, ppo);
-        else runEnsemble<NeelProblem<Cubic>, CartesianStepper<NeelProblem<Cubic>>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelProblem<Cubic>(q.mp, q.T, q.ax); }, rr, nThreads, keep, ppo);
+        if (rs.aniso == 1) NEEL_RUN(Uniaxial); else if (rs.aniso == 4) NEEL_RUN(UniaxialCubic); else NEEL_RUN(Cubic);
     } else if (rs.problem == PROB_SPHERICAL) {
-        if (rs.aniso == 1) runEnsemble<NeelSphericalProblem<Uniaxial>, SphericalStepper<Uniaxial>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelSphericalProblem<Uniaxial>(q.mp, q.T, q.ax, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo);
-        else runEnsemble<NeelSphericalProblem<Cubic>, SphericalStepper<Cubic>>(rs, parts.data(), [&](const ParticleIn& q) { return NeelSphericalProblem<Cubic>(q.mp, q.T, q.ax, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo);
+        if (rs.aniso == 1) SPH_RUN(Uniaxial); else if (rs.aniso == 4) SPH_RUN(UniaxialCubic); else SPH_RUN(Cubic);
     } else if (rs.problem == PROB_BN) {
+        if (rs.aniso == 4) return 2;   // the reference's torque is NaN for this combination
 #define BN_RUN(A, S) runEnsemble<BrownAndNeelProblem<A, S>, CartesianStepper<BrownAndNeelProblem<A, S>>>(rs, parts.data(), [&](const ParticleIn& q) { return BrownAndNeelProblem<A, S>(q.mp, q.T, q.r_h, q.visc); }, rr, nThreads, keep, ppo)
         if (rs.aniso == 1) { if (rs.bnSimple) BN_RUN(Uniaxial, true); else BN_RUN(Uniaxial, false); }
         else { if (rs.bnSimple) BN_RUN(Cubic, true); else BN_RUN(Cubic, false); }
 #undef BN_RUN
     } else return 2;
+#undef NEEL_RUN
+#undef SPH_RUN
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::copy(rr.mean.begin(), rr.mean.end(), mean);
     std::copy(rr.weight.begin(), rr.weight.end(), weight);

@@ -113,7 +113,9 @@ inline double sphere_volume(double r) { return 4.0 / 3.0 * PI * r * r * r; }
 
 struct MagneticProperties {          // Properties/MagneticProperties.h:100-267
     double radius, Ms, alpha, gamma, K;   // K: K_uniaxial or K_cubic
-    int aniso;                            // 1 uniaxial, 2 cubic
+    int aniso;                            // 1 uniaxial, 2 cubic, 4 uniaxialcubic
+    double K2 = 0.0;                      // uniaxialcubic: K_cubic (K is K_uniaxial)
+    double urot[3] = {0.0, 0.0, 0.0};     // uniaxialcubic: Uniaxial_Rotation (UniaxialCubic.hpp:26)
     double volume() const { return sphere_volume(radius); }           // :159
     double moment() const { return Ms * volume(); }                   // :166
 };
@@ -199,6 +201,38 @@ struct Cubic {
                 J(r, s) = J(r, s) + K1_MS * (a[i][r] * ((c[j] * c[j] + c[k] * c[k]) * a[i][s] + 2.0 * c[i] * c[j] * a[j][s] + 2.0 * c[i] * c[k] * a[k][s]));
         }
         return J;
+    }
+};
+
+// UniaxialCubicAnisotropy.hpp:40-146: a uniaxial part about u = R^T z (R from the Euler angles of
+// Uniaxial_Rotation, calcRotation :61-90) plus a cubic part about the particle's axes (:114).  Only the
+// field is restated: the problems this oracle runs with it (Neel, NeelSpherical) use no torque.
+struct UniaxialCubic {
+    double prefactorField;       // -2 K_uniaxial / Ms  (UniaxialAnisotropy.h:71-77)
+    Cubic cubic;                 // -2 K_cubic / Ms
+    Mat<double, 3, 3> R;
+    static MagneticProperties cubicPart(const MagneticProperties& mp) { MagneticProperties c = mp; c.K = mp.K2; return c; }
+    explicit UniaxialCubic(const MagneticProperties& mp)
+        : prefactorField(-2.0 * mp.K / mp.Ms), cubic(cubicPart(mp)), R(rotation(mp.urot)) {}
+    static Mat<double, 3, 3> rotation(const double* e) {                     // :61-90
+        const Vec<double, 3> ang = v3(e[0], e[1], e[2]);
+        const double cphi = std::cos(ang[0]), ctheta = std::cos(ang[1]), cpsi = std::cos(ang[2]);
+        const double sphi = std::sin(ang[0]), stheta = std::sin(ang[1]), spsi = std::sin(ang[2]);
+        const double cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
+        Mat<double, 3, 3> r;
+        r(0, 0) = cphicpsi - ctheta * sphispsi;  r(0, 1) = -sphicpsi - ctheta * cphispsi; r(0, 2) = stheta * spsi;
+        r(1, 0) = ctheta * sphicpsi + cphispsi;  r(1, 1) = ctheta * cphicpsi - sphispsi;  r(1, 2) = -stheta * cpsi;
+        r(2, 0) = stheta * sphi;                 r(2, 1) = stheta * cphi;                 r(2, 2) = ctheta;
+        return r;
+    }
+    Vec<double, 3> easyAxis(const Axes& ax) const {                          // uniaxial_rotation.transpose() * zi
+        Vec<double, 3> u;
+        for (int i = 0; i < 3; ++i) u[i] = R(0, i) * ax.z[0] + R(1, i) * ax.z[1] + R(2, i) * ax.z[2];
+        return u;
+    }
+    template <class S> Vec<S, 3> field(const Vec<S, 3>& m, const Axes& ax) const {
+        const Vec<S, 3> u = lift3<S>(easyAxis(ax));
+        return (prefactorField * dot(m, u)) * u + cubic.field(m, ax);
     }
 };
 

@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libstophys_gpu.so"
 PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3)
 SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
 FIELD = dict(none=1, constant=2, sinusoidal=3, lissajous=4, triangular=5, rectangular=6, sinc=7)
-ANISO = dict(uniaxial=1, cubic=2)
+ANISO = dict(uniaxial=1, cubic=2, uniaxialcubic=4)
 PREC = dict(fp64=0, mixed=1)
 NOISE = dict(philox=0, replay=1)
 FLAG_BN_SIMPLE, FLAG_COORD_TRANSFORM, FLAG_KEEP_FINAL, FLAG_NO_FALLBACK, FLAG_SKIP_CHECK = 1, 2, 4, 8, 16
@@ -22,7 +22,7 @@ FLAG_RENORMALIZE = 32
 DIM_Y = {1: 3, 2: 2, 3: 6}
 DIM_W = {1: 3, 2: 3, 3: 6}
 DIM_OUT = {1: 3, 2: 3, 3: 6}
-NPARAM = 19
+NPARAM = 23
 
 SP_OK, SP_ERR_NOT_CONVERGED, SP_ERR_TIMEOUT, SP_ERR_NOT_READY = 0, 5, 7, 9
 
@@ -42,18 +42,20 @@ class Request(C.Structure):
                 ("field_freq", C.c_double * 3), ("field_phase", C.c_double * 3),
                 ("replay_normals", C.POINTER(C.c_double)), ("replay_count", C.c_uint64),
                 ("field_period", C.c_double), ("field_time_offset", C.c_double), ("field_tau", C.c_double),
-                ("field_alternating", C.c_uint32), ("reserved", C.c_uint32)]
+                ("field_alternating", C.c_uint32), ("save_interval", C.c_uint32)]
 
 
 class Particles(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in (
         "magnetic_radius", "saturation_magnetisation", "damping", "gyromagnetic_ratio", "temperature",
-        "anisotropy_constant", "hydrodynamic_radius", "viscosity", "axes", "initial_state")]
+        "anisotropy_constant", "hydrodynamic_radius", "viscosity", "axes", "initial_state",
+        "anisotropy_constant_cubic", "uniaxial_rotation")]
 
 
 class Result(C.Structure):
     _fields_ = [("time", C.POINTER(C.c_double)), ("field", C.POINTER(C.c_double)), ("mean", C.POINTER(C.c_double)),
                 ("weight", C.POINTER(C.c_double)), ("final_state", C.POINTER(C.c_double)),
+                ("trajectories", C.POINTER(C.c_double)), ("n_saved", C.c_uint64),
                 ("n_out", C.c_uint64), ("dim_out", C.c_uint32), ("dim_y", C.c_uint32),
                 ("device_ms", C.c_double), ("latency_ms", C.c_double), ("particle_steps", C.c_double),
                 ("check_error", C.c_double), ("check_noise", C.c_double), ("used_fallback", C.c_uint32),
@@ -72,7 +74,8 @@ EXPORTS = [
     "sp_ctx_set_memory_budget", "sp_ctx_last_run_stats", "sp_run", "sp_run_batch", "sp_submit", "sp_wait",
     "sp_result_get", "sp_cancel", "sp_server_stats", "sp_autotune", "sp_dev_philox", "sp_dev_normals", "sp_dev_box_muller", "sp_host_box_muller",
     "sp_host_normals", "sp_prepare", "sp_execute", "sp_release", "sp_settings_load", "sp_settings_request",
-    "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free",
+    "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free", "sp_settings_single_savefile",
+    "sp_settings_write_single_results",
 ]
 
 _LIB = None
@@ -117,6 +120,9 @@ def lib():
         L.sp_settings_savefile.argtypes = [C.c_void_p]
         L.sp_settings_savefile.restype = C.c_char_p
         L.sp_settings_write_result.argtypes = [C.c_void_p, P(Result), C.c_char_p]
+        L.sp_settings_single_savefile.argtypes = [C.c_void_p]
+        L.sp_settings_single_savefile.restype = C.c_char_p
+        L.sp_settings_write_single_results.argtypes = [C.c_void_p, P(Result), C.c_char_p]
         L.sp_settings_free.argtypes = [C.c_void_p]
         L.sp_settings_free.restype = None
         L.sp_host_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d]

@@ -129,3 +129,22 @@ def serving_request(rng, idx, n_range=(256, 4096), step_range=(2_000, 10_000), o
                     K=full(n, -1.1e4), axes=random_axes(rng, n), y0=to_spherical(unit_sphere(rng, n)),
                     coord_transform=True, min_angle=math.pi / 36, newton_tol=1e-10, newton_max_iter=20,
                     field_amplitude=(0.012, 0.012, 0.0), field_freq=(25.25e3, 24.5e3, 0.0))
+
+
+def uniaxialcubic(n=4096, problem="Neel", solver="Heun_Strong", n_steps=20_000, oversampling=100, seed=66,
+                  precision="fp64", Ku=8e3, Kc=-1.1e4, T=300.0):
+    """Uniaxialcubic anisotropy (UniaxialCubicAnisotropy.hpp): a uniaxial part rotated against the cube axes by
+    per-particle Euler angles (phi U(0, 2 pi), theta U(0, pi), psi U(0, 2 pi), as UseRandomCubicOrientation draws
+    them), random cube orientations, a 2D Lissajous field; Neel (Cartesian) or NeelSpherical with chart rotation."""
+    rng = np.random.default_rng(seed)
+    radius = 11e-9 * np.exp(0.1 * rng.standard_normal(n))
+    axes = random_axes(rng, n)
+    rot = rng.uniform([0.0, 0.0, 0.0], [2 * math.pi, math.pi, 2 * math.pi], size=(n, 3))
+    m0 = unit_sphere(rng, n)
+    y0 = to_spherical(m0) if problem == "NeelSpherical" else m0
+    return Ensemble(problem=problem, solver=solver, field="lissajous", anisotropy="uniaxialcubic", precision=precision,
+                    n_steps=n_steps, oversampling=oversampling, dt=2e-12, seed=seed, radius=radius,
+                    Ms=full(n, 4.8e5), alpha=full(n, 0.1), gamma=full(n, GAMMA), T=full(n, T), K=full(n, Ku),
+                    K_cubic=full(n, Kc), uniaxial_rotation=rot, axes=axes, y0=y0,
+                    coord_transform=problem == "NeelSpherical", min_angle=math.pi / 36,
+                    field_amplitude=(0.012, 0.012, 0.0), field_freq=(25.25e3, 24.5e3, 0.0))

@@ -37,10 +37,13 @@ struct Dist {
 
 struct ParticleFile {
     double T = 295.0, radius = 0, Ms = 0, alpha = 0, gamma = 0, K = 0, r_h = 0, visc = 0;
+    double K2 = 0;                           // uniaxialcubic: K_cubic (K is K_uniaxial)
+    double urot[3] = {0, 0, 0};              // uniaxialcubic: Uniaxial_Rotation (Euler angles)
+    bool rand_urot = false;                  // uniaxialcubic: UseRandomCubicOrientation
     int aniso = SP_ANISO_UNIAXIAL;
     bool rand_orient = false, rand_mag = false;
     double euler[3] = {0, 0, 0}, mag[3] = {0, 0, 1};
-    Dist rad, shell, aniso_dist;
+    Dist rad, shell, aniso_dist, aniso_dist2;
     uint64_t count = 0;
 };
 
@@ -48,9 +51,9 @@ struct Settings {
     sp_request rq{};
     bool discrete = false;
     std::vector<ParticleFile> particles;
-    std::string savefile;
+    std::string savefile, single_savefile, single_prefix = "Single";
     // materialised arrays
-    std::vector<double> radius, Ms, alpha, gamma, T, K, r_h, visc, axes, y0;
+    std::vector<double> radius, Ms, alpha, gamma, T, K, r_h, visc, axes, y0, K2, urot;
 };
 
 static const std::string* opt(const ini::File& f, const std::string& sec, std::initializer_list<const char*> keys) {
@@ -89,7 +92,15 @@ static ParticleFile load_particle(const std::string& path) {
     p.gamma = num(f, MP, {"Gyromagnetic_ratio"}, 0, true);
     const std::string* at = opt(f, MP, {"Type_of_anisotropy"});
     p.aniso = at ? parse_enum_or_throw("anisotropy", *at) : SP_ANISO_UNIAXIAL;
-    p.K = num(f, PP, {p.aniso == SP_ANISO_CUBIC ? "K_cubic" : "K_uniaxial", "Anisotropy_1"}, 0, true);
+    if (p.aniso == SP_ANISO_UNIAXIALCUBIC) {
+        // Anisotropy_Properties.{Uniaxial_Part, Cubic_Part}, Uniaxial_Rotation (UniaxialCubic.hpp:45-50)
+        p.K = num(f, PP, {"K_uniaxial"}, 0, true);
+        p.K2 = num(f, PP, {"K_cubic"}, 0, true);
+        const auto ur = vec(f, PP, {"Uniaxial_Rotation"});
+        for (int i = 0; i < 3 && i < (int)ur.size(); ++i) p.urot[i] = ur[i];
+    } else {
+        p.K = num(f, PP, {p.aniso == SP_ANISO_CUBIC ? "K_cubic" : "K_uniaxial", "Anisotropy_1"}, 0, true);
+    }
     const std::string HP = PP + ".Hydrodynamic_Properties";
     p.r_h = num(f, HP, {"Hydrodynamic_Radius"}, 0.0);
     p.visc = num(f, HP, {"Viscosity"}, 0.0);
@@ -106,6 +117,20 @@ static ParticleFile load_particle(const std::string& path) {
     if (const std::string* v = opt(f, SS, {"Hydrodynamic_shell_distribution_type"})) p.shell.type = *v;
     p.shell.width = num(f, SS, {"Hydrodynamic_shell_distribution_width"}, 0.0);
     p.shell.relative = flag(f, SS, {"Use_relative_distribution_width_for_hydrodynamic_shell"}, true);
+    if (p.aniso == SP_ANISO_UNIAXIALCUBIC) {
+        // Anisotropy_Distribution.{Uniaxial_Distribution, Cubic_Distribution}, UseRandomCubicOrientation
+        // (UniaxialCubic.hpp:81-86, Uniaxial.hpp:86-101, Cubic.hpp:79-94)
+        const std::string AD = SS + ".Anisotropy_Distribution";
+        auto read = [&](const std::string& sec, const char* sigma, Dist& d) {
+            if (const std::string* v = f.find(sec, "Distribution_type")) d.type = *v;
+            if (const std::string* v = f.find(sec, sigma)) d.width = ini::to_double(*v);
+            if (const std::string* v = f.find(sec, "Use_relative_distribution_width")) d.relative = ini::to_bool(*v);
+        };
+        read(AD + ".Uniaxial_Distribution", "Sigma_K_uniaxial", p.aniso_dist);
+        read(AD + ".Cubic_Distribution", "Sigma_K_cubic", p.aniso_dist2);
+        p.rand_urot = flag(f, AD, {"UseRandomCubicOrientation"}, false);
+        return p;
+    }
     if (const std::string* v = opt(f, SS, {"Distribution_type", "Anisotropy_constants_distribution_type"})) p.aniso_dist.type = *v;
     const auto w = vec(f, SS, {"Sigma_K_uniaxial", "Sigma_K_cubic", "Anisotropy_constants_distribution_width"});
     p.aniso_dist.width = w.empty() ? 0.0 : w[0];
@@ -203,7 +228,13 @@ Settings* load(const std::string& path) {
         }
     }
     // [Result_Settings] (ResultSettings.h:90-118)
-    if (const std::string* sf = opt(f, "Result_Settings", {"Savefile"})) s->savefile = *sf;
+    const std::string RS = "Result_Settings";
+    if (const std::string* sf = opt(f, RS, {"Savefile"})) s->savefile = *sf;
+    if (flag(f, RS, {"Save_single_simulations"}, false)) {
+        r.save_interval = (uint32_t)std::max(1.0, num(f, RS, {"Save_intervall"}, 1.0));
+        if (const std::string* v = opt(f, RS, {"Single_simulations_savefile"})) s->single_savefile = *v;
+        if (const std::string* v = opt(f, RS, {"Save_single_name_prefix"})) s->single_prefix = *v;
+    }
     // [Particle_Provider] (ParticleProvider.h:230-295)
     const std::string PP = "Particle_Provider";
     s->discrete = flag(f, PP, {"Use_discrete_distribution_to_select_particle"}, false);
@@ -223,6 +254,7 @@ Settings* load(const std::string& path) {
     if (request_dims(r.problem, &dy, &dw, &dout)) throw std::runtime_error("problem type not available on the GPU backend");
     s->radius.resize(N); s->Ms.resize(N); s->alpha.resize(N); s->gamma.resize(N); s->T.resize(N); s->K.resize(N);
     s->r_h.resize(N); s->visc.resize(N); s->axes.resize(N * 9); s->y0.resize(N * dy);
+    if (s->particles[0].aniso == SP_ANISO_UNIAXIALCUBIC) { s->K2.resize(N); s->urot.resize(N * 3); }
     std::vector<uint64_t> used(s->particles.size(), 0);
     uint64_t total_count = 0;
     for (auto& p : s->particles) total_count += p.count;
@@ -244,6 +276,11 @@ Settings* load(const std::string& path) {
         s->radius[i] = rad; s->Ms[i] = pf.Ms; s->alpha[i] = pf.alpha; s->gamma[i] = pf.gamma; s->T[i] = pf.T;
         s->K[i] = draw_dist(pf.aniso_dist, pf.K, rng);
         s->r_h[i] = shell + rad; s->visc[i] = pf.visc;
+        if (pf.aniso == SP_ANISO_UNIAXIALCUBIC) {      // UniaxialCubic_Distribution::applyDistribution (:63-76)
+            s->K2[i] = draw_dist(pf.aniso_dist2, pf.K2, rng);
+            const double scale[3] = {6.283185307179586, 3.141592653589793, 6.283185307179586};
+            for (int k = 0; k < 3; ++k) s->urot[i * 3 + k] = pf.rand_urot ? rng.uniform() * scale[k] : pf.urot[k];
+        }
         // axes from Euler angles (NeelRelaxation.h:223-261); random: each angle U(0, 2 pi)
         double e[3] = {pf.euler[0], pf.euler[1], pf.euler[2]};
         if (pf.rand_orient) for (double& x : e) x = rng.uniform() * 6.283185307179586;
@@ -313,6 +350,47 @@ int write_result(const Settings& s, const sp_result& res, const std::string& pat
     return o ? SP_OK : set_error(SP_ERR_INVALID_ARG, "write failed: " + path);
 }
 
+// Single-simulation results (SimulationResultManager.h:115-149, SingleSimulationResult::save :96-104):
+// one named entry per kept particle with its properties, Time, Weight, Field and its own (unweighted)
+// window means.  The entry of particle p is named <Save_single_name_prefix>_<p + 1>: the reference numbers
+// its entries by completion order, which has no fixed meaning; here the number is the particle's.
+int write_single_results(const Settings& s, const sp_result& res, const std::string& path) {
+    if (!res.trajectories || res.n_saved == 0) return set_error(SP_ERR_INVALID_ARG, "the result holds no single-simulation outputs");
+    std::ofstream o(path);
+    if (!o) return set_error(SP_ERR_INVALID_ARG, "cannot write " + path);
+    const uint64_t k = s.rq.save_interval, S = res.n_saved, D = res.dim_out;
+    char buf[512];
+    o << '{';
+    std::vector<double> row(D);
+    for (uint64_t j = 0; j < S; ++j) {
+        const uint64_t p = (j + 1) * k - 1;
+        const double V = 4.0 / 3.0 * 3.14159265358979323846 * s.radius[p] * s.radius[p] * s.radius[p];
+        const double mu = s.Ms[p] * V;
+        std::snprintf(buf, sizeof buf,
+                      "%s\"%s_%llu\":{\"UsedProperties\":{\"Temperature\":%.17g,\"Magnetic_radius\":%.17g,"
+                      "\"Saturation_magnetisation\":%.17g,\"Damping_constant\":%.17g,\"Gyromagnetic_ratio\":%.17g,"
+                      "\"Anisotropy_constant\":%.17g,\"Hydrodynamic_Radius\":%.17g,\"Viscosity\":%.17g},\"Time\":",
+                      j ? "," : "", s.single_prefix.c_str(), (unsigned long long)(p + 1), s.T[p], s.radius[p], s.Ms[p],
+                      s.alpha[p], s.gamma[p], s.K[p], s.r_h[p], s.visc[p]);
+        o << buf;
+        write_array(o, res.time, res.n_out);
+        for (uint64_t c = 0; c < D; ++c) row[c] = (D == 6 && c < 3) ? 1.0 : mu;   // getWeighting of the problem
+        o << ",\"Weight\":";
+        write_array(o, row.data(), D);
+        o << ",\"Field\":[";
+        for (uint64_t w = 0; w < res.n_out; ++w) { if (w) o << ','; write_array(o, res.field + 3 * w, 3); }
+        o << "],\"Results\":[";
+        for (uint64_t w = 0; w < res.n_out; ++w) {
+            for (uint64_t c = 0; c < D; ++c) row[c] = res.trajectories[(w * D + c) * S + j];
+            if (w) o << ',';
+            write_array(o, row.data(), D);
+        }
+        o << "]}";
+    }
+    o << "}\n";
+    return o ? SP_OK : set_error(SP_ERR_INVALID_ARG, "write failed: " + path);
+}
+
 }  // namespace sp::frontend
 
 struct sp_settings {
@@ -348,8 +426,17 @@ int sp_settings_request(const sp_settings* h, sp_request* req, sp_particles* par
         parts->viscosity = s.visc.data();
         parts->axes = s.axes.data();
         parts->initial_state = s.y0.data();
+        parts->anisotropy_constant_cubic = s.K2.empty() ? nullptr : s.K2.data();
+        parts->uniaxial_rotation = s.urot.empty() ? nullptr : s.urot.data();
     }
     return SP_OK;
+}
+
+const char* sp_settings_single_savefile(const sp_settings* h) { return (h && h->s) ? h->s->single_savefile.c_str() : nullptr; }
+
+int sp_settings_write_single_results(const sp_settings* h, const sp_result* res, const char* path) {
+    if (!h || !h->s || !res || !path) return sp::set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    return sp::frontend::write_single_results(*h->s, *res, path);
 }
 
 const char* sp_settings_savefile(const sp_settings* h) { return (h && h->s) ? h->s->savefile.c_str() : nullptr; }

@@ -120,26 +120,37 @@ __device__ __forceinline__ void normals12(const DevRequest& rq, uint64_t p, uint
 __device__ __forceinline__ double ld(const DevRequest& rq, int k, uint64_t p) { return rq.params[(uint64_t)k * rq.n_particles + p]; }
 
 // ---------------------------------------------------------------- anisotropy
-template <class R> struct AnisoLane { R k1; V3<R> ex, ey, ez; };
+// ANISO is the reference's IAnisotropy value: 1 uniaxial, 2 cubic, 4 uniaxialcubic
+template <class R> struct AnisoLane { R k1; V3<R> ex, ey, ez; R ku; V3<R> eu; };
 template <int ANISO, class R>
 __device__ __forceinline__ void load_aniso(const DevRequest& rq, uint64_t p, AnisoLane<R>& A) {
     A.k1 = (R)ld(rq, P_K1, p);
     A.ez = {(R)ld(rq, P_AX + 6, p), (R)ld(rq, P_AX + 7, p), (R)ld(rq, P_AX + 8, p)};
-    if (ANISO == 2) {
+    if (ANISO != 1) {
         A.ex = {(R)ld(rq, P_AX + 0, p), (R)ld(rq, P_AX + 1, p), (R)ld(rq, P_AX + 2, p)};
         A.ey = {(R)ld(rq, P_AX + 3, p), (R)ld(rq, P_AX + 4, p), (R)ld(rq, P_AX + 5, p)};
     } else {
         A.ex = A.ey = {(R)0, (R)0, (R)0};
     }
+    if (ANISO == 4) {
+        A.ku = (R)ld(rq, P_KU, p);
+        A.eu = {(R)ld(rq, P_U + 0, p), (R)ld(rq, P_U + 1, p), (R)ld(rq, P_U + 2, p)};
+    } else {
+        A.ku = (R)0;
+        A.eu = {(R)0, (R)0, (R)0};
+    }
 }
 // UniaxialAnisotropy.h:141-144: H = (-2K/Ms)(m.z) z
 // CubicAnisotropy.h:109-146:   H = K1_MS sum_i c_i (c_j^2 + c_k^2) axis_i, c_i = m.axis_i
+// UniaxialCubicAnisotropy.hpp:114: H = uniaxial field about u = R^T z (-2K_u/Ms) + cubic field (-2K_c/Ms)
 template <int ANISO, class R>
 __device__ __forceinline__ V3<R> aniso_field(const AnisoLane<R>& A, V3<R> m) {
     if (ANISO == 1) return (A.k1 * dot(m, A.ez)) * A.ez;
     const R c1 = dot(m, A.ex), c2 = dot(m, A.ey), c3 = dot(m, A.ez);
     const R c1s = c1 * c1, c2s = c2 * c2, c3s = c3 * c3;
-    return A.k1 * (((c2s + c3s) * c1) * A.ex + ((c1s + c3s) * c2) * A.ey + ((c1s + c2s) * c3) * A.ez);
+    const V3<R> hc = A.k1 * (((c2s + c3s) * c1) * A.ex + ((c1s + c3s) * c2) * A.ey + ((c1s + c2s) * c3) * A.ez);
+    if (ANISO == 2) return hc;
+    return (A.ku * dot(m, A.eu)) * A.eu + hc;
 }
 
 enum { S_EM = 1, S_IM = 3, S_MIL = 6, S_HEUN = 7 };
@@ -539,7 +550,10 @@ template <int N, int ANISO> __device__ __forceinline__ DV3<N> daniso(const Aniso
     const DV3<N> ex = dconst<N>(A.ex), ey = dconst<N>(A.ey);
     const Dual<N> c1 = ddot(m, ex), c2 = ddot(m, ey), c3 = ddot(m, ez);
     const Dual<N> c1s = c1 * c1, c2s = c2 * c2, c3s = c3 * c3;
-    return A.k1 * (((c2s + c3s) * c1) * ex + ((c1s + c3s) * c2) * ey + ((c1s + c2s) * c3) * ez);
+    const DV3<N> hc = A.k1 * (((c2s + c3s) * c1) * ex + ((c1s + c3s) * c2) * ey + ((c1s + c2s) * c3) * ez);
+    if (ANISO == 2) return hc;
+    const DV3<N> eu = dconst<N>(A.eu);
+    return (A.ku * ddot(m, eu)) * eu + hc;
 }
 
 // linear solve A x = b (DY <= 6), Gaussian elimination with partial pivoting
@@ -776,6 +790,13 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     }
     const uint32_t OS = rq.oversampling;
     uint32_t row = 0;
+    // single-simulation outputs (SimulationResultManager.h:115-149): particle p is kept when
+    // (p + 1) % save_interval == 0, as slot (p + 1) / save_interval - 1; window rows are global
+    double* const traj = rq.traj;
+    const uint64_t n_saved = rq.n_saved;
+    const uint64_t first_window = tw.step_begin == 0 ? 0 : tw.step_begin / OS + 1;
+    uint64_t slot = ~0ull;
+    if (traj != nullptr && active && (pr + 1) % rq.save_interval == 0) slot = (pr + 1) / rq.save_interval - 1;
     auto flush = [&](const double* v) {
         double sums[DOUT];
 #pragma unroll
@@ -783,6 +804,10 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
         if (lane == 0) {
 #pragma unroll
             for (int c = 0; c < DOUT; ++c) tw.partial[((uint64_t)row * DOUT + c) * tw.pstride + tw.pslot] = sums[c];
+        }
+        if (slot != ~0ull) {
+#pragma unroll
+            for (int c = 0; c < DOUT; ++c) traj[((first_window + row) * DOUT + c) * n_saved + slot] = v[c];
         }
         ++row;
     };

@@ -45,6 +45,7 @@ struct HostRequest {
     std::vector<double> replay;   // [K][N]
     double wsum[6] = {0}, wsum_check[6] = {0};
     uint32_t n_check_tiles = 0;
+    uint64_t n_saved = 0;         // particles with single-simulation outputs (save_interval)
 };
 
 inline uint64_t gcd64(uint64_t a, uint64_t b) { while (b) { const uint64_t t = a % b; a = b; b = t; } return a; }
@@ -62,8 +63,18 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if ((rq->field == SP_FIELD_TRIANGULAR || rq->field == SP_FIELD_RECTANGULAR || rq->field == SP_FIELD_SINC) &&
         !(rq->field_period > 0.0))
         return set_error(SP_ERR_INVALID_ARG, "the field's Periode must be > 0");
-    if (rq->anisotropy != SP_ANISO_UNIAXIAL && rq->anisotropy != SP_ANISO_CUBIC)
+    if (rq->anisotropy != SP_ANISO_UNIAXIAL && rq->anisotropy != SP_ANISO_CUBIC && rq->anisotropy != SP_ANISO_UNIAXIALCUBIC)
         return set_error(SP_ERR_UNSUPPORTED, "anisotropy type not available on the GPU backend");
+    if (rq->anisotropy == SP_ANISO_UNIAXIALCUBIC) {
+        // BrownAndNeel passes zero x/y axes and zero Euler sines/cosines to the anisotropy
+        // (BrownAndNeelRelaxation.inl:236-242); the cubic part's torque then divides 0 by sin(theta) = 0
+        // (CubicAnisotropy.h:183-192) and the reference's deterministic vector is NaN for every state
+        if (rq->problem == SP_PROBLEM_BROWN_AND_NEEL)
+            return set_error(SP_ERR_UNSUPPORTED, "BrownAndNeel with uniaxialcubic anisotropy: the reference's effective "
+                                                 "torque is 0/0 (NaN) for this combination");
+        if (!pt || !pt->anisotropy_constant_cubic || !pt->uniaxial_rotation)
+            return set_error(SP_ERR_INVALID_ARG, "uniaxialcubic needs anisotropy_constant_cubic and uniaxial_rotation");
+    }
     if (rq->precision != SP_PREC_FP64 && rq->precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
     if (rq->noise != SP_NOISE_PHILOX && rq->noise != SP_NOISE_REPLAY) return set_error(SP_ERR_INVALID_ARG, "bad noise source");
     if (rq->n_particles == 0 || rq->n_particles > 0xFFFFFFFFull) return set_error(SP_ERR_INVALID_ARG, "n_particles must be in [1, 2^32)");
@@ -117,6 +128,7 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
         hr.rq.replay_count = K;
     }
     hr.rq.replay_normals = nullptr;
+    hr.n_saved = rq->save_interval ? hr.N / rq->save_interval : 0;
     hr.n_check_tiles = check_tiles_for(hr.n_tiles);
     const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
     for (uint32_t c = 0; c < hr.dout; ++c) { hr.wsum[c] = 0.0; hr.wsum_check[c] = 0.0; }
@@ -140,12 +152,16 @@ struct ReqDev {
     double* check = nullptr;      // [n_out*dout][n_check_tiles] weighted sums of each of the first tiles
     unsigned long long* newton_fail = nullptr;
     DevRequest* dreq = nullptr;   // device copy of the request descriptor (plain path)
+    double* traj = nullptr;       // [n_out][dout][n_saved] single-simulation outputs
     size_t bytes = 0;
 };
 
+inline size_t traj_bytes(const HostRequest& hr) { return (size_t)hr.n_out * hr.dout * hr.n_saved * 8; }
+
 inline size_t request_bytes(const HostRequest& hr) {
     return (size_t)NPARAM * hr.N * 8 + (size_t)hr.dy * hr.N * 8 + hr.replay.size() * 8 +
-           (1 + (size_t)std::max<uint32_t>(1, hr.n_check_tiles)) * hr.n_out * hr.dout * 8 + 64 + sizeof(DevRequest);
+           (1 + (size_t)std::max<uint32_t>(1, hr.n_check_tiles)) * hr.n_out * hr.dout * 8 + 64 + sizeof(DevRequest) +
+           traj_bytes(hr);
 }
 
 inline size_t check_bytes(const HostRequest& hr) {
@@ -160,6 +176,7 @@ inline void free_dev(ReqDev& d) {
     if (d.check) hipFree(d.check);
     if (d.newton_fail) hipFree(d.newton_fail);
     if (d.dreq) hipFree(d.dreq);
+    if (d.traj) hipFree(d.traj);
     d = ReqDev{};
 }
 
@@ -177,6 +194,7 @@ inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
     if ((e = hipMalloc(&d.check, check_bytes(hr))) != hipSuccess) return fail("check", e);
     if ((e = hipMalloc(&d.newton_fail, 8)) != hipSuccess) return fail("counter", e);
     if ((e = hipMalloc(&d.dreq, sizeof(DevRequest))) != hipSuccess) return fail("descriptor", e);
+    if (traj_bytes(hr) && (e = hipMalloc(&d.traj, traj_bytes(hr))) != hipSuccess) return fail("trajectories", e);
     d.bytes = request_bytes(hr);
     SP_HIP(hipMemcpyAsync(d.params, hr.params.data(), hr.params.size() * 8, hipMemcpyHostToDevice, s));
     SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
@@ -211,12 +229,15 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     dr.state = d.state;
     dr.replay = d.replay;
     dr.newton_fail = d.newton_fail;
+    dr.traj = d.traj;
+    dr.n_saved = d.traj ? hr.n_saved : 0;
+    dr.save_interval = d.traj ? r.save_interval : 0;
     return dr;
 }
 
 // ------------------------------------------------------------------ results
 struct Outputs {
-    std::vector<double> time, field, mean, weight, final_state;
+    std::vector<double> time, field, mean, weight, final_state, traj;
     std::vector<double> check_tiles;   // [n_out*dout][n_check_tiles] raw weighted tile sums
     double device_ms = 0.0, latency_ms = 0.0, particle_steps = 0.0, check_error = -1.0, check_noise = 0.0;
     uint32_t used_fallback = 0, precision_used = SP_PREC_FP64;
@@ -234,6 +255,10 @@ inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t 
     }
     unsigned long long nf = 0;
     SP_HIP(hipMemcpyAsync(&nf, d.newton_fail, 8, hipMemcpyDeviceToHost, s));
+    if (d.traj) {
+        o.traj.resize(traj_bytes(hr) / 8);
+        SP_HIP(hipMemcpyAsync(o.traj.data(), d.traj, traj_bytes(hr), hipMemcpyDeviceToHost, s));
+    }
     if (keep_final) {
         o.final_state.resize(hr.N * hr.dy);
         std::vector<double> soa(hr.N * hr.dy);
@@ -268,6 +293,8 @@ inline void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
     if (r->mean) std::copy(o.mean.begin(), o.mean.end(), r->mean);
     if (r->weight) std::copy(o.weight.begin(), o.weight.end(), r->weight);
     if (r->final_state && !o.final_state.empty()) std::copy(o.final_state.begin(), o.final_state.end(), r->final_state);
+    r->n_saved = o.traj.empty() ? 0 : hr.n_saved;
+    if (r->trajectories && !o.traj.empty()) std::copy(o.traj.begin(), o.traj.end(), r->trajectories);
     r->device_ms = o.device_ms;
     r->latency_ms = o.latency_ms;
     r->particle_steps = o.particle_steps;

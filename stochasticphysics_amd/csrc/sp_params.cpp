@@ -3,7 +3,7 @@
 // Evaluates, once per particle, the reference's parameter calculators
 // (Problems/Helpers/ParameterCalculatorNeel.h:44-92, ParameterCalculatorBrown.h:39-58,
 // ParameterCalculatorBrownAndNeel.h:67-76, UniaxialAnisotropy.h:71-85,
-// CubicAnisotropy.h:92-94) into the param-major block the kernels load.
+// CubicAnisotropy.h:92-94, UniaxialCubicAnisotropy.hpp:92-96) into the param-major block the kernels load.
 #include "sp_host.h"
 
 #include <cmath>
@@ -62,12 +62,43 @@ void compute_params(const sp_request& rq, const sp_particles& pt, double* out, u
         out[P_C * stride + p] = c;
         out[P_CF * stride + p] = cF;
         out[P_DD * stride + p] = d;
+        double ax[9];
         for (int k = 0; k < 9; ++k) {
-            double v = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
-            if (pt.axes) v = pt.axes[p * 9 + k];
-            out[(P_AX + k) * stride + p] = v;
+            ax[k] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+            if (pt.axes) ax[k] = pt.axes[p * 9 + k];
+            out[(P_AX + k) * stride + p] = ax[k];
         }
+        double ku = 0.0, u[3] = {0.0, 0.0, 0.0};
+        if (rq.anisotropy == SP_ANISO_UNIAXIALCUBIC) {
+            // the cubic part takes P_K1 / P_PT (CubicAnisotropy.h:92-94): K is K_cubic here
+            const double Kc = pt.anisotropy_constant_cubic[p];
+            out[P_K1 * stride + p] = -2.0 * Kc / Ms;
+            out[P_PT * stride + p] = -2.0 * Kc * V;
+            ku = -2.0 * K / Ms;                              // UniaxialAnisotropy.h:71-77 with K_uniaxial
+            double R[3][3];
+            uniaxial_rotation_matrix(pt.uniaxial_rotation + 3 * p, R);
+            for (int i = 0; i < 3; ++i) u[i] = R[0][i] * ax[6] + R[1][i] * ax[7] + R[2][i] * ax[8];   // R^T z
+        }
+        out[P_KU * stride + p] = ku;
+        for (int i = 0; i < 3; ++i) out[(P_U + i) * stride + p] = u[i];
     }
+}
+
+// UniaxialCubicAnisotropy::calcRotation (UniaxialCubicAnisotropy.hpp:61-90): the matrix built from the
+// Euler angles (phi, theta, psi) of Uniaxial_Rotation
+void uniaxial_rotation_matrix(const double* e, double R[3][3]) {
+    const double cphi = std::cos(e[0]), ctheta = std::cos(e[1]), cpsi = std::cos(e[2]);
+    const double sphi = std::sin(e[0]), stheta = std::sin(e[1]), spsi = std::sin(e[2]);
+    const double cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
+    R[0][0] = cphicpsi - ctheta * sphispsi;
+    R[0][1] = -sphicpsi - ctheta * cphispsi;
+    R[0][2] = stheta * spsi;
+    R[1][0] = ctheta * sphicpsi + cphispsi;
+    R[1][1] = ctheta * cphicpsi - sphispsi;
+    R[1][2] = -stheta * cpsi;
+    R[2][0] = stheta * sphi;
+    R[2][1] = stheta * cphi;
+    R[2][2] = ctheta;
 }
 
 FieldParams field_params(const sp_request& rq) {

@@ -184,6 +184,8 @@ static void make_shadow(const HostRequest& parent, HostRequest& sh) {
     sh.rq = parent.rq;
     sh.rq.precision = SP_PREC_FP64;
     sh.rq.flags |= SP_FLAG_SKIP_CHECK;
+    sh.rq.save_interval = 0;                    // the shadow keeps no single-simulation outputs
+    sh.n_saved = 0;
     sh.dy = parent.dy; sh.dw = parent.dw; sh.dout = parent.dout;
     sh.N = std::min<uint64_t>(parent.N, 64ull * parent.n_check_tiles);
     sh.rq.n_particles = sh.N;

@@ -23,7 +23,9 @@ enum ParamIndex : int {
     P_CF = 8,     // Brown_F_Noise cF = c sqrt(2 kB T 6 eta V_h)     :45-56
     P_DD = 9,     // d = c mu                                         BrownAndNeelRelaxation.inl:48
     P_AX = 10,    // body axes x(3) y(3) z(3): params 10..18 (Neel, NeelSpherical)
-    NPARAM = 19
+    P_KU = 19,    // uniaxialcubic: field prefactor -2K_u/Ms of the uniaxial part (P_K1 holds -2K_c/Ms)
+    P_U = 20,     // uniaxialcubic: easy axis R^T z of the uniaxial part (3), UniaxialCubicAnisotropy.hpp:114
+    NPARAM = 23
 };
 
 // Uniform (per request) description of one ensemble on the device.
@@ -44,6 +46,10 @@ struct DevRequest {
     double* state;                 // [DY][N]
     const double* replay;          // [K][N] standard normals (replay mode)
     unsigned long long* newton_fail;
+    double* traj;                  // single-simulation outputs [window][DOUT][n_saved], or null
+    uint64_t n_saved;              // particles p with (p + 1) % save_interval == 0
+    uint32_t save_interval;
+    uint32_t pad;
 };
 
 enum : uint32_t { RQ_BN_SIMPLE = 1u, RQ_REPLAY = 2u, RQ_NORMALIZE = 4u };
@@ -72,10 +78,12 @@ struct SlotWork {
     uint32_t pad;
 };
 
-// kernel class id = problem * 64 + solver * 4 + aniso * 2 + mixed  (host side bookkeeping)
+// kernel class id = problem * 128 + solver * 8 + anisotropy slot * 2 + mixed (host side bookkeeping);
+// anisotropy slots: uniaxial 0, cubic 1, uniaxialcubic 2
 inline constexpr uint32_t kernel_kind(int problem, int solver, int aniso, int mixed) {
-    return (uint32_t)(problem * 64 + solver * 4 + (aniso - 1) * 2 + mixed);
+    return (uint32_t)(problem * 128 + solver * 8 + (aniso == 4 ? 2 : aniso - 1) * 2 + mixed);
 }
+inline constexpr bool kind_is_uniaxialcubic(uint32_t kind) { return ((kind >> 1) & 3u) == 2u; }
 
 // deterministic reduction of partial sums (one block per output row)
 struct ReduceWork {

@@ -56,6 +56,9 @@ class Ensemble:
     field_tau: float = 0.0                     # rectangular: Tau [s]
     field_alternating: bool = False            # rectangular: Alternating
     replay: Optional[np.ndarray] = None        # [K, n] standard normals
+    K_cubic: Optional[np.ndarray] = None       # uniaxialcubic: K of the cubic part (K is K_uniaxial)
+    uniaxial_rotation: Optional[np.ndarray] = None   # uniaxialcubic: [n, 3] Euler angles of the uniaxial part
+    save_interval: int = 0                     # Save_single_simulations: keep particles p with (p+1) % k == 0
 
     @property
     def n(self):
@@ -88,7 +91,8 @@ class Ensemble:
         """Ensemble restricted to particles idx (same seeds; particle indices are renumbered)."""
         import copy
         e = copy.copy(self)
-        for k in ("radius", "Ms", "alpha", "gamma", "T", "K", "r_h", "viscosity", "axes", "y0"):
+        for k in ("radius", "Ms", "alpha", "gamma", "T", "K", "r_h", "viscosity", "axes", "y0", "K_cubic",
+                  "uniaxial_rotation"):
             v = getattr(self, k)
             if v is not None:
                 setattr(e, k, np.ascontiguousarray(v[idx]))
@@ -97,12 +101,20 @@ class Ensemble:
         return e
 
     def oracle_props(self):
-        """[n, 8] property rows in the oracle's layout."""
+        """[n, 8] property rows in the oracle's layout ([n, 12] with K_cubic and the uniaxial rotation for
+        uniaxialcubic anisotropy)."""
         n = self.n
         z = np.zeros(n)
-        return np.stack([self.radius, self.Ms, self.alpha, self.gamma, self.T, self.K,
-                         self.r_h if self.r_h is not None else z,
-                         self.viscosity if self.viscosity is not None else z], axis=1)
+        cols = [self.radius, self.Ms, self.alpha, self.gamma, self.T, self.K,
+                self.r_h if self.r_h is not None else z, self.viscosity if self.viscosity is not None else z]
+        if self.anisotropy == "uniaxialcubic":
+            rot = np.broadcast_to(np.asarray(self.uniaxial_rotation, dtype=np.float64), (n, 3))
+            cols += [np.broadcast_to(np.asarray(self.K_cubic, dtype=np.float64), (n,)), rot[:, 0], rot[:, 1], rot[:, 2]]
+        return np.stack([np.broadcast_to(np.asarray(c, dtype=np.float64), (n,)) for c in cols], axis=1)
+
+    @property
+    def n_saved(self):
+        return self.n // self.save_interval if self.save_interval else 0
 
     def field_dict(self):
         return dict(type=FIELD[self.field], offset=self.field_offset, amp=self.field_amplitude,
@@ -135,6 +147,7 @@ class Ensemble:
             getattr(r, name)[:] = [float(x) for x in getattr(self, name)]
         r.field_period, r.field_time_offset, r.field_tau = self.field_period, self.field_time_offset, self.field_tau
         r.field_alternating = 1 if self.field_alternating else 0
+        r.save_interval = int(self.save_interval)
         if self.replay is not None:
             rep = np.ascontiguousarray(self.replay, dtype=np.float64)
             keep.append(rep)
@@ -147,6 +160,9 @@ class Ensemble:
         p.hydrodynamic_radius, p.viscosity = arr(self.r_h), arr(self.viscosity)
         p.axes = arr(self.axes, (n, 9)) if self.axes is not None else None
         p.initial_state = arr(self.y0, (n, self.dim_y))
+        if self.anisotropy == "uniaxialcubic":
+            p.anisotropy_constant_cubic = arr(self.K_cubic)
+            p.uniaxial_rotation = arr(self.uniaxial_rotation, (n, 3))
         return r, p, keep
 
 
@@ -158,6 +174,9 @@ def _alloc_result(e: Ensemble, keep_final):
     res = Result()
     res.time, res.field, res.mean, res.weight = dptr(o["time"]), dptr(o["field"]), dptr(o["mean"]), dptr(o["weight"])
     res.final_state = dptr(o["final_state"]) if keep_final else None
+    if e.n_saved:
+        o["trajectories"] = np.zeros((e.n_out, e.dim_out, e.n_saved))
+        res.trajectories = dptr(o["trajectories"])
     return res, o
 
 
@@ -165,7 +184,7 @@ def _finish_result(res: Result, o):
     o.update(device_ms=res.device_ms, latency_ms=res.latency_ms, particle_steps=res.particle_steps,
              check_error=res.check_error, check_noise=res.check_noise, used_fallback=res.used_fallback,
              precision_used=res.precision_used,
-             newton_failures=res.newton_failures)
+             newton_failures=res.newton_failures, n_saved=res.n_saved)
     return o
 
 

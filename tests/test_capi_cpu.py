@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lib().sp_abi_version() == 3
+    assert lib().sp_abi_version() == 4
 
 
 @pytest.mark.parametrize("kind,name,value", [("solver", "EulerMaruyama", 1), ("solver", "Implicit_Midpoint", 3),
@@ -139,6 +139,34 @@ def test_particle_params_match_oracle():
         assert P[4, p] == pytest.approx(mu, rel=1e-15)
         assert P[7, p] == b["BrownPrefactor"] and P[8, p] == b["Brown_F_Noise"]
         assert P[9, p] == pytest.approx(b["BrownPrefactor"] * mu, rel=1e-15)
+
+
+def test_uniaxialcubic_params():
+    """Uniaxialcubic rows: the cubic part's prefactor -2K_c/Ms in P_K1, -2K_u/Ms in P_KU and the easy axis
+    R^T z in P_U (UniaxialCubicAnisotropy.hpp:61-114), R checked against scipy's extrinsic z-x-z rotation."""
+    from scipy.spatial.transform import Rotation
+    e = configs.uniaxialcubic(n=32)
+    P = compute_params(e)
+    assert P.shape == (capi.NPARAM, e.n)
+    for p in range(e.n):
+        assert P[5, p] == -2.0 * e.K_cubic[p] / e.Ms[p] and P[19, p] == -2.0 * e.K[p] / e.Ms[p]
+        u = Rotation.from_euler("zxz", e.uniaxial_rotation[p]).as_matrix().T @ e.axes[p].reshape(3, 3)[2]
+        assert np.allclose(P[20:23, p], u, rtol=0, atol=1e-15)
+
+
+def test_brown_and_neel_uniaxialcubic_is_rejected():
+    """The reference's BrownAndNeel effective torque is 0/0 for uniaxialcubic (zero Euler sines passed by
+    BrownAndNeelRelaxation.inl:236-242 into CubicAnisotropy.h:183-192): the engine refuses the combination."""
+    e = configs.uniaxialcubic(n=8)
+    b = configs.config2(n=8, n_steps=100, oversampling=50)
+    b.anisotropy, b.K_cubic, b.uniaxial_rotation = "uniaxialcubic", e.K_cubic, e.uniaxial_rotation
+    r, p, keep = b.to_c()
+    out = np.zeros(capi.NPARAM * 8)
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), None) == 2
+    assert "uniaxialcubic" in lib().sp_last_error().decode()
+    e.K_cubic = None
+    r, p, keep = e.to_c()
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), None) == 1
 
 
 def test_request_validation_errors():

@@ -27,14 +27,15 @@ def ctx():
     c.close()
 
 
-def run_oracle(e: Ensemble, keep_final=True, threads=8):
+def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False):
     return oracle.simulate(e.pid, {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint": 3, "Millstein": 6,
                                    "Heun_Strong": 7}[e.solver],
-                           {"uniaxial": 1, "cubic": 2}[e.anisotropy], e.n_steps, e.oversampling, e.dt, e.seed,
+                           {"uniaxial": 1, "cubic": 2, "uniaxialcubic": 4}[e.anisotropy], e.n_steps, e.oversampling,
+                           e.dt, e.seed,
                            e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
                            coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
                            newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, replay=e.replay, threads=threads,
-                           keep_final=keep_final)
+                           keep_final=keep_final, per_particle=per_particle)
 
 
 # ---------------------------------------------------------------- noise
@@ -270,6 +271,33 @@ def test_implicit_midpoint_parity(ctx, problem):
     # the GPU must stall on exactly the same particle-steps as the oracle
     assert g["newton_failures"] == o["newton_failures"]
     compare(g, o, 1e-10, spherical=problem == "NeelSpherical")
+
+
+UC_CASES = {"neel_em": ("Neel", "EulerMaruyama", 130, 2000), "neel_heun": ("Neel", "Heun_Strong", 130, 2000),
+            "neel_millstein": ("Neel", "Millstein", 130, 2000), "neel_im": ("Neel", "Implicit_Midpoint", 64, 500),
+            "sph_em": ("NeelSpherical", "EulerMaruyama", 96, 2000), "sph_heun": ("NeelSpherical", "Heun_Strong", 96, 2000),
+            "sph_im": ("NeelSpherical", "Implicit_Midpoint", 64, 500)}
+
+
+@pytest.mark.parametrize("case", list(UC_CASES))
+def test_uniaxialcubic_parity(ctx, case):
+    """Uniaxialcubic anisotropy (UniaxialCubicAnisotropy.hpp:114): random cube axes, random rotations of the
+    uniaxial part, Lissajous drive, Philox noise at 300 K; an FMA-contracted oracle build differs from the plain one
+    by at most 6e-12 on these cases, and stalls Newton on the same particle-steps."""
+    problem, solver, n, steps = UC_CASES[case]
+    e = configs.uniaxialcubic(n=n, problem=problem, solver=solver, n_steps=steps, oversampling=10)
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    assert g["newton_failures"] == o["newton_failures"]
+    compare(g, o, 1e-10, spherical=problem == "NeelSpherical")
+
+
+def test_uniaxialcubic_mixed_precision(ctx):
+    e = configs.uniaxialcubic(n=4096, problem="NeelSpherical", solver="Heun_Strong", n_steps=4000, oversampling=10,
+                              precision="mixed")
+    g = ctx.run(e)
+    o = run_oracle(e, keep_final=False)
+    assert g["precision_used"] == 1
+    assert np.max(np.abs(g["mean"] - o["mean"])) <= 1e-3
 
 
 def stable_t0(solver, n=64, n_steps=5000):

@@ -250,3 +250,39 @@ def test_fields_closed_forms():
     assert np.allclose(h(sinc, half + half / 11), off, atol=1e-18)            # sinc(pi) = 0
     x = math.pi * 0.5
     assert np.allclose(h(sinc, half + half / 22), A * (math.sin(x) / x) ** 2 + off, rtol=1e-12)
+
+
+def test_uniaxialcubic_field_is_energy_gradient():
+    """Uniaxialcubic anisotropy has no reference test: pin H = (-2K_u/Ms)(m.u)u + cubic field (UniaxialCubicAnisotropy.hpp:114)
+    with u = R^T z, R the matrix of calcRotation (:61-90).  R is checked against a library rotation (scipy's
+    extrinsic z-x-z Euler rotation), H against -(1/Ms) grad of K_u (m.u)^2 + K_c sum_{i<j} c_i^2 c_j^2 by central
+    differences, and the two limits K_c = 0 (the uniaxial field about u) and K_u = 0 (the cubic field)."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(31)
+    Ms, Ku, Kc = 4.8e5, 7.0e3, -1.1e4
+    base = oracle.props_array(11e-9, Ms, 0.05, 1.76e11, 300.0, Ku)
+    for _ in range(8):
+        ax = Rotation.random(random_state=rng).as_matrix()   # rows: cube axes x, y, z
+        e = rng.uniform([0, 0, 0], [2 * math.pi, math.pi, 2 * math.pi])
+        R = Rotation.from_euler("zxz", e).as_matrix()
+        u = R.T @ ax[2]
+        props = np.concatenate([base, [Kc, *e]])
+        m = rng.standard_normal(3)
+        m /= np.linalg.norm(m)
+
+        def energy(v):
+            c = ax @ v
+            return Ku * (v @ u) ** 2 + Kc * (c[0] ** 2 * c[1] ** 2 + c[1] ** 2 * c[2] ** 2 + c[0] ** 2 * c[2] ** 2)
+        H, _ = oracle.aniso_field(4, props, m, ax)
+        h = 1e-6
+        grad = np.array([(energy(m + h * d) - energy(m - h * d)) / (2 * h) for d in np.eye(3)])
+        scale = (abs(Ku) + abs(Kc)) / Ms
+        assert np.allclose(H, -grad / Ms, rtol=1e-7, atol=1e-12 * scale)
+        # K_c = 0: the uniaxial field about u; K_u = 0: the cubic field about the particle's axes
+        ax_u = ax.copy()
+        ax_u[2] = u
+        Hu = oracle.aniso_field(4, np.concatenate([base, [0.0, *e]]), m, ax)[0]
+        assert np.allclose(Hu, oracle.aniso_field(1, base, m, ax_u)[0], rtol=1e-14, atol=1e-16 * scale)
+        Hc = oracle.aniso_field(4, np.concatenate([base[:5], [0.0], base[6:], [Kc, *e]]), m, ax)[0]
+        assert np.allclose(Hc, oracle.aniso_field(2, np.concatenate([base[:5], [Kc], base[6:]]), m, ax)[0],
+                           rtol=1e-14, atol=1e-16 * scale)

@@ -96,3 +96,48 @@ def test_rectangular_field_ini(tmp_path):
         assert (r.field_tau, r.field_time_offset, r.field_period, r.field_alternating) == (6e-11, 5e-11, 8e-10, 1)
     finally:
         lib().sp_settings_free(h)
+
+
+def test_uniaxialcubic_ini_with_single_simulations():
+    """A uniaxialcubic particle file (Anisotropy_Properties.{Uniaxial_Part, Cubic_Part}, Uniaxial_Rotation and their
+    distributions, UniaxialCubic.hpp:45-86) and Save_single_simulations / Save_intervall (ResultSettings.h:93-94)."""
+    h, r, p = load("uc_spherical.ini")
+    try:
+        assert (r.problem, r.solver, r.field, r.anisotropy) == (2, 1, 4, 4)
+        assert r.save_interval == 40
+        assert lib().sp_settings_single_savefile(h) == b"uc_spherical_single.json"
+        n = 200
+        assert np.all(arr(p.anisotropy_constant, n) == 8e3)
+        kc = arr(p.anisotropy_constant_cubic, n)
+        assert np.all(kc < 0) and np.std(kc) == pytest.approx(1.1e3, rel=0.3)        # normal, 10 % relative width
+        rot = arr(p.uniaxial_rotation, 3 * n).reshape(n, 3)
+        assert np.array_equal(rot, np.tile([0.3, 1.2, -0.5], (n, 1)))
+        axes = arr(p.axes, 9 * n).reshape(n, 3, 3)                                    # random orientations
+        assert np.allclose(np.einsum("nij,nkj->nik", axes, axes), np.eye(3), atol=1e-14)
+        assert np.std(axes[:, 2, 2]) > 0.3
+    finally:
+        lib().sp_settings_free(h)
+
+
+def test_single_results_writer(tmp_path):
+    """sp_settings_write_single_results: one entry per kept particle, named after the particle."""
+    from stochasticphysics_amd.capi import Result
+    h, r, p = load("uc_spherical.ini")
+    try:
+        n_out, dout, S = 40, 3, 5
+        t, f, m, w = np.arange(n_out, dtype=float), np.zeros((n_out, 3)), np.zeros((n_out, dout)), np.ones(dout)
+        traj = np.arange(n_out * dout * S, dtype=float).reshape(n_out, dout, S)
+        res = Result()
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        res.time, res.field, res.mean, res.weight, res.trajectories = dp(t), dp(f), dp(m), dp(w), dp(traj)
+        res.n_out, res.dim_out, res.n_saved = n_out, dout, S
+        out = tmp_path / "single.json"
+        assert lib().sp_settings_write_single_results(h, C.byref(res), str(out).encode()) == 0, lib().sp_last_error()
+        doc = json.loads(out.read_text())
+        assert sorted(doc, key=lambda k: int(k.split("_")[1])) == [f"Single_{40 * (j + 1)}" for j in range(S)]
+        for j in range(S):
+            e = doc[f"Single_{40 * (j + 1)}"]
+            assert np.array_equal(np.array(e["Results"]), traj[:, :, j]) and len(e["Time"]) == n_out
+            assert e["UsedProperties"]["Magnetic_radius"] == 1.1e-8
+    finally:
+        lib().sp_settings_free(h)
