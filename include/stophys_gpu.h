@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 4
+#define SP_ABI_VERSION 5
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -44,7 +44,8 @@ typedef enum sp_status {
 
 /* ---- selector enums: numeric values equal the reference's enums ---- */
 /* Settings::IProblem (Settings/ProblemSettings.h:55-70) */
-enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_NEEL = 3 };
+enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_NEEL = 3,
+       SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL = 4 };
 /* Settings::ISolver (Settings/SolverSettings.h:34-47) */
 enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_EULER_MARUYAMA_NORMALIZED = 2, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
        SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7 };
@@ -69,9 +70,12 @@ enum {
     SP_FLAG_KEEP_FINAL_STATE = 1u << 2,  /* copy per-particle final state back */
     SP_FLAG_NO_FALLBACK = 1u << 3,       /* mixed precision: report the check error but never re-run in FP64 */
     SP_FLAG_SKIP_CHECK = 1u << 4,        /* mixed precision: no FP64 shadow check */
-    SP_FLAG_RENORMALIZE = 1u << 5        /* apply the problem's normalize() (NeelRelaxation.h:204-208,
+    SP_FLAG_RENORMALIZE = 1u << 5,       /* apply the problem's normalize() (NeelRelaxation.h:204-208,
                                             BrownAndNeelRelaxation.h:168-172) after every step of any
                                             solver, in FP64; EulerMaruyama_Normalized always does */
+    SP_FLAG_EULER_TRANSFORM = 1u << 6    /* BrownAndNeelEulerSpherical Use_euler_coordinate_transformation
+                                            (ProblemSettings.h:146-159); its Use_spherical_coordinate_transformation
+                                            is SP_FLAG_COORD_TRANSFORM with min_angle */
 };
 
 /*
@@ -94,7 +98,8 @@ typedef struct sp_request {
     uint64_t seed;            /* Philox key */
     double dt;                /* Timestep [s] */
     double newton_tol;        /* Implicit_Midpoint Accuracy_Goal (SolverSettings.h:122) */
-    double min_angle;         /* NeelSpherical Min_angle_transformation [rad] */
+    double min_angle;         /* NeelSpherical Min_angle_transformation [rad]; BrownAndNeelEulerSpherical
+                                 Neel_Min_angle_before_transformation */
     double check_tol;         /* mixed precision: tolerated bias of <m>_mixed - <m>_fp64 (0: 1e-3; a negative
                                  value fails every check, which forces the FP64 re-run) */
     /* field parameters (Properties/Fields/*.hpp): Offset_Field, Amplitude for every field but zero;
@@ -116,6 +121,7 @@ typedef struct sp_request {
                                  Results/SimulationResultManager.h:115-149): 0 = off; k > 0 keeps the window
                                  means of every particle p with (p + 1) % k == 0 (the reference saves every k-th
                                  finished simulation, in completion order) */
+    double brown_min_angle;   /* BrownAndNeelEulerSpherical Brown_Min_angle_before_transformation [rad] */
 } sp_request;
 
 /*
@@ -134,7 +140,8 @@ typedef struct sp_particles {
     const double* hydrodynamic_radius;       /* [n] m (BrownAndNeel only, else NULL) */
     const double* viscosity;                 /* [n] Pa s (BrownAndNeel only, else NULL) */
     const double* axes;                      /* [n][9] body axes x,y,z (Neel/NeelSpherical; NULL = identity) */
-    const double* initial_state;             /* [n][dim_y]: m (Neel), (theta,phi) (NeelSpherical), (n,m) (BrownAndNeel) */
+    const double* initial_state;             /* [n][dim_y]: m (Neel), (theta,phi) (NeelSpherical), (n,m) (BrownAndNeel),
+                                                (phi, theta, psi, theta_N, phi_N) (BrownAndNeelEulerSpherical) */
     /* uniaxialcubic only (Properties/Anisotropy/UniaxialCubic.hpp:20-51), else NULL */
     const double* anisotropy_constant_cubic; /* [n] K_cubic of the cubic part, J/m^3 */
     const double* uniaxial_rotation;         /* [n][3] Euler angles of the uniaxial part (Uniaxial_Rotation): its easy

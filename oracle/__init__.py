@@ -15,9 +15,9 @@ import numpy as np
 _DIR = Path(__file__).resolve().parent
 _LIBS = {}
 
-DIM_Y = {1: 3, 2: 2, 3: 6}
-DIM_W = {1: 3, 2: 3, 3: 6}
-DIM_OUT = {1: 3, 2: 3, 3: 6}
+DIM_Y = {1: 3, 2: 2, 3: 6, 4: 5}
+DIM_W = {1: 3, 2: 3, 3: 6, 4: 6}
+DIM_OUT = {1: 3, 2: 3, 3: 6, 4: 9}
 
 
 def build(force=False):
@@ -135,17 +135,24 @@ def euler_axes(phi, theta, psi):
     return out.reshape(3, 3)
 
 
-WHAT = dict(det=0, drift=1, sto=2, jacdet_ref=3, jacsto_ref=4, finish=5, rotated=6)
+WHAT = dict(det=0, drift=1, sto=2, jacdet_ref=3, jacsto_ref=4, finish=5, rotated=6, output=7)
 
 
-def evaluate(problem, aniso, props, what, y, h=None, axes=None, simple=False, rotate=False, min_angle=0.0, dt=0.0):
+def evaluate(problem, aniso, props, what, y, h=None, axes=None, simple=False, rotate=False, min_angle=0.0, dt=0.0,
+             euler_rotate=False, brown_min_angle=0.0):
+    """One problem term at state y.  BrownAndNeelEulerSpherical (problem 4): rotate / min_angle are the Neel chart
+    settings, euler_rotate / brown_min_angle the Brown ones; 'finish' returns the prepared state then the state after
+    finishCalculations (10 values), 'rotated' the (Brown, Neel) flags."""
     dy = DIM_Y[problem]
-    size = {0: dy, 1: dy, 2: dy * DIM_W[problem], 3: 9, 4: 9, 5: 2, 6: 1}[WHAT[what]]
+    bnes = problem == 4
+    size = {0: dy, 1: dy, 2: dy * DIM_W[problem], 3: 9, 4: 9, 5: 10 if bnes else 2, 6: 2 if bnes else 1, 7: 9}[WHAT[what]]
     out = np.zeros(size)
     y = np.asarray(y, dtype=np.float64)
     hh = None if h is None else np.asarray(h, dtype=np.float64)
     ax = None if axes is None else np.ascontiguousarray(axes, dtype=np.float64).ravel()
-    flags = (1 if simple else 0) | (2 if rotate else 0)
+    flags = (1 if simple else 0) | (2 if rotate else 0) | (8 if euler_rotate else 0)
+    if bnes:
+        dt = brown_min_angle
     rc = lib().or_eval(problem, aniso, _p(np.asarray(props, float)), _p(ax), flags, min_angle, _p(y), _p(hh), dt,
                        WHAT[what], _p(out))
     if rc:
@@ -162,8 +169,11 @@ def newton_testfn(fn, x0, tol, max_iter):
 
 def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0, axes=None, field=None,
              bn_simple=False, coord_transform=False, min_angle=0.0, newton_tol=1e-10, newton_max_iter=20, renormalize=False,
-             replay=None, threads=1, keep_final=False, per_particle=False, native=False):
-    """Run the oracle ensemble.  props: [N,8] (radius, Ms, alpha, gamma, T, K, r_h, viscosity);
+             replay=None, threads=1, keep_final=False, per_particle=False, native=False, euler_transform=False,
+             brown_min_angle=0.0):
+    """Run the oracle ensemble.  props: [N,8] (radius, Ms, alpha, gamma, T, K, r_h, viscosity; [N,12] with K_cubic and
+    the uniaxial rotation for uniaxialcubic); BrownAndNeelEulerSpherical: coord_transform / min_angle for its Neel part,
+    euler_transform / brown_min_angle for its Brown part;
     y0: [N, dim_y]; axes: [N, 9] or None; field: dict(type, offset, amp, freq, phase) or None;
     replay: [K, N] standard normals or None."""
     props = np.ascontiguousarray(props, dtype=np.float64)
@@ -172,13 +182,14 @@ def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0,
     dy, dout = DIM_Y[problem], DIM_OUT[problem]
     n_out = n_steps // oversampling
     f = field or dict(type=1)
-    dspec = np.zeros(20)
+    dspec = np.zeros(21)
     dspec[0], dspec[1], dspec[2], dspec[3] = dt, min_angle, newton_tol, f.get("type", 1)
+    dspec[20] = brown_min_angle
     for j, key in enumerate(("offset", "amp", "freq", "phase")):
         dspec[4 + 3 * j:7 + 3 * j] = np.asarray(f.get(key, [0.0, 0.0, 0.0]), dtype=np.float64)
     dspec[16], dspec[17], dspec[18] = f.get("period", 0.0), f.get("time_offset", 0.0), f.get("tau", 0.0)
     dspec[19] = 1.0 if f.get("alternating", False) else 0.0
-    flags = (1 if bn_simple else 0) | (2 if coord_transform else 0) | (4 if renormalize else 0)
+    flags = (1 if bn_simple else 0) | (2 if coord_transform else 0) | (4 if renormalize else 0) | (8 if euler_transform else 0)
     ispec = np.array([problem, solver, aniso, flags, newton_max_iter, threads], dtype=np.int64)
     rep = None if replay is None else np.ascontiguousarray(replay, dtype=np.float64)
     uspec = np.array([N, n_steps, oversampling, seed, 0 if rep is None else rep.shape[0]], dtype=np.uint64)

@@ -93,7 +93,9 @@ void or_euler_axes(double phi, double theta, double psi, double* out) {
 // Evaluate one problem term at state y with field h (golden-vector checks).
 // what: 0 det, 1 drift, 2 sto (row-major), 3 Neel reference JacDet (dt), 4 Neel reference
 // JacSto (h = dW), 5 NeelSpherical prepare+finish round trip, 6 NeelSpherical isRotated
-// flags: bit0 BN simple model, bit1 coordinate transformation
+// flags: bit0 BN simple model, bit1 coordinate transformation (NeelSpherical; Neel part of
+// BrownAndNeelEulerSpherical), bit3 Euler coordinate transformation (its Brown part, minimum angle passed as dt);
+// what 7: output vector
 int or_eval(int problem, int aniso, const double* props, const double* axes, int flags, double minAngle,
             const double* y, const double* h, double dt, int what, double* out) {
     const MagneticProperties mp = magFrom(props, aniso);
@@ -126,6 +128,20 @@ int or_eval(int problem, int aniso, const double* props, const double* axes, int
         if (what == 6) { out[0] = c.isRotated ? 1.0 : 0.0; return 0; }
         return 1;
     };
+    auto bnes = [&](auto an) -> int {   // flags bit1 spherical, bit3 Euler transformation; dt carries the Brown angle
+        using A = decltype(an);
+        BrownAndNeelEulerSphericalProblem<A> P(mp, T, props[6], props[7], (flags & 8) != 0, dt, (flags & 2) != 0, minAngle);
+        Vec<double, 5> yv; for (int i = 0; i < 5; ++i) yv[i] = y[i];
+        BNESCache c;
+        P.prepare(yv, c);
+        if (what == 0) { auto r = P.det(c, hv); for (int i = 0; i < 5; ++i) out[i] = r[i]; return 0; }
+        if (what == 1) { auto r = P.drift(yv, c); for (int i = 0; i < 5; ++i) out[i] = r[i]; return 0; }
+        if (what == 2) { auto M = P.sto(c); for (int i = 0; i < 5; ++i) for (int j = 0; j < 6; ++j) out[i * 6 + j] = M(i, j); return 0; }
+        if (what == 5) { for (int i = 0; i < 5; ++i) out[i] = yv[i]; P.finish(yv, c); for (int i = 0; i < 5; ++i) out[5 + i] = yv[i]; return 0; }
+        if (what == 6) { out[0] = c.brownRot ? 1.0 : 0.0; out[1] = c.neelRot ? 1.0 : 0.0; return 0; }
+        if (what == 7) { auto o = P.output(yv); for (int i = 0; i < 9; ++i) out[i] = o[i]; return 0; }
+        return 1;
+    };
     auto bn = [&](auto an, auto simple) -> int {
         using A = decltype(an);
         BrownAndNeelProblem<A, decltype(simple)::value> P(mp, T, props[6], props[7]);
@@ -137,6 +153,7 @@ int or_eval(int problem, int aniso, const double* props, const double* axes, int
     };
     if (problem == PROB_NEEL) return aniso == 1 ? neel(Uniaxial(mp)) : (aniso == 4 ? neel(UniaxialCubic(mp)) : neel(Cubic(mp)));
     if (problem == PROB_SPHERICAL) return aniso == 1 ? sph(Uniaxial(mp)) : (aniso == 4 ? sph(UniaxialCubic(mp)) : sph(Cubic(mp)));
+    if (problem == PROB_BNES) return aniso == 1 ? bnes(Uniaxial(mp)) : (aniso == 4 ? bnes(UniaxialCubic(mp)) : bnes(Cubic(mp)));
     if (problem == PROB_BN) {
         const bool simple = flags & 1;
         if (aniso == 1) return simple ? bn(Uniaxial(mp), std::true_type{}) : bn(Uniaxial(mp), std::false_type{});
@@ -176,7 +193,9 @@ int or_newton_testfn(int fn, const double* x0, double tol, int maxit, double* xo
 
 // Full ensemble run.  ispec = [problem, solver, aniso, flags, newton_max_iter, nthreads];
 // uspec = [n_particles, n_steps, oversampling, seed, replay_count];
-// dspec = [dt, min_angle, newton_tol, field_type, offset(3), amp(3), freq(3), phase(3)]
+// dspec = [dt, min_angle, newton_tol, field_type, offset(3), amp(3), freq(3), phase(3), Periode, TimeOffset, Tau,
+// Alternating, brown_min_angle]; ispec[3] flags: 1 BN simple, 2 spherical chart rotation, 4 renormalize, 8 Euler chart
+// rotation (BrownAndNeelEulerSpherical)
 // props [N][8], axes [N][9] (or NULL), y0 [N][dim_y].
 // Outputs: mean [n_out*dout], weight [dout], times [n_out], fields [n_out*3],
 // final_state [N*dim_y] (or NULL), per_particle [N*n_out*dout] weighted outputs (or NULL).
@@ -188,6 +207,7 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
     RunSpec rs{};
     rs.problem = (int)ispec[0]; rs.solver = (int)ispec[1]; rs.aniso = (int)ispec[2];
     rs.bnSimple = (ispec[3] & 1) != 0; rs.coordTransform = (ispec[3] & 2) != 0; rs.renormalize = (ispec[3] & 4) != 0;
+    rs.eulerTransform = (ispec[3] & 8) != 0;
     rs.newtonMaxIter = (int)ispec[4];
     const int nThreads = (int)ispec[5];
     if (rs.solver != SOLVER_EM && rs.solver != SOLVER_EM_NORMALIZED && rs.solver != SOLVER_IM && rs.solver != SOLVER_MILSTEIN &&
@@ -195,11 +215,12 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
         return 3;
     rs.nParticles = uspec[0]; rs.nSteps = uspec[1]; rs.oversampling = uspec[2]; rs.seed = uspec[3];
     rs.replayCount = uspec[4]; rs.replay = replay;
-    rs.dt = dspec[0]; rs.minAngle = dspec[1]; rs.newtonTol = dspec[2];
+    rs.dt = dspec[0]; rs.minAngle = dspec[1]; rs.newtonTol = dspec[2]; rs.brownMinAngle = dspec[20];
     rs.field.type = (int)dspec[3];
     rs.field = fieldFrom(dspec);
     const uint64_t N = rs.nParticles;
-    const int dy = rs.problem == PROB_NEEL ? 3 : (rs.problem == PROB_SPHERICAL ? 2 : 6);
+    const int dy = rs.problem == PROB_NEEL ? 3 : (rs.problem == PROB_SPHERICAL ? 2 : (rs.problem == PROB_BNES ? 5 : 6));
+    if (rs.problem == PROB_BNES && rs.solver == SOLVER_IM) return 3;
     std::vector<ParticleIn> parts(N);
     const int ps = propsStride(rs.aniso);
     for (uint64_t p = 0; p < N; ++p) {
@@ -219,6 +240,10 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
         if (rs.aniso == 1) NEEL_RUN(Uniaxial); else if (rs.aniso == 4) NEEL_RUN(UniaxialCubic); else NEEL_RUN(Cubic);
     } else if (rs.problem == PROB_SPHERICAL) {
         if (rs.aniso == 1) SPH_RUN(Uniaxial); else if (rs.aniso == 4) SPH_RUN(UniaxialCubic); else SPH_RUN(Cubic);
+    } else if (rs.problem == PROB_BNES) {
+#define BNES_RUN(A) runEnsemble<BrownAndNeelEulerSphericalProblem<A>, BNESStepper<A>>(rs, parts.data(), [&](const ParticleIn& q) { return BrownAndNeelEulerSphericalProblem<A>(q.mp, q.T, q.r_h, q.visc, rs.eulerTransform, rs.brownMinAngle, rs.coordTransform, rs.minAngle); }, rr, nThreads, keep, ppo)
+        if (rs.aniso == 1) BNES_RUN(Uniaxial); else if (rs.aniso == 4) BNES_RUN(UniaxialCubic); else BNES_RUN(Cubic);
+#undef BNES_RUN
     } else if (rs.problem == PROB_BN) {
         if (rs.aniso == 4) return 2;   // the reference's torque is NaN for this combination
 #define BN_RUN(A, S) runEnsemble<BrownAndNeelProblem<A, S>, CartesianStepper<BrownAndNeelProblem<A, S>>>(rs, parts.data(), [&](const ParticleIn& q) { return BrownAndNeelProblem<A, S>(q.mp, q.T, q.r_h, q.visc); }, rr, nThreads, keep, ppo)

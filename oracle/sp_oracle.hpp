@@ -170,6 +170,10 @@ struct Uniaxial {
     template <class S> Vec<S, 3> torque_axis(const Vec<S, 3>& m, const Vec<S, 3>& n) const {  // :146-150
         return cross(n, (prefactorTorque * dot(m, n)) * m);
     }
+    // getEffTorque with body axes (:108-118): the easy axis is the body z axis
+    Vec<double, 3> torqueBody(const Vec<double, 3>& m, const Axes& ax, const double*, const double*) const {
+        return torque_axis<double>(m, ax.z);
+    }
     Mat<double, 3, 3> jacobian(const Axes& ax) const {                                       // :130-137
         Mat<double, 3, 3> J;
         for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) J(i, j) = (prefactorField * ax.z[i]) * ax.z[j];
@@ -187,6 +191,21 @@ struct Cubic {
         const S c1m_2 = c1m * c1m, c2m_2 = c2m * c2m, c3m_2 = c3m * c3m;
         const Vec<S, 3> t = ((c2m_2 + c3m_2) * c1m) * xa + ((c1m_2 + c3m_2) * c2m) * ya + ((c1m_2 + c2m_2) * c3m) * za;
         return K1_MS * t;                                                                     // :139
+    }
+    // getEffTorque (:161-193) from the prepareField cache (:109-121) and the sines / cosines of the body's Euler
+    // angles (phi, theta, psi); division by sin(theta) as written (0/0 when theta = 0)
+    Vec<double, 3> torqueBody(const Vec<double, 3>& m, const Axes& ax, const double* sn, const double* cs) const {
+        const double c1m = dot(m, ax.x), c2m = dot(m, ax.y), c3m = dot(m, ax.z);
+        const double c1m_2 = c1m * c1m, c2m_2 = c2m * c2m;
+        const double ctheta = cs[1], cpsi = cs[2], stheta = sn[1], spsi = sn[2];
+        const Vec<double, 3> theta = v3(ctheta * spsi, -ctheta * cpsi, -stheta);
+        Vec<double, 3> symetric;
+        for (int i = 0; i < 3; ++i) symetric[i] = (m[i] * c3m) * (c1m_2 + c2m_2);
+        const Vec<double, 3> symetricresult = cross(ax.z, symetric);
+        Vec<double, 3> asymetricdir;
+        for (int i = 0; i < 3; ++i) asymetricdir[i] = ((theta[i] * ctheta) * 1.0) / stheta + ax.z[i];
+        const double asymetricterm = c1m * c2m * (c2m_2 - c1m_2);
+        return K1VM * (symetricresult + asymetricterm * asymetricdir);
     }
     // Jacobian of the field (the reference returns Identity here, :228-236 "TODO");
     // analytic form d/dm of the expression above, used for documentation tests.
@@ -212,8 +231,10 @@ struct UniaxialCubic {
     Cubic cubic;                 // -2 K_cubic / Ms
     Mat<double, 3, 3> R;
     static MagneticProperties cubicPart(const MagneticProperties& mp) { MagneticProperties c = mp; c.K = mp.K2; return c; }
+    double uniTorque;            // -2 K_uniaxial V (UniaxialAnisotropy.h:79-85)
     explicit UniaxialCubic(const MagneticProperties& mp)
-        : prefactorField(-2.0 * mp.K / mp.Ms), cubic(cubicPart(mp)), R(rotation(mp.urot)) {}
+        : prefactorField(-2.0 * mp.K / mp.Ms), cubic(cubicPart(mp)), R(rotation(mp.urot)),
+          uniTorque(-2.0 * mp.K * mp.volume()) {}
     static Mat<double, 3, 3> rotation(const double* e) {                     // :61-90
         const Vec<double, 3> ang = v3(e[0], e[1], e[2]);
         const double cphi = std::cos(ang[0]), ctheta = std::cos(ang[1]), cpsi = std::cos(ang[2]);
@@ -233,6 +254,12 @@ struct UniaxialCubic {
     template <class S> Vec<S, 3> field(const Vec<S, 3>& m, const Axes& ax) const {
         const Vec<S, 3> u = lift3<S>(easyAxis(ax));
         return (prefactorField * dot(m, u)) * u + cubic.field(m, ax);
+    }
+    // getEffTorque (:120-127): the uniaxial part's torque takes R m (not the field's R^T z), plus the cubic torque
+    Vec<double, 3> torqueBody(const Vec<double, 3>& m, const Axes& ax, const double* sn, const double* cs) const {
+        const Vec<double, 3> Rm = matvec(R, m);
+        const Vec<double, 3> tu = cross(ax.z, (uniTorque * dot(Rm, ax.z)) * Rm);
+        return tu + cubic.torqueBody(m, ax, sn, cs);
     }
 };
 
@@ -454,6 +481,230 @@ template <class Aniso, bool Simple> struct BrownAndNeelProblem {
     }
     Vec<double, 6> output(const Vec<double, 6>& y) const { return y; }
     Vec<double, 6> weight() const { Vec<double, 6> w; for (int i = 0; i < 3; ++i) { w[i] = 1.0; w[i + 3] = MagneticMoment; } return w; }  // .h:152-159
+};
+
+// ------------------------------------------------------------------ BrownAndNeelEulerSpherical problem
+// Problems/BrownAndNeelRelaxationEulerSpherical.h:40-927.  y = (phi, theta, psi, theta_N, phi_N): the Euler angles
+// (R313) of the particle body and the spherical angles of the magnetisation; dim_W = 6 (torque noise, field noise);
+// output (x axis, y axis, m), all weighted by mu (:674-760).  Stratonovich.
+struct BNESCache {
+    bool brownRot = false, neelRot = false;
+    double sn[5], cs[5];                 // sines / cosines of the prepared state (StateSines, StateCosines)
+    Mat<double, 3, 3> R;                 // EulerRotationMatrix: columns are the body axes in world coordinates
+    Mat<double, 3, 3> E;                 // EulerProjectionMatrix: omega -> Euler angle rates
+    double csctheta = 0.0;
+    Vec<double, 3> M;                    // MagnetisationDir
+    double ods = 0.0;                    // one_div_sin_t
+    Mat<double, 2, 3> PS;                // SphericalProjectionMatrix
+};
+
+template <class Aniso> struct BrownAndNeelEulerSphericalProblem {
+    static constexpr int DY = 5, DW = 6, DOUT = 9;
+    NeelParams mNeel; BrownParams mBrown; Aniso mAnisotropy; double MagneticMoment;
+    bool brownRotate, neelRotate; double brownMin, neelMin;
+    BrownAndNeelEulerSphericalProblem(const MagneticProperties& mp, double T, double r_h, double visc, bool useEuler,
+                                      double brownMinAngle, bool useSpherical, double neelMinAngle)    // :124-133
+        : mNeel(calcNeelParams(mp, T)), mBrown(calcBrownRotationParams(r_h, visc, T)), mAnisotropy(mp),
+          MagneticMoment(mp.moment()), brownRotate(useEuler), neelRotate(useSpherical),
+          brownMin(brownMinAngle), neelMin(neelMinAngle) {}
+
+    // needsBrownCoordRotation (:796-814), needsNeelCoordRotation (:770-788): |theta| against the window
+    bool needsBrown(const Vec<double, 5>& y) const {
+        const double t = std::abs(y[1]);
+        return brownRotate && (t < brownMin || t > PI - brownMin);
+    }
+    bool needsNeel(const Vec<double, 5>& y) const {
+        const double t = std::abs(y[3]);
+        return neelRotate && (t < neelMin || t > PI - neelMin);
+    }
+    // EulertoEulerRotated (:859-877), EulerRotatedtoEuler (:880-898)
+    static void eulerToRotated(Vec<double, 5>& y) {
+        const double S0 = std::sin(y[0]), S1 = std::sin(y[1]), S2 = std::sin(y[2]);
+        const double C0 = std::cos(y[0]), C1 = std::cos(y[1]), C2 = std::cos(y[2]);
+        const double phi = std::atan2(-C2 * C0 + C1 * S0 * S2, S0 * C2 + C1 * C0 * S2);
+        const double theta = std::acos(-S1 * S2);
+        const double psi = std::atan2(C1, C2 * S1);
+        y[0] = phi; y[1] = theta; y[2] = psi;
+    }
+    static void rotatedToEuler(Vec<double, 5>& y) {
+        const double S0 = std::sin(y[0]), S1 = std::sin(y[1]), S2 = std::sin(y[2]);
+        const double C0 = std::cos(y[0]), C1 = std::cos(y[1]), C2 = std::cos(y[2]);
+        const double phi = std::atan2(C0 * C2 - C1 * S0 * S2, -C2 * S0 - C1 * C0 * S2);
+        const double theta = std::acos(S1 * S2);
+        const double psi = std::atan2(-C1, C2 * S1);
+        y[0] = phi; y[1] = theta; y[2] = psi;
+    }
+    // Rotate2DSphericalCoordinate90DegreeAroundYAxis (:823-834) and its inverse (:844-855) on (theta_N, phi_N)
+    static void neelToRotated(Vec<double, 5>& y) {
+        const double sin_t = std::sin(y[3]);
+        const double t = std::acos(-std::cos(y[4]) * sin_t), p = std::atan2(std::sin(y[4]) * sin_t, std::cos(y[3]));
+        y[3] = t; y[4] = p;
+    }
+    static void rotatedToNeel(Vec<double, 5>& y) {
+        const double sin_t = std::sin(y[3]);
+        const double t = std::acos(std::cos(y[4]) * sin_t), p = std::atan2(std::sin(y[4]) * sin_t, -std::cos(y[3]));
+        y[3] = t; y[4] = p;
+    }
+    // the part of prepareCalculations after the rotation decision (:170-361): caches of state y in the given charts
+    void chartAt(const Vec<double, 5>& y, bool brownRot, bool neelRot, BNESCache& c) const {
+        c.brownRot = brownRot; c.neelRot = neelRot;
+        for (int i = 0; i < 5; ++i) { c.sn[i] = std::sin(y[i]); c.cs[i] = std::cos(y[i]); }
+        const double cphi = c.cs[0], ctheta = c.cs[1], cpsi = c.cs[2], sphi = c.sn[0], stheta = c.sn[1], spsi = c.sn[2];
+        const double cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
+        const double cthetacpsi = ctheta * cpsi, cthetaspsi = ctheta * spsi;
+        c.csctheta = 1.0 / stheta;
+        if (std::isinf(c.csctheta)) c.csctheta = 0.0;
+        const double csc = c.csctheta;
+        if (!brownRot) {                                                             // :183-238
+            c.R(0, 0) = cphicpsi - ctheta * sphispsi; c.R(0, 1) = -sphicpsi - ctheta * cphispsi; c.R(0, 2) = stheta * spsi;
+            c.R(1, 0) = ctheta * sphicpsi + cphispsi; c.R(1, 1) = ctheta * cphicpsi - sphispsi; c.R(1, 2) = -stheta * cpsi;
+            c.R(2, 0) = stheta * sphi;                c.R(2, 1) = stheta * cphi;                c.R(2, 2) = ctheta;
+            c.E(0, 0) = spsi * csc;        c.E(0, 1) = -cpsi * csc;       c.E(0, 2) = 0.0;
+            c.E(1, 0) = cpsi;              c.E(1, 1) = spsi;              c.E(1, 2) = 0.0;
+            c.E(2, 0) = -cthetaspsi * csc; c.E(2, 1) = cthetacpsi * csc;  c.E(2, 2) = 1.0;
+        } else {                                                                     // :239-303
+            c.R(0, 0) = -stheta * sphi;               c.R(0, 1) = -stheta * cphi;               c.R(0, 2) = -ctheta;
+            c.R(1, 0) = ctheta * sphicpsi + cphispsi; c.R(1, 1) = ctheta * cphicpsi - sphispsi; c.R(1, 2) = -stheta * cpsi;
+            c.R(2, 0) = cphicpsi - ctheta * sphispsi; c.R(2, 1) = -sphicpsi - ctheta * cphispsi; c.R(2, 2) = stheta * spsi;
+            c.E(0, 0) = 0.0;   c.E(0, 1) = -cpsi * csc;       c.E(0, 2) = spsi * csc;
+            c.E(1, 0) = 0.0;   c.E(1, 1) = spsi;              c.E(1, 2) = cpsi;
+            c.E(2, 0) = -1.0;  c.E(2, 1) = cthetacpsi * csc;  c.E(2, 2) = -cthetaspsi * csc;
+        }
+        const double cos_t = c.cs[3], cos_p = c.cs[4], sin_t = c.sn[3], sin_p = c.sn[4];
+        c.ods = 1.0 / sin_t;
+        if (std::isinf(c.ods)) c.ods = 0.0;
+        if (!neelRot) {                                                              // :307-330
+            c.M = v3(sin_t * cos_p, sin_t * sin_p, cos_t);
+            c.PS(0, 0) = -sin_p; c.PS(1, 0) = -cos_p * cos_t * c.ods;
+            c.PS(0, 1) = cos_p;  c.PS(1, 1) = -sin_p * cos_t * c.ods;
+            c.PS(0, 2) = 0.0;    c.PS(1, 2) = 1.0;
+        } else {                                                                     // :331-360
+            c.M = v3(-cos_t, sin_t * sin_p, sin_t * cos_p);
+            c.PS(0, 0) = 0.0;    c.PS(1, 0) = -1.0;
+            c.PS(0, 1) = cos_p;  c.PS(1, 1) = -sin_p * cos_t * c.ods;
+            c.PS(0, 2) = -sin_p; c.PS(1, 2) = -cos_p * cos_t * c.ods;
+        }
+    }
+    // prepareCalculations (:142-361)
+    void prepare(Vec<double, 5>& y, BNESCache& c) const {
+        const bool br = needsBrown(y);
+        if (br) eulerToRotated(y);
+        const bool nr = needsNeel(y);
+        if (nr) neelToRotated(y);
+        chartAt(y, br, nr, c);
+    }
+    Axes axes(const BNESCache& c) const {
+        Axes ax;
+        ax.x = v3(c.R(0, 0), c.R(1, 0), c.R(2, 0)); ax.y = v3(c.R(0, 1), c.R(1, 1), c.R(2, 1)); ax.z = v3(c.R(0, 2), c.R(1, 2), c.R(2, 2));
+        return ax;
+    }
+    // getDeterministicVector (:364-425)
+    Vec<double, 5> det(const BNESCache& c, const Vec<double, 3>& xi) const {
+        const Axes ax = axes(c);
+        const Vec<double, 3> Heff = mAnisotropy.field(c.M, ax) + xi;
+        const Vec<double, 3> Teff = mAnisotropy.torqueBody(c.M, ax, c.sn, c.cs);
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, cc = mBrown.BrownPrefactor, d = cc * MagneticMoment;
+        const Vec<double, 3> mxHeff = cross(c.M, Heff);
+        const Vec<double, 3> omegabrown = cc * Teff + d * mxHeff;
+        const Vec<double, 3> omeganeel = (-a) * Heff + (b + d) * mxHeff + cc * Teff;
+        const Vec<double, 3> br = matvec(c.E, omegabrown);
+        const Vec<double, 2> ne = matvec(c.PS, omeganeel);
+        Vec<double, 5> r; r[0] = br[0]; r[1] = br[1]; r[2] = br[2]; r[3] = ne[0]; r[4] = ne[1];
+        return r;
+    }
+    // getStochasticMatrix (:428-497)
+    Mat<double, 5, 6> sto(const BNESCache& c) const {
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, cc = mBrown.BrownPrefactor, d = cc * MagneticMoment;
+        const double F = mBrown.Brown_F_Noise, sig = mNeel.NoisePrefactor;
+        auto skew = [](const Vec<double, 3>& t) {     // the matrices built at :449-457 and :473-481: Mat v = t x v
+            Mat<double, 3, 3> K = Mat<double, 3, 3>::zero();
+            K(1, 0) = t[2]; K(2, 0) = -t[1]; K(0, 1) = -t[2]; K(2, 1) = t[0]; K(0, 2) = t[1]; K(1, 2) = -t[0];
+            return K;
+        };
+        const Mat<double, 3, 3> BF = matmul(c.E, skew((d * sig) * c.M));
+        Mat<double, 3, 3> NF = skew(((b + d) * sig) * c.M);
+        for (int i = 0; i < 3; ++i) NF(i, i) = NF(i, i) - a * sig;
+        const Mat<double, 2, 3> NH = matmul(c.PS, NF);
+        Mat<double, 5, 6> S = Mat<double, 5, 6>::zero();
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) { S(i, j) = F * c.E(i, j); S(i, 3 + j) = BF(i, j); }
+        for (int i = 0; i < 2; ++i) for (int j = 0; j < 3; ++j) { S(3 + i, j) = c.PS(i, j) * F; S(3 + i, 3 + j) = NH(i, j); }
+        return S;
+    }
+    // getDrift (:500-601); y is the prepared state
+    Vec<double, 5> drift(const Vec<double, 5>& y, const BNESCache& c) const {
+        const double a = mNeel.NeelFactor1, b = mNeel.NeelFactor2, cc = mBrown.BrownPrefactor, d = cc * MagneticMoment;
+        const double DN = mNeel.NoisePrefactor, DB = mBrown.BrownDiffusion;
+        const double ctheta = c.cs[1], cpsi = c.cs[2], spsi = c.sn[2];
+        const double cos_t = c.cs[3], cos_p = c.cs[4], sin_t = c.sn[3], sin_p = c.sn[4];
+        const double csctheta = c.csctheta;
+        const double DN_2 = DN * DN, DB_2 = DB * DB, c_2_half = 0.5 * cc * cc, d_2_fourth = 0.25 * d * d, a_d = a * d;
+        const double bpd = b + d;
+        Vec<double, 5> r;
+        if ((!c.brownRot && !c.neelRot) || (c.brownRot && c.neelRot)) {             // :542-558
+            const double cotb = csctheta * ctheta;
+            const double cos_2t = cos_t * cos_t - sin_t * sin_t, sin_2t = 2.0 * cos_t * sin_t, sin_t_2 = sin_t * sin_t;
+            const double cos_pm = cos_p * cpsi + spsi * sin_p, sin_pm = cpsi * sin_p - cos_p * spsi;
+            const double two = 2.0 * (y[4] - y[2]);
+            const double cos_2pm = std::cos(two), sin_2pm = std::sin(two);
+            r[0] = -DN_2 * csctheta * sin_t * (-a * d * sin_pm + 2.0 * d_2_fourth * (cos_t * cos_pm + cotb * sin_t * sin_2pm));
+            r[1] = c_2_half * DB_2 * cotb - DN_2 * a_d * (cos_pm * sin_t) +
+                   d_2_fourth * DN_2 * (0.5 * (3.0 + cos_2t + 2.0 * cos_2pm * sin_t_2) * cotb - sin_2t * sin_pm);
+            r[2] = -DN_2 * (a_d * (cos_t + sin_t * cotb * sin_pm) -
+                            d_2_fourth * sin_t * (2.0 * cos_t * cotb * cos_pm + (-1.0 + 2.0 * csctheta * csctheta) * sin_t * sin_2pm));
+        } else {
+            const double sin_2t = 2.0 * cos_t * sin_t, sin_t_2 = sin_t * sin_t;
+            const double cpsi_2 = cpsi * cpsi, spsi_2 = spsi * spsi, c2psi = cpsi_2 - spsi_2, s2psi = 2.0 * cpsi * spsi;
+            const double sin_2p = 2.0 * cos_p * sin_p, cotb = csctheta * ctheta;
+            const double cos_t_2 = cos_t * cos_t, sin_p_2 = sin_p * sin_p, cos_p_2 = cos_p * cos_p;
+            if (c.brownRot) {                                                        // :559-576
+                r[0] = DN_2 * csctheta * (a_d * (-spsi * cos_t + sin_p * cpsi * sin_t) +
+                                          2.0 * d_2_fourth * (-c2psi * cotb * sin_2t * sin_p + cos_p * sin_t * (cos_t * cpsi + sin_t * sin_p * spsi) +
+                                                              cotb * s2psi * (cos_t_2 - sin_t_2 * sin_p_2)));
+                r[1] = c_2_half * DB_2 * cotb + DN_2 * (a_d * (-cos_t * cpsi - spsi * sin_t * sin_p) +
+                                                        d_2_fourth * (2.0 * cpsi_2 * cotb * (cos_t_2 + cos_p_2 * sin_t_2) + cpsi * sin_t_2 * sin_2p +
+                                                                      2.0 * spsi * (-sin_t * cos_t * cos_p + cotb * sin_t_2 * spsi) +
+                                                                      cotb * sin_2t * sin_p * s2psi));
+                r[2] = DN_2 * (a_d * (sin_t * cos_p - cpsi * cotb * sin_t * sin_p + cotb * cos_t * spsi) -
+                               d_2_fourth * (cos_t * cpsi + sin_t * sin_p * spsi) *
+                                   (2.0 * cos_p * cotb * sin_t - (3.0 + c2psi) * csctheta * csctheta * (cpsi * sin_t * sin_p - cos_t * spsi)));
+            } else {                                                                 // :577-594
+                r[0] = DN_2 * csctheta * (a_d * (spsi * cos_t + sin_p * cpsi * sin_t) +
+                                          2.0 * d_2_fourth * (c2psi * cotb * sin_2t * sin_p + cos_p * sin_t * (cos_t * cpsi - sin_t * sin_p * spsi) +
+                                                              cotb * s2psi * (cos_t_2 - sin_t_2 * sin_p_2)));
+                r[1] = c_2_half * DB_2 * cotb + DN_2 * (a_d * (+cos_t * cpsi - spsi * sin_t * sin_p) +
+                                                        d_2_fourth * (2.0 * cpsi_2 * cotb * (cos_t_2 + cos_p_2 * sin_t_2) - cpsi * sin_t_2 * sin_2p +
+                                                                      2.0 * spsi * (-sin_t * cos_t * cos_p + cotb * sin_t_2 * spsi) -
+                                                                      cotb * sin_2t * sin_p * s2psi));
+                r[2] = DN_2 * (a_d * (-sin_t * cos_p - cpsi * cotb * sin_t * sin_p - cotb * cos_t * spsi) -
+                               d_2_fourth * (cos_t * cpsi - sin_t * sin_p * spsi) *
+                                   (2.0 * cos_p * cotb * sin_t + (3.0 + c2psi) * csctheta * csctheta * (cpsi * sin_t * sin_p + cos_t * spsi)));
+            }
+        }
+        r[3] = (c_2_half * DB_2 + 0.5 * (a * a + bpd * bpd) * DN_2) * c.ods * cos_t;   // :596-598
+        r[4] = 0.0;
+        return r;
+    }
+    // finishCalculations (:635-665)
+    void finish(Vec<double, 5>& y, const BNESCache& c) const {
+        if (c.brownRot) rotatedToEuler(y);
+        else { y[0] = std::fmod(y[0], TWO_PI); y[1] = std::fmod(y[1], TWO_PI); y[2] = std::fmod(y[2], TWO_PI); }
+        if (c.neelRot) rotatedToNeel(y);
+        else { y[3] = std::fmod(y[3], TWO_PI); y[4] = std::fmod(y[4], TWO_PI); }
+    }
+    void normalize(Vec<double, 5>&) const {}                                             // :738-739
+    // calculateOutputResult (:674-735): body x axis, body y axis, magnetisation direction
+    Vec<double, 9> output(const Vec<double, 5>& y) const {
+        double sn[5], cs[5];
+        for (int i = 0; i < 5; ++i) { sn[i] = std::sin(y[i]); cs[i] = std::cos(y[i]); }
+        const double cphi = cs[0], ctheta = cs[1], cpsi = cs[2], sphi = sn[0], stheta = sn[1], spsi = sn[2];
+        const double cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
+        Vec<double, 9> o;
+        o[0] = cphicpsi - ctheta * sphispsi; o[1] = ctheta * sphicpsi + cphispsi; o[2] = stheta * sphi;
+        o[3] = -sphicpsi - ctheta * cphispsi; o[4] = ctheta * cphicpsi - sphispsi; o[5] = stheta * cphi;
+        o[6] = sn[3] * cs[4]; o[7] = sn[3] * sn[4]; o[8] = cs[3];
+        return o;
+    }
+    Vec<double, 9> weight() const { Vec<double, 9> w; for (int i = 0; i < 9; ++i) w[i] = MagneticMoment; return w; }   // :755-759
 };
 
 // ------------------------------------------------------------------ fields

@@ -22,11 +22,13 @@
 namespace oracle {
 
 enum { SOLVER_EM = 1, SOLVER_EM_NORMALIZED = 2, SOLVER_IM = 3, SOLVER_MILSTEIN = 6, SOLVER_HEUN = 7 };
-enum { PROB_NEEL = 1, PROB_SPHERICAL = 2, PROB_BN = 3 };
+enum { PROB_NEEL = 1, PROB_SPHERICAL = 2, PROB_BN = 3, PROB_BNES = 4 };
 
 struct RunSpec {
     int problem, solver, aniso;
     bool bnSimple, coordTransform;
+    bool eulerTransform = false;   // BrownAndNeelEulerSpherical: Use_euler_coordinate_transformation
+    double brownMinAngle = 0.0;    // BrownAndNeelEulerSpherical: Brown_Min_angle_before_transformation
     bool renormalize;          // problem.normalize() after every step, any solver (engine option)
     double dt, minAngle, newtonTol;
     int newtonMaxIter;
@@ -236,6 +238,59 @@ template <class Aniso> struct SphericalStepper {
         Y out; out[0] = y[0]; out[1] = y[1];
         return out;
     }
+};
+
+// ---------------------------------------------------------------- BrownAndNeelEulerSpherical step functions
+// The same chart convention as SphericalStepper: the charts (Brown and Neel rotation) chosen from y_i hold for
+// every stage of a step, and the result is mapped back at the end of the step.  Implicit_Midpoint has no
+// counterpart: the reference's Jacobians of this problem are not implemented (assert(false),
+// BrownAndNeelRelaxationEulerSpherical.h:606-671).
+template <class Aniso> struct BNESStepper {
+    using Prob = BrownAndNeelEulerSphericalProblem<Aniso>;
+    using Y = Vec<double, 5>;
+    Prob& P; const RunSpec& rs; uint64_t newtonFail = 0;
+    BNESStepper(Prob& p, const RunSpec& r) : P(p), rs(r) {}
+    Y em(const Y& yi, double time, NoiseSource& ns) {                                  // EulerMaruyama.inl:41-72
+        Y res = yi; BNESCache c;
+        P.prepare(res, c);
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, 6> dW = noiseVec<Prob, 6>(ns, std::sqrt(dt));
+        const Y a_ = P.det(c, xi), bb = P.drift(res, c);
+        const Y bdW = matvec(P.sto(c), dW);
+        for (int i = 0; i < 5; ++i) res[i] = res[i] + ((a_[i] + bb[i]) * dt + bdW[i]);
+        P.finish(res, c);
+        return res;
+    }
+    Y heun(const Y& yi, double time, NoiseSource& ns) {                                // Heun_Strong.inl:36-47
+        Y res = yi; BNESCache c0, c1;
+        P.prepare(res, c0);
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, 6> dW = noiseVec<Prob, 6>(ns, std::sqrt(dt));
+        const Y ayi = P.det(c0, xi) + P.drift(res, c0);
+        const Y bdW = matvec(P.sto(c0), dW);
+        Y support; for (int i = 0; i < 5; ++i) support[i] = res[i] + ayi[i] * dt + bdW[i];
+        P.chartAt(support, c0.brownRot, c0.neelRot, c1);
+        const Y a2 = P.det(c1, xi) + P.drift(support, c1);
+        Y out; for (int i = 0; i < 5; ++i) out[i] = res[i] + 0.5 * (a2[i] + ayi[i]) * dt + bdW[i];
+        P.finish(out, c0);
+        return out;
+    }
+    Y milstein(const Y& yi, double time, NoiseSource& ns) {                            // Millstein.inl:34-44
+        Y res = yi; BNESCache c;
+        P.prepare(res, c);
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, 6> dW = noiseVec<Prob, 6>(ns, std::sqrt(dt));
+        const Y a_ = P.det(c, xi), bb = P.drift(res, c);
+        const Y bdW = matvec(P.sto(c), dW);
+        const double dWdW = dot(dW, dW);
+        for (int i = 0; i < 5; ++i) res[i] = res[i] + a_[i] * dt + bdW[i] + bb[i] * dWdW;
+        P.finish(res, c);
+        return res;
+    }
+    Y implicitMidpoint(const Y& yi, double, NoiseSource&) { ++newtonFail; return yi; }   // not offered (see above)
 };
 
 // ---------------------------------------------------------------- simulator

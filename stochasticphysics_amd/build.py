@@ -28,7 +28,8 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
              "-Wno-unused-command-line-argument"]
 
 SOURCES = [
-    "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_uc.hip", "sp_kernels_common.hip",
+    "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_uc.hip", "sp_kernels_bnes.hip",
+    "sp_kernels_common.hip",
     "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
     "frontend/ini.cpp", "frontend/settings.cpp",
 ]

@@ -11,7 +11,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "libstophys_gpu.so"
 
 # enum values (identical to the reference's selector enums)
-PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3)
+PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3, BrownAndNeelEulerSpherical=4)
 SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
 FIELD = dict(none=1, constant=2, sinusoidal=3, lissajous=4, triangular=5, rectangular=6, sinc=7)
 ANISO = dict(uniaxial=1, cubic=2, uniaxialcubic=4)
@@ -19,9 +19,10 @@ PREC = dict(fp64=0, mixed=1)
 NOISE = dict(philox=0, replay=1)
 FLAG_BN_SIMPLE, FLAG_COORD_TRANSFORM, FLAG_KEEP_FINAL, FLAG_NO_FALLBACK, FLAG_SKIP_CHECK = 1, 2, 4, 8, 16
 FLAG_RENORMALIZE = 32
-DIM_Y = {1: 3, 2: 2, 3: 6}
-DIM_W = {1: 3, 2: 3, 3: 6}
-DIM_OUT = {1: 3, 2: 3, 3: 6}
+FLAG_EULER_TRANSFORM = 64
+DIM_Y = {1: 3, 2: 2, 3: 6, 4: 5}
+DIM_W = {1: 3, 2: 3, 3: 6, 4: 6}
+DIM_OUT = {1: 3, 2: 3, 3: 6, 4: 9}
 NPARAM = 23
 
 SP_OK, SP_ERR_NOT_CONVERGED, SP_ERR_TIMEOUT, SP_ERR_NOT_READY = 0, 5, 7, 9
@@ -42,7 +43,7 @@ class Request(C.Structure):
                 ("field_freq", C.c_double * 3), ("field_phase", C.c_double * 3),
                 ("replay_normals", C.POINTER(C.c_double)), ("replay_count", C.c_uint64),
                 ("field_period", C.c_double), ("field_time_offset", C.c_double), ("field_tau", C.c_double),
-                ("field_alternating", C.c_uint32), ("save_interval", C.c_uint32)]
+                ("field_alternating", C.c_uint32), ("save_interval", C.c_uint32), ("brown_min_angle", C.c_double)]
 
 
 class Particles(C.Structure):

@@ -191,8 +191,12 @@ Settings* load(const std::string& path) {
     if (!pt) throw std::runtime_error("missing [Problem_Settings] Type_of_problem");
     r.problem = parse_enum_or_throw("problem", *pt);
     if (flag(f, PS, {"Use_simple_model"}, false)) r.flags |= SP_FLAG_BN_SIMPLE_MODEL;
-    if (flag(f, PS, {"Use_coordinate_transformation"}, false)) r.flags |= SP_FLAG_COORD_TRANSFORM;
-    r.min_angle = num(f, PS, {"Min_angle_transformation"}, 0.0);
+    if (flag(f, PS, {"Use_coordinate_transformation", "Use_spherical_coordinate_transformation"}, false))
+        r.flags |= SP_FLAG_COORD_TRANSFORM;
+    r.min_angle = num(f, PS, {"Min_angle_transformation", "Neel_Min_angle_before_transformation"}, 0.0);
+    // BrownAndNeelEulerSpherical (ProblemSettings.h:146-159): the Brown part's Euler chart
+    if (flag(f, PS, {"Use_euler_coordinate_transformation"}, false)) r.flags |= SP_FLAG_EULER_TRANSFORM;
+    r.brown_min_angle = num(f, PS, {"Brown_Min_angle_before_transformation"}, 0.0);
     // [Field_Parameters] (FieldProperties.h:61, Properties/Fields/*.hpp)
     std::string FS = f.has_section("Field_Parameters") ? "Field_Parameters" : "Field_Properties";
     const std::string* ft = opt(f, FS, {"Type_of_field"});
@@ -281,9 +285,12 @@ Settings* load(const std::string& path) {
             const double scale[3] = {6.283185307179586, 3.141592653589793, 6.283185307179586};
             for (int k = 0; k < 3; ++k) s->urot[i * 3 + k] = pf.rand_urot ? rng.uniform() * scale[k] : pf.urot[k];
         }
-        // axes from Euler angles (NeelRelaxation.h:223-261); random: each angle U(0, 2 pi)
+        // axes from Euler angles (NeelRelaxation.h:223-261); random: each angle U(0, 2 pi), or for
+        // BrownAndNeelEulerSpherical phi, psi U(0, 2 pi) and theta U(0, pi) (ParticleStateInitializer.h:42-62)
+        const bool euler_state = r.problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL;
         double e[3] = {pf.euler[0], pf.euler[1], pf.euler[2]};
-        if (pf.rand_orient) for (double& x : e) x = rng.uniform() * 6.283185307179586;
+        if (pf.rand_orient)
+            for (int k = 0; k < 3; ++k) e[k] = rng.uniform() * (euler_state && k == 1 ? 3.141592653589793 : 6.283185307179586);
         const double cphi = std::cos(e[0]), cth = std::cos(e[1]), cpsi = std::cos(e[2]);
         const double sphi = std::sin(e[0]), sth = std::sin(e[1]), spsi = std::sin(e[2]);
         const double ax[9] = {cphi * cpsi - cth * sphi * spsi, -sphi * cpsi - cth * cphi * spsi, sth * spsi,
@@ -291,10 +298,19 @@ Settings* load(const std::string& path) {
                               sth * sphi, sth * cphi, cth};
         for (int k = 0; k < 9; ++k) s->axes[i * 9 + k] = ax[k];
         double m[3] = {pf.mag[0], pf.mag[1], pf.mag[2]};
-        if (pf.rand_mag) for (double& x : m) x = rng.normal();
+        if (pf.rand_mag && euler_state) {   // ParticleStateInitializer.h:107-130: phi U(0, 2 pi), theta U(0, pi)
+            const double ph = rng.uniform() * 6.283185307179586, th = rng.uniform() * 3.141592653589793;
+            m[0] = std::cos(ph) * std::sin(th); m[1] = std::sin(ph) * std::sin(th); m[2] = std::cos(th);
+        } else if (pf.rand_mag) {
+            for (double& x : m) x = rng.normal();
+        }
         const double mn = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
         for (double& x : m) x /= mn;
-        if (r.problem == SP_PROBLEM_NEEL) { for (int k = 0; k < 3; ++k) s->y0[i * 3 + k] = m[k]; }
+        if (euler_state) {                  // getStart (BrownAndNeelRelaxationEulerSpherical.h:741-753)
+            for (int k = 0; k < 3; ++k) s->y0[i * 5 + k] = e[k];
+            s->y0[i * 5 + 3] = std::acos(m[2]);
+            s->y0[i * 5 + 4] = std::atan2(m[1], m[0]);
+        } else if (r.problem == SP_PROBLEM_NEEL) { for (int k = 0; k < 3; ++k) s->y0[i * 3 + k] = m[k]; }
         else if (r.problem == SP_PROBLEM_NEEL_SPHERICAL) {   // ParticleStateInitializer.h:132-144
             s->y0[i * 2 + 0] = std::acos(m[2]);
             s->y0[i * 2 + 1] = std::atan2(m[1], m[0]);

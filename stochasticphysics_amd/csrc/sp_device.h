@@ -361,6 +361,9 @@ __device__ __forceinline__ void sincos_fast(double x, double* s_out, double* c_o
     *c_out = cc;
 }
 
+// std::fmod(x, 2 pi): x itself when |x| < 2 pi (exact), the library routine otherwise
+__device__ __forceinline__ double fmod_2pi(double x) { return fabs(x) < TWO_PI ? x : fmod(x, TWO_PI); }
+
 template <class R> struct Trig { R st, ct, sp, cp; };
 __device__ __forceinline__ void trig_of(double th, double ph, Trig<double>& T) {
     sincos_fast(th, &T.st, &T.ct);
@@ -447,8 +450,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
             y[0] = acos(cp * st);
             y[1] = atan2(sp * st, -ct);
         } else {
-            y[0] = fmod(r[0], TWO_PI);
-            y[1] = fmod(r[1], TWO_PI);
+            y[0] = fmod_2pi(r[0]);
+            y[1] = fmod_2pi(r[1]);
         }
         trig_of(y[0], y[1], T);
     }

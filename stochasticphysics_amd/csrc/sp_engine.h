@@ -43,7 +43,7 @@ struct HostRequest {
     std::vector<double> params;   // [NPARAM][N]
     std::vector<double> y0;       // [dy][N]
     std::vector<double> replay;   // [K][N]
-    double wsum[6] = {0}, wsum_check[6] = {0};
+    double wsum[9] = {0}, wsum_check[9] = {0};
     uint32_t n_check_tiles = 0;
     uint64_t n_saved = 0;         // particles with single-simulation outputs (save_interval)
 };
@@ -72,6 +72,13 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
         if (rq->problem == SP_PROBLEM_BROWN_AND_NEEL)
             return set_error(SP_ERR_UNSUPPORTED, "BrownAndNeel with uniaxialcubic anisotropy: the reference's effective "
                                                  "torque is 0/0 (NaN) for this combination");
+    }
+    if (rq->problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL && rq->solver == SP_SOLVER_IMPLICIT_MIDPOINT) {
+        // the reference's Jacobians of this problem are not implemented (assert(false),
+        // BrownAndNeelRelaxationEulerSpherical.h:606-671)
+        return set_error(SP_ERR_UNSUPPORTED, "Implicit_Midpoint is not available for BrownAndNeelEulerSpherical");
+    }
+    if (rq->anisotropy == SP_ANISO_UNIAXIALCUBIC) {
         if (!pt || !pt->anisotropy_constant_cubic || !pt->uniaxial_rotation)
             return set_error(SP_ERR_INVALID_ARG, "uniaxialcubic needs anisotropy_constant_cubic and uniaxial_rotation");
     }
@@ -84,8 +91,9 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (!pt->magnetic_radius || !pt->saturation_magnetisation || !pt->damping || !pt->gyromagnetic_ratio ||
         !pt->temperature || !pt->anisotropy_constant || !pt->initial_state)
         return set_error(SP_ERR_INVALID_ARG, "missing particle property array");
-    if (rq->problem == SP_PROBLEM_BROWN_AND_NEEL && (!pt->hydrodynamic_radius || !pt->viscosity))
-        return set_error(SP_ERR_INVALID_ARG, "BrownAndNeel needs hydrodynamic_radius and viscosity");
+    if ((rq->problem == SP_PROBLEM_BROWN_AND_NEEL || rq->problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL) &&
+        (!pt->hydrodynamic_radius || !pt->viscosity))
+        return set_error(SP_ERR_INVALID_ARG, "BrownAndNeel problems need hydrodynamic_radius and viscosity");
     if (rq->solver == SP_SOLVER_IMPLICIT_MIDPOINT && (rq->newton_max_iter == 0 || !(rq->newton_tol > 0.0)))
         return set_error(SP_ERR_INVALID_ARG, "Implicit_Midpoint needs Max_Iterations > 0 and Accuracy_Goal > 0");
     const uint64_t total = (rq->n_steps / rq->oversampling - 1) * rq->oversampling;
@@ -135,7 +143,7 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     for (uint64_t p = 0; p < hr.N; ++p) {
         const double mu = hr.params[P_MU * hr.N + p];
         for (uint32_t c = 0; c < hr.dout; ++c) {
-            const double w = (hr.dout == 6 && c < 3) ? 1.0 : mu;   // BrownAndNeelRelaxation.h:152-159
+            const double w = (hr.dout == 6 && c < 3) ? 1.0 : mu;   // BrownAndNeelRelaxation.h:152-159; all mu otherwise
             hr.wsum[c] += w;
             if (p < n_check) hr.wsum_check[c] += w;
         }
@@ -213,6 +221,9 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     const bool rot = (r.flags & SP_FLAG_COORD_TRANSFORM) != 0;
     dr.tmin = rot ? r.min_angle : -1e300;   // no chart rotation: theta may leave [0, pi] freely
     dr.tmax = rot ? (3.14159265358979323846 - r.min_angle) : 1e300;
+    const bool erot = (r.flags & SP_FLAG_EULER_TRANSFORM) != 0;     // BrownAndNeelEulerSpherical Brown chart
+    dr.bmin = erot ? r.brown_min_angle : -1e300;
+    dr.bmax = erot ? (3.14159265358979323846 - r.brown_min_angle) : 1e300;
     dr.newton_tol = r.newton_tol;
     dr.field = field_params(r);
     dr.n_particles = hr.N;

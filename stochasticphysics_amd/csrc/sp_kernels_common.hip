@@ -5,30 +5,33 @@ namespace sp {
 
 bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                   uint64_t se, double* partial, uint32_t pstride) {
-    if (kind_is_uniaxialcubic(kind)) return launch_plain_uc(kind, grid, block, s, rq, sb, se, partial, pstride);
+    if (kind_is_uniaxialcubic(kind) && kind / 128 <= 2) return launch_plain_uc(kind, grid, block, s, rq, sb, se, partial, pstride);
     switch (kind / 128) {
         case 1: return launch_plain_neel(kind, grid, block, s, rq, sb, se, partial, pstride);
         case 2: return launch_plain_sph(kind, grid, block, s, rq, sb, se, partial, pstride);
         case 3: return launch_plain_bn(kind, grid, block, s, rq, sb, se, partial, pstride);
+        case 4: return launch_plain_bnes(kind, grid, block, s, rq, sb, se, partial, pstride);
     }
     return false;
 }
 bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs, const SlotWork* slots,
                   uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
-    if (kind_is_uniaxialcubic(kind)) return launch_batch_uc(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+    if (kind_is_uniaxialcubic(kind) && kind / 128 <= 2) return launch_batch_uc(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
     switch (kind / 128) {
         case 1: return launch_batch_neel(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
         case 2: return launch_batch_sph(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
         case 3: return launch_batch_bn(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+        case 4: return launch_batch_bnes(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
     }
     return false;
 }
 bool kernel_info(uint32_t kind, KernelInfo* info) {
-    if (kind_is_uniaxialcubic(kind)) return info_uc(kind, info);
+    if (kind_is_uniaxialcubic(kind) && kind / 128 <= 2) return info_uc(kind, info);
     switch (kind / 128) {
         case 1: return info_neel(kind, info);
         case 2: return info_sph(kind, info);
         case 3: return info_bn(kind, info);
+        case 4: return info_bnes(kind, info);
     }
     return false;
 }

@@ -22,13 +22,16 @@ int request_dims(int problem, uint32_t* dy, uint32_t* dw, uint32_t* dout) {
         case SP_PROBLEM_NEEL: *dy = 3; *dw = 3; *dout = 3; return 0;
         case SP_PROBLEM_NEEL_SPHERICAL: *dy = 2; *dw = 3; *dout = 3; return 0;
         case SP_PROBLEM_BROWN_AND_NEEL: *dy = 6; *dw = 6; *dout = 6; return 0;
+        // (phi, theta, psi, theta_N, phi_N), 6 noise components, output (x axis, y axis, m)
+        case SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL: *dy = 5; *dw = 6; *dout = 9; return 0;
     }
     return -1;
 }
 
 void compute_params(const sp_request& rq, const sp_particles& pt, double* out, uint64_t stride) {
     const uint64_t N = rq.n_particles;
-    const bool bn = rq.problem == SP_PROBLEM_BROWN_AND_NEEL;
+    const bool euler = rq.problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL;
+    const bool bn = rq.problem == SP_PROBLEM_BROWN_AND_NEEL || euler;
     for (uint64_t p = 0; p < N; ++p) {
         const double r = pt.magnetic_radius[p], Ms = pt.saturation_magnetisation[p];
         const double alpha = pt.damping[p], gamma = pt.gyromagnetic_ratio[p], T = pt.temperature[p];
@@ -77,7 +80,12 @@ void compute_params(const sp_request& rq, const sp_particles& pt, double* out, u
             ku = -2.0 * K / Ms;                              // UniaxialAnisotropy.h:71-77 with K_uniaxial
             double R[3][3];
             uniaxial_rotation_matrix(pt.uniaxial_rotation + 3 * p, R);
-            for (int i = 0; i < 3; ++i) u[i] = R[0][i] * ax[6] + R[1][i] * ax[7] + R[2][i] * ax[8];   // R^T z
+            if (euler) {   // the body axes move with the state: keep R itself (axis rows) and -2K_u V (torque)
+                for (int k = 0; k < 9; ++k) out[(P_AX + k) * stride + p] = R[k / 3][k % 3];
+                u[0] = -2.0 * K * V;
+            } else {
+                for (int i = 0; i < 3; ++i) u[i] = R[0][i] * ax[6] + R[1][i] * ax[7] + R[2][i] * ax[8];   // R^T z
+            }
         }
         out[P_KU * stride + p] = ku;
         for (int i = 0; i < 3; ++i) out[(P_U + i) * stride + p] = u[i];

@@ -30,6 +30,7 @@ namespace sp {
 // relative cost of one particle-step per kernel class (for longest-first ordering)
 static double class_cost(const HostRequest& hr) {
     double c = hr.rq.problem == SP_PROBLEM_BROWN_AND_NEEL ? 3.0 : (hr.rq.problem == SP_PROBLEM_NEEL_SPHERICAL ? 1.6 : 1.0);
+    if (hr.rq.problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL) c = 5.0;
     if (hr.rq.solver == SP_SOLVER_HEUN_STRONG) c *= 1.5;
     if (hr.rq.solver == SP_SOLVER_IMPLICIT_MIDPOINT) c *= 6.0;
     if (hr.mixed) c *= 0.6;

@@ -24,7 +24,8 @@ enum ParamIndex : int {
     P_DD = 9,     // d = c mu                                         BrownAndNeelRelaxation.inl:48
     P_AX = 10,    // body axes x(3) y(3) z(3): params 10..18 (Neel, NeelSpherical)
     P_KU = 19,    // uniaxialcubic: field prefactor -2K_u/Ms of the uniaxial part (P_K1 holds -2K_c/Ms)
-    P_U = 20,     // uniaxialcubic: easy axis R^T z of the uniaxial part (3), UniaxialCubicAnisotropy.hpp:114
+    P_U = 20,     // uniaxialcubic: easy axis R^T z of the uniaxial part (3), UniaxialCubicAnisotropy.hpp:114;
+                  // BrownAndNeelEulerSpherical: -2K_u V, and R itself in the axis rows (the body axes are state)
     NPARAM = 23
 };
 
@@ -32,6 +33,7 @@ enum ParamIndex : int {
 struct DevRequest {
     double dt, sqdt;
     double tmin, tmax;             // NeelSpherical chart rotation window (rotation when theta outside)
+    double bmin, bmax;             // BrownAndNeelEulerSpherical: Euler chart window of the body angle theta
     double newton_tol;
     FieldParams field;             // applied field (sp_field.h)
     uint64_t n_particles;

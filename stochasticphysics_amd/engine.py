@@ -59,6 +59,8 @@ class Ensemble:
     K_cubic: Optional[np.ndarray] = None       # uniaxialcubic: K of the cubic part (K is K_uniaxial)
     uniaxial_rotation: Optional[np.ndarray] = None   # uniaxialcubic: [n, 3] Euler angles of the uniaxial part
     save_interval: int = 0                     # Save_single_simulations: keep particles p with (p+1) % k == 0
+    euler_transform: bool = False              # BrownAndNeelEulerSpherical: Use_euler_coordinate_transformation
+    brown_min_angle: float = 0.0               # BrownAndNeelEulerSpherical: Brown_Min_angle_before_transformation
 
     @property
     def n(self):
@@ -139,7 +141,8 @@ class Ensemble:
         r.noise = NOISE["replay"] if self.replay is not None else NOISE["philox"]
         r.flags = ((capi.FLAG_BN_SIMPLE if self.bn_simple else 0) | (capi.FLAG_COORD_TRANSFORM if self.coord_transform else 0)
                    | (capi.FLAG_NO_FALLBACK if self.no_fallback else 0) | (capi.FLAG_SKIP_CHECK if self.skip_check else 0)
-                   | (capi.FLAG_RENORMALIZE if self.renormalize else 0))
+                   | (capi.FLAG_RENORMALIZE if self.renormalize else 0)
+                   | (capi.FLAG_EULER_TRANSFORM if self.euler_transform else 0))
         r.newton_max_iter = self.newton_max_iter
         r.n_particles, r.n_steps, r.oversampling, r.seed = n, self.n_steps, self.oversampling, self.seed
         r.dt, r.newton_tol, r.min_angle, r.check_tol = self.dt, self.newton_tol, self.min_angle, self.check_tol
@@ -148,6 +151,7 @@ class Ensemble:
         r.field_period, r.field_time_offset, r.field_tau = self.field_period, self.field_time_offset, self.field_tau
         r.field_alternating = 1 if self.field_alternating else 0
         r.save_interval = int(self.save_interval)
+        r.brown_min_angle = self.brown_min_angle
         if self.replay is not None:
             rep = np.ascontiguousarray(self.replay, dtype=np.float64)
             keep.append(rep)

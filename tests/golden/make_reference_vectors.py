@@ -208,6 +208,57 @@ exs = [x for x in s if x[2] == "Expected"]
 add("bn_drift", cite(rel, y[0], exs[0][1]), y=y[3], simple=False, expected=exs[0][3], rtol=1e-6)
 add("bn_drift", cite(rel, y[0], exs[1][1]), y=y[3], simple=True, expected=exs[1][3], rtol=1e-6)
 
+# ---------------------------------------------------------------- BrownAndNeelEulerSpherical
+BNES_FIXTURE = dict(T=295.0, alpha=0.1, gamma=1.76e11, Ms=4.77464e5, radius=10e-9, K=-1e4, r_h=20e-9, viscosity=1e-3,
+                    brown_min_angle=math.pi / 6.0, neel_min_angle=math.pi / 6.0,
+                    cite="Tests/NeelSpherical_BrownEuler_Relaxation_Test/Test_NeelSpherical_BrownEuler_Problem.h:68-107")
+rel = "NeelSpherical_BrownEuler_Relaxation_Test/Test_NeelSpherical_BrownEuler_Problem.cpp"
+st, tr = statements(rel), test_ranges(rel)
+lines = (REF / rel).read_text(encoding="latin-1").splitlines()
+for test, (lo, hi) in tr.items():
+    body = lines[lo - 1:hi]
+    s = in_test(st, (lo, hi))
+    state = [x for x in s if x[2] == "state"]
+    if not state:
+        continue
+    flags = {}
+    for l in body:
+        m = re.search(r"EXPECT_(TRUE|FALSE)\((Brown|Neel)Cache\.isRotated\)", l)
+        if m:
+            flags[m.group(2).lower()] = m.group(1) == "TRUE"
+    prec = EIGEN_PREC
+    for l in body:
+        m = re.search(r"isApprox\(expectedresult(?:\s*,\s*([0-9.Ee+-]+))?\)", l)
+        if m and m.group(1):
+            prec = float(m.group(1))
+    c = dict(y=state[0][3], brown_rotated=flags["brown"], neel_rotated=flags["neel"], rtol=prec)
+    exp = [x for x in s if "expectedresult" in x[2]]
+    if test.startswith("Deterministic"):
+        h = [0.0, 0.0, 0.0]
+        if "WithoutField" not in test:
+            m = [re.search(r"Testfield\(([^)]*)\)", l) for l in body]
+            h = [float(v) for v in [x for x in m if x][0].group(1).split(",")]
+        c.update(h=h, expected=exp[0][3])
+        kind = "bnes_det"
+    elif test.startswith("StochasticMatrix"):
+        c.update(expected=exp[0][3])
+        kind = "bnes_sto"
+    elif test.startswith("Drift"):
+        c.update(expected=exp[0][3])
+        kind = "bnes_drift"
+    else:   # NeelRotation, BrownRot, BothRotated: prepareCalculations, then finishCalculations back
+        if exp:
+            expected = exp[0][3]
+        else:  # ``Vec5D expectedresult(state); expectedresult(i) = v;``
+            expected = list(state[0][3])
+            for l in body:
+                m = re.search(r"expectedresult\((\d)\)\s*=\s*([-0-9.Ee+]+)\s*;", l)
+                if m:
+                    expected[int(m.group(1))] = float(m.group(2))
+        c.update(expected=expected)
+        kind = "bnes_rotation"
+    add(kind, cite(rel, lo, hi), **c)
+
 # ---------------------------------------------------------------- Newton (Implicit_Solver_Test)
 roots = {
     1: ([7.0, 1.0], [17.0 / 6.0, 37.0 / 12.0], "Implicit_Solver_Test/Test_Function_1.h:42-50"),
@@ -234,7 +285,8 @@ doc = dict(
     source="Neumann-A/StochasticPhysics Tests/ (gtest, values computed in Mathematica per the test comments)",
     generator="tests/golden/make_reference_vectors.py",
     eigen_isApprox="|a-b| <= rtol * min(|a|, |b|) (Frobenius norms)",
-    fixtures=dict(neel=NEEL_FIXTURE, spherical=SPH_FIXTURE, brown_and_neel=BN_FIXTURE),
+    fixtures=dict(neel=NEEL_FIXTURE, spherical=SPH_FIXTURE, brown_and_neel=BN_FIXTURE,
+                  brown_and_neel_euler_spherical=BNES_FIXTURE),
     cases=cases,
 )
 OUT.write_text(json.dumps(doc, indent=1))

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lib().sp_abi_version() == 4
+    assert lib().sp_abi_version() == 5
 
 
 @pytest.mark.parametrize("kind,name,value", [("solver", "EulerMaruyama", 1), ("solver", "Implicit_Midpoint", 3),
@@ -167,6 +167,20 @@ def test_brown_and_neel_uniaxialcubic_is_rejected():
     e.K_cubic = None
     r, p, keep = e.to_c()
     assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), None) == 1
+
+
+def test_bnes_implicit_midpoint_is_rejected():
+    """The reference has no Jacobians for BrownAndNeelEulerSpherical (assert(false),
+    BrownAndNeelRelaxationEulerSpherical.h:606-671): Implicit_Midpoint is refused for it."""
+    import sys
+    sys.path.insert(0, str(Path(__file__).parent))
+    from test_gpu_parity import bnes_ensemble
+    e = bnes_ensemble(n=16, n_steps=100, solver="Implicit_Midpoint")
+    r, p, keep = e.to_c()
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(np.zeros(capi.NPARAM * 16)), None) == 2
+    e.solver = "Heun_Strong"
+    r, p, keep = e.to_c()
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(np.zeros(capi.NPARAM * 16)), None) == 0
 
 
 def test_request_validation_errors():

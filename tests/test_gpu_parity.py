@@ -33,7 +33,8 @@ def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False):
                            {"uniaxial": 1, "cubic": 2, "uniaxialcubic": 4}[e.anisotropy], e.n_steps, e.oversampling,
                            e.dt, e.seed,
                            e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
-                           coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
+                           coord_transform=e.coord_transform, min_angle=e.min_angle, euler_transform=e.euler_transform,
+                           brown_min_angle=e.brown_min_angle, newton_tol=e.newton_tol,
                            newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, replay=e.replay, threads=threads,
                            keep_final=keep_final, per_particle=per_particle)
 
@@ -135,6 +136,30 @@ def sph_ensemble(n=96, n_steps=2000, os_=10, solver="Heun_Strong", aniso="cubic"
                     K=np.full(n, -1.1e4 if aniso == "cubic" else 1e4), axes=configs.random_axes(rng, n),
                     y0=configs.to_spherical(m0), coord_transform=rotate, min_angle=math.pi / 36,
                     field_amplitude=(0.012, 0.012, 0.0), field_freq=(2.5e8, 2.45e8, 0.0))
+
+
+def bnes_ensemble(n=96, n_steps=2000, os_=10, solver="Heun_Strong", aniso="uniaxial", field="sinusoidal", T=295.0, seed=6,
+                  precision="fp64", rotate=True, dt=2e-12):
+    """BrownAndNeelEulerSpherical: random body orientations and moments, a few of each near the chart poles."""
+    rng = np.random.default_rng(seed)
+    eul = np.stack([rng.uniform(0, 2 * math.pi, n), np.arccos(rng.uniform(-1, 1, n)), rng.uniform(0, 2 * math.pi, n)], 1)
+    eul[:6, 1] = [0.01, 0.03, 0.05, math.pi - 0.01, math.pi - 0.03, math.pi - 0.05]   # body theta near 0 / pi
+    m0 = configs.unit_sphere(rng, n)
+    m0[6:12] = [[0, 0, 1]] * 3 + [[0, 0, -1]] * 3                                      # moments near the poles
+    m0[6:12] += 0.02 * rng.standard_normal((6, 3))
+    m0 /= np.linalg.norm(m0, axis=1, keepdims=True)
+    e = Ensemble(problem="BrownAndNeelEulerSpherical", solver=solver, field=field, anisotropy=aniso, precision=precision,
+                 n_steps=n_steps, oversampling=os_, dt=dt, seed=seed, radius=rng.uniform(9e-9, 13e-9, n),
+                 Ms=np.full(n, 4.8e5), alpha=rng.uniform(0.05, 0.3, n), gamma=np.full(n, 1.76e11), T=np.full(n, T),
+                 K=np.full(n, {"uniaxial": 5e3, "cubic": -1.1e4, "uniaxialcubic": 6e3}[aniso]),
+                 r_h=rng.uniform(20e-9, 30e-9, n), viscosity=np.full(n, 1e-3),
+                 y0=np.concatenate([eul, configs.to_spherical(m0)], axis=1), coord_transform=rotate,
+                 min_angle=math.pi / 36, euler_transform=rotate, brown_min_angle=math.pi / 36,
+                 field_amplitude=(0.0, 0.004, 0.012), field_freq=(2e8, 0, 0))
+    if aniso == "uniaxialcubic":
+        e.K_cubic = np.full(n, -1.1e4)
+        e.uniaxial_rotation = rng.uniform([0, 0, 0], [2 * math.pi, math.pi, 2 * math.pi], (n, 3))
+    return e
 
 
 def with_replay(e: Ensemble, seed=99):
@@ -294,6 +319,43 @@ def test_uniaxialcubic_parity(ctx, case):
 def test_uniaxialcubic_mixed_precision(ctx):
     e = configs.uniaxialcubic(n=4096, problem="NeelSpherical", solver="Heun_Strong", n_steps=4000, oversampling=10,
                               precision="mixed")
+    g = ctx.run(e)
+    o = run_oracle(e, keep_final=False)
+    assert g["precision_used"] == 1
+    assert np.max(np.abs(g["mean"] - o["mean"])) <= 1e-3
+
+
+def bnes_points(y):
+    """Body x and y axes and the moment of BrownAndNeelEulerSpherical states (angles are chart dependent)."""
+    c, s_ = np.cos(y), np.sin(y)
+    cphi, cth, cpsi, sphi, sth, spsi = c[:, 0], c[:, 1], c[:, 2], s_[:, 0], s_[:, 1], s_[:, 2]
+    return np.stack([cphi * cpsi - cth * sphi * spsi, cth * sphi * cpsi + cphi * spsi, sth * sphi,
+                     -sphi * cpsi - cth * cphi * spsi, cth * cphi * cpsi - sphi * spsi, sth * cphi,
+                     s_[:, 3] * c[:, 4], s_[:, 3] * s_[:, 4], c[:, 3]], 1)
+
+
+def compare_bnes(g, o, tol):
+    compare(g, o, tol, state=False)
+    assert np.max(np.abs(bnes_points(g["final_state"]) - bnes_points(o["final_state"]))) <= tol
+
+
+@pytest.mark.parametrize("solver", ["EulerMaruyama", "Heun_Strong", "Millstein"])
+@pytest.mark.parametrize("aniso", ["uniaxial", "cubic", "uniaxialcubic"])
+def test_bnes_parity(ctx, solver, aniso):
+    """BrownAndNeelEulerSpherical (BrownAndNeelRelaxationEulerSpherical.h): both charts rotate near their poles
+    (theta_min = pi/36, particles start there); an FMA-contracted oracle build differs from the plain one by at most
+    7e-12 on these cases."""
+    e = bnes_ensemble(solver=solver, aniso=aniso)
+    compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+def test_bnes_replay_parity(ctx):
+    e = with_replay(bnes_ensemble(solver="EulerMaruyama", n=130))   # ragged: 3 tiles
+    compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+
+
+def test_bnes_mixed_precision(ctx):
+    e = bnes_ensemble(solver="Heun_Strong", n=4096, n_steps=4000, precision="mixed")
     g = ctx.run(e)
     o = run_oracle(e, keep_final=False)
     assert g["precision_used"] == 1

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from stochasticphysics_amd import Context, configs
-from test_gpu_parity import bn_ensemble, run_oracle, sph_ensemble
+from test_gpu_parity import bn_ensemble, bnes_ensemble, run_oracle, sph_ensemble
 
 pytestmark = pytest.mark.gpu
 
@@ -25,6 +25,7 @@ def kept(e):
 
 
 @pytest.mark.parametrize("make", [lambda: bn_ensemble(n=130, n_steps=9000, os_=50),
+                                  lambda: bnes_ensemble(n=100, n_steps=3000, os_=100),
                                   lambda: sph_ensemble(n=96, n_steps=3000, os_=100),
                                   lambda: configs.uniaxialcubic(n=100, problem="Neel", n_steps=3000, oversampling=100)])
 def test_trajectories_match_oracle(make):
@@ -59,7 +60,8 @@ def test_trajectory_row0_is_the_start_output(ctx):
 def test_trajectories_same_on_every_path(ctx):
     """Plain, batched (persistent launch, hipGraph) and served results carry bitwise the same trajectories."""
     sets = [bn_ensemble(n=96, n_steps=3000, os_=50, seed=21), sph_ensemble(n=70, n_steps=3000, os_=100, seed=22),
-            configs.uniaxialcubic(n=90, problem="NeelSpherical", n_steps=3000, oversampling=100, seed=23)]
+            configs.uniaxialcubic(n=90, problem="NeelSpherical", n_steps=3000, oversampling=100, seed=23),
+            bnes_ensemble(n=80, n_steps=3000, os_=100, seed=24)]
     for k, e in enumerate(sets):
         e.save_interval = 3 + k
     plain = [ctx.run(e) for e in sets]

@@ -25,9 +25,10 @@ SOLVER = {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint"
 
 
 def run(e: Ensemble, threads=8):
-    return oracle.simulate(e.pid, SOLVER[e.solver], {"uniaxial": 1, "cubic": 2}[e.anisotropy], e.n_steps, e.oversampling,
+    return oracle.simulate(e.pid, SOLVER[e.solver], {"uniaxial": 1, "cubic": 2, "uniaxialcubic": 4}[e.anisotropy], e.n_steps, e.oversampling,
                            e.dt, e.seed, e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
-                           coord_transform=e.coord_transform, min_angle=e.min_angle, newton_tol=e.newton_tol,
+                           coord_transform=e.coord_transform, min_angle=e.min_angle, euler_transform=e.euler_transform,
+                           brown_min_angle=e.brown_min_angle, newton_tol=e.newton_tol,
                            newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, threads=threads, keep_final=True)
 
 
@@ -56,12 +57,17 @@ def precession(problem, solver, dt, n_steps, os_=10, rotate=False, m0=(0.6, 0.0,
         kw.update(y0=configs.to_spherical(m0[None, :]), coord_transform=rotate, min_angle=math.pi / 6)
     elif problem == "BrownAndNeel":   # K = 0: the axis n is carried along, m follows the Neel law with b + d
         kw.update(y0=np.concatenate([[0.0, 0.0, 1.0], m0])[None, :], r_h=np.full(n, 25e-9), viscosity=np.full(n, 1e-3))
+    elif problem == "BrownAndNeelEulerSpherical":   # the same law for the Neel angles; both charts may rotate
+        kw.update(y0=np.concatenate([[0.3, 0.2 if rotate else 1.0, 0.5], configs.to_spherical(m0[None, :])[0]])[None, :],
+                  r_h=np.full(n, 25e-9), viscosity=np.full(n, 1e-3), coord_transform=rotate, min_angle=math.pi / 6,
+                  euler_transform=rotate, brown_min_angle=math.pi / 6)
     else:
         kw.update(y0=m0[None, :])
     e = Ensemble(**kw)
     o = run(e, threads=1)
     y = o["final_state"][0]
-    if problem == "NeelSpherical":
+    if problem in ("NeelSpherical", "BrownAndNeelEulerSpherical"):
+        y = y[-2:]
         m = np.array([math.sin(y[0]) * math.cos(y[1]), math.sin(y[0]) * math.sin(y[1]), math.cos(y[0])])
     elif problem == "BrownAndNeel":
         m = y[3:]
@@ -69,7 +75,7 @@ def precession(problem, solver, dt, n_steps, os_=10, rotate=False, m0=(0.6, 0.0,
         m = y
     t = dt * ((n_steps // os_) - 1) * os_
     b = B_N
-    if problem == "BrownAndNeel":     # omega_N = -a H + (b + d) m x H: the Neel law with b -> b + d, d = c mu
+    if problem.startswith("BrownAndNeel"):     # omega_N = -a H + (b + d) m x H: the Neel law with b -> b + d, d = c mu
         mu = 4.8e5 * 4.0 / 3.0 * math.pi * R ** 3
         b = B_N + mu / (6e-3 * 4.0 / 3.0 * math.pi * (25e-9) ** 3)
     return np.max(np.abs(m - exact_m(m0, t, b)))
@@ -78,13 +84,14 @@ def precession(problem, solver, dt, n_steps, os_=10, rotate=False, m0=(0.6, 0.0,
 ORDERS = [("Neel", "EulerMaruyama", 1), ("Neel", "Heun_Strong", 2), ("Neel", "Millstein", 1),
           ("Neel", "Implicit_Midpoint", 2), ("NeelSpherical", "EulerMaruyama", 1), ("NeelSpherical", "Heun_Strong", 2),
           ("NeelSpherical", "Implicit_Midpoint", 2), ("BrownAndNeel", "EulerMaruyama", 1),
-          ("BrownAndNeel", "Heun_Strong", 2)]
+          ("BrownAndNeel", "Heun_Strong", 2), ("BrownAndNeelEulerSpherical", "EulerMaruyama", 1),
+          ("BrownAndNeelEulerSpherical", "Heun_Strong", 2), ("BrownAndNeelEulerSpherical", "Millstein", 1)]
 
 
 @pytest.mark.parametrize("problem,solver,order", ORDERS)
 def test_zero_temperature_convergence_order(problem, solver, order):
     """T = 0, K = 0, constant field: the final state converges to the closed form with the solver's order."""
-    dt = 4e-13 if problem != "BrownAndNeel" else 2e-13
+    dt = 4e-13 if not problem.startswith("BrownAndNeel") else 2e-13
     e1 = precession(problem, solver, dt, 5000)
     e2 = precession(problem, solver, dt / 2, 10000 - 10)   # same final time: (n_out - 1) OS steps
     ratio = e1 / e2
@@ -187,3 +194,65 @@ def test_brownian_rotation_decay():
     assert abs(o["final_state"][:, 2].mean() - expect) < 3 * math.sqrt(var / n), (o["final_state"][:, 2].mean(), expect)
     assert np.allclose(np.linalg.norm(o["final_state"][:, :3], axis=1), 1.0, atol=1e-12)
     assert np.allclose(np.linalg.norm(o["final_state"][:, 3:], axis=1), 1.0, atol=1e-12)
+
+
+def test_bnes_charts_track_the_closed_form():
+    """BrownAndNeelEulerSpherical with both chart rotations on (pi/6): the magnetisation starts in the rotated Neel chart
+    (theta_N > 5 pi/6) and the body in the rotated Euler chart (theta = 0.2 < pi/6); the Neel angles follow the closed
+    form with b + d, and Heun_Strong keeps its order 2 through the chart switches."""
+    m0 = (math.sqrt(0.19), 0.0, -0.9)
+    e1 = precession("BrownAndNeelEulerSpherical", "Heun_Strong", 2e-13, 10000, rotate=True, m0=m0)
+    e2 = precession("BrownAndNeelEulerSpherical", "Heun_Strong", 1e-13, 19990, rotate=True, m0=m0)
+    assert e1 < 2e-4 and 3.2 < e1 / e2 < 5.0, (e1, e2)
+
+
+def bnes_free(n, steps, dt, os_, eta, r_h, T=300.0, Bz=0.0, seed=5, euler0=(0.3, 1.0, 0.5), solver="Heun_Strong"):
+    rng = np.random.default_rng(seed)
+    m0 = configs.unit_sphere(rng, n)
+    y0 = np.concatenate([np.tile(euler0, (n, 1)), configs.to_spherical(m0)], axis=1)
+    e = Ensemble(problem="BrownAndNeelEulerSpherical", solver=solver, anisotropy="uniaxial", field="sinusoidal",
+                 n_steps=steps, oversampling=os_, dt=dt, seed=seed, radius=np.full(n, R), Ms=np.full(n, 4.8e5),
+                 alpha=np.full(n, 0.3), gamma=np.full(n, configs.GAMMA), T=np.full(n, T), K=np.zeros(n),
+                 r_h=np.full(n, r_h), viscosity=np.full(n, eta), y0=y0, field_offset=(0.0, 0.0, Bz),
+                 coord_transform=True, min_angle=math.pi / 36, euler_transform=True, brown_min_angle=math.pi / 36)
+    return e, run(e)
+
+
+def test_bnes_brownian_rotation_decay():
+    """K = 0, no field: the body diffuses freely, so its x axis decorrelates as <x(t).x(0)> = exp(-2 D_r t),
+    D_r = kB T c (ParameterCalculatorBrown.h:43; the Neel-noise share of the body's rotational diffusion,
+    (d sigma_N)^2 / cF^2 = c mu alpha / gamma, is 1e-6 here).  This pins the Euler-angle drift (the c^2 D_B^2 / 2 cot
+    theta term) and the projection of the torque noise; the body passes the Euler chart window many times."""
+    n = 2000
+    r_h, eta, T = 12e-9, 2e-6, 300.0
+    Dr = KB * T / (6 * eta * 4.0 / 3.0 * math.pi * r_h ** 3)
+    dt, os_ = 1e-12, 50
+    steps = int(0.6 / (2 * Dr) / dt) // os_ * os_
+    euler0 = (0.3, 1.0, 0.5)
+    e, o = bnes_free(n, steps, dt, os_, eta, r_h, T, euler0=euler0)
+    x0 = oracle.evaluate(4, 1, e.oracle_props()[0], "output", np.concatenate([euler0, [1.0, 1.0]]))[:3]
+    k = len(o["times"]) - 1
+    tt = dt * k * os_
+    expect = math.exp(-2 * Dr * tt)
+    f = o["final_state"]
+    xs = np.array([oracle.evaluate(4, 1, e.oracle_props()[0], "output", y)[:3] for y in f])
+    proj = xs @ x0
+    var = (1 + 2 * math.exp(-6 * Dr * tt)) / 3 - math.exp(-4 * Dr * tt)
+    assert abs(proj.mean() - expect) < 3 * math.sqrt(var / n), (proj.mean(), expect)
+    assert abs(o["mean"][-1, 0:3] @ x0 - math.exp(-2 * Dr * (tt - dt * os_ / 2))) < 0.05
+
+
+def test_bnes_langevin_equilibrium():
+    """K = 0 in a constant field: the stationary law of m is the Langevin one whatever the body does,
+    <m_z> -> L(mu B / kB T) (3 sigma / sqrt(N)); this pins the Neel block's noise and its Stratonovich drift."""
+    n = 1500
+    mu = 4.8e5 * 4.0 / 3.0 * math.pi * R ** 3
+    Bz = 0.004
+    xi = mu * Bz / (KB * 300.0)
+    e, o = bnes_free(n, 40_000, 1e-12, 100, 1e-3, 25e-9, Bz=Bz, solver="EulerMaruyama")
+    f = o["final_state"][:, 3:]
+    mz = np.cos(f[:, 0])
+    sig = math.sqrt(1.0 - 2.0 * langevin(xi) / xi - langevin(xi) ** 2)
+    assert abs(mz.mean() - langevin(xi)) < 3 * sig / math.sqrt(n), (mz.mean(), langevin(xi))
+    tail = o["mean"][len(o["mean"]) // 2:, 8]
+    assert abs(tail.mean() - langevin(xi)) < 3 * sig / math.sqrt(n)

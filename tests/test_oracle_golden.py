@@ -158,6 +158,67 @@ def test_newton_roots(c):
     assert ok and it > 0, (x, it)
 
 
+def bnes_kw():
+    f = FIX["brown_and_neel_euler_spherical"]
+    return dict(rotate=True, min_angle=f["neel_min_angle"], euler_rotate=True, brown_min_angle=f["brown_min_angle"])
+
+
+@pytest.mark.parametrize("c", ids("bnes_det") + ids("bnes_drift") + ids("bnes_sto"), ids=cid)
+def test_bnes_terms(c):
+    """BrownAndNeelEulerSpherical det (with and without field), stochastic matrix and drift in all four chart
+    combinations (Test_NeelSpherical_BrownEuler_Problem.cpp), with the rotation flags the reference test asserts."""
+    props = props_of(FIX["brown_and_neel_euler_spherical"])
+    what = {"bnes_det": "det", "bnes_drift": "drift", "bnes_sto": "sto"}[c["kind"]]
+    got = oracle.evaluate(4, 1, props, what, c["y"], c.get("h"), **bnes_kw())
+    assert is_approx(got, c["expected"], c["rtol"]), (got, c["expected"])
+    flags = oracle.evaluate(4, 1, props, "rotated", c["y"], **bnes_kw())
+    assert (bool(flags[0]), bool(flags[1])) == (c["brown_rotated"], c["neel_rotated"])
+
+
+@pytest.mark.parametrize("c", ids("bnes_rotation"), ids=cid)
+def test_bnes_rotation_roundtrip(c):
+    """prepareCalculations rotates the chart(s) to the Mathematica state; finishCalculations maps it back (the
+    reference test adds 2 pi to negative angles of the rotated blocks before comparing)."""
+    props = props_of(FIX["brown_and_neel_euler_spherical"])
+    out = oracle.evaluate(4, 1, props, "finish", c["y"], **bnes_kw())
+    prepared, back = out[:5], out[5:].copy()
+    assert is_approx(prepared, c["expected"], c["rtol"]), (prepared, c["expected"])
+    blocks = ([0, 1, 2] if c["brown_rotated"] else []) + ([3, 4] if c["neel_rotated"] else [])
+    for i in blocks:
+        if back[i] < 0:
+            back[i] += 2 * math.pi
+    assert is_approx(back, c["y"], 1e-12), (back, c["y"])
+
+
+def test_bnes_output_and_chart_invariance():
+    """Output = (body x axis, body y axis, m) (BrownAndNeelRelaxationEulerSpherical.h:674-735): the axes are the
+    columns of scipy's extrinsic z-x-z rotation of the Euler angles, m the spherical unit vector.  Closed forms of the
+    chart maps: the rotated Euler angles and spherical angles describe the body axes and the moment turned by +90
+    degrees about y (Ry R, Ry m), and mapping back returns the same body axes and the same m."""
+    from scipy.spatial.transform import Rotation
+    props = props_of(FIX["brown_and_neel_euler_spherical"])
+    rng = np.random.default_rng(4)
+    for _ in range(16):
+        y = np.concatenate([rng.uniform(0, 2 * math.pi, 1), rng.uniform(0.05, math.pi - 0.05, 1), rng.uniform(0, 2 * math.pi, 1),
+                            rng.uniform(0.05, math.pi - 0.05, 1), rng.uniform(0, 2 * math.pi, 1)])
+        o = oracle.evaluate(4, 1, props, "output", y)
+        R = Rotation.from_euler("zxz", y[:3]).as_matrix()
+        assert np.allclose(o[0:3], R[:, 0], atol=1e-15) and np.allclose(o[3:6], R[:, 1], atol=1e-15)
+        th, ph = y[3], y[4]
+        assert np.allclose(o[6:9], [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)], atol=1e-15)
+        # force both charts (windows wider than the state): prepared angles describe the same body / moment seen from
+        # the frame turned by 90 degrees about y, and finish maps them back
+        kw = dict(rotate=True, min_angle=math.pi / 2 + 0.01, euler_rotate=True, brown_min_angle=math.pi / 2 + 0.01)
+        out = oracle.evaluate(4, 1, props, "finish", y, **kw)
+        Ry = Rotation.from_euler("y", 90, degrees=True).as_matrix()
+        Rp = Rotation.from_euler("zxz", out[:3]).as_matrix()
+        assert np.allclose(Ry @ R, Rp, atol=1e-12)
+        tp, pp = out[3], out[4]
+        assert np.allclose(Ry @ o[6:9], [math.sin(tp) * math.cos(pp), math.sin(tp) * math.sin(pp), math.cos(tp)], atol=1e-12)
+        ob = oracle.evaluate(4, 1, props, "output", out[5:])
+        assert np.allclose(ob, o, atol=1e-12)
+
+
 def test_neel_param_closed_forms():
     """Tests/Neel_Problem_Test/Test_ParamCalc_Neel.cpp:12-126 (Ms=477464, r=10nm, alpha=0.1, gamma=1.76e11)."""
     Ms, r, al, g, T = 477464.0, 10e-9, 0.1, 1.76e11, 295.0
