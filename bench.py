@@ -227,6 +227,40 @@ def neel_relaxation(ctx, configs):
                 note="the GPU is latency bound here: 4,096 particles are 64 waves on 1,024 SIMDs")
 
 
+def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
+    """FP64 and mixed throughput of every problem class on one 65,536-particle ensemble each (20,000 steps, inputs
+    resident): Neel EulerMaruyama, NeelSpherical Heun_Strong (cubic), BrownAndNeel Heun_Strong, BrownAndNeelEulerSpherical
+    Heun_Strong (both charts on), Neel and NeelSpherical with uniaxialcubic anisotropy."""
+    makers = {
+        "Neel uniaxial EulerMaruyama": lambda prec: configs.config1(n=n, n_steps=n_steps, precision=prec),
+        "NeelSpherical cubic Heun_Strong": lambda prec: configs.config3(n=n, n_steps=n_steps, precision=prec),
+        "BrownAndNeel uniaxial Heun_Strong": lambda prec: configs.config2(n=n, n_steps=n_steps, precision=prec),
+        "BrownAndNeelEulerSpherical uniaxial Heun_Strong": lambda prec: configs.bnes(n=n, n_steps=n_steps, precision=prec),
+        "Neel uniaxialcubic Heun_Strong": lambda prec: configs.uniaxialcubic(n=n, n_steps=n_steps, precision=prec),
+        "NeelSpherical uniaxialcubic Heun_Strong": lambda prec: configs.uniaxialcubic(
+            n=n, problem="NeelSpherical", n_steps=n_steps, precision=prec),
+    }
+    out = {}
+    for name, make in makers.items():
+        row = {}
+        for prec in ("fp64", "mixed"):
+            e = make(prec)
+            e.skip_check = True
+            p = ctx.prepare(e)
+            p.execute()
+            t0 = time.perf_counter()
+            p.execute()
+            dt = time.perf_counter() - t0
+            st = ctx.last_run_stats()
+            p.release()
+            row[prec] = e.particle_steps() / dt
+            row[prec + "_vgprs"] = st["vgprs"]
+        log(f"bench: {name}: fp64 {row['fp64']:.3e}, mixed {row['mixed']:.3e} particle-steps/s")
+        out[name] = row
+    return dict(workload=f"{n} particles x {n_steps} steps per ensemble (inputs resident, mixed without its FP64 check)",
+                unit="particle-steps/s", results=out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,9 +273,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-sets", type=int, default=48, help="uneven sets in the mixed/batched leg")
-    ap.add_argument("--workload", choices=("headline", "precision", "neel"), default="headline",
-                    help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation "
-                         "(each prints its own JSON line instead of the headline one)")
+    ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems"), default="headline",
+                    help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation; "
+                         "problems: every problem class on a 65,536-particle ensemble (each prints its own JSON line "
+                         "instead of the headline one)")
+    ap.add_argument("--no-problems", action="store_true", help="headline run without the per-problem leg")
     ap.add_argument("--particles", type=int, default=1_048_576, help="precision study: particles")
     ap.add_argument("--study-steps", type=int, default=500_000, help="precision study: steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -269,6 +305,8 @@ def main():
         ctx = Context(0)
         if args.workload == "precision":
             res = precision_study(ctx, configs, args.particles, args.study_steps)
+        elif args.workload == "problems":
+            res = problem_rates(ctx, configs)
         else:
             res = neel_relaxation(ctx, configs)
         ctx.close()
@@ -317,6 +355,8 @@ def main():
         out["mixed"] = mixed_leg(ctx, configs, args.batch_sets if not args.quick else 4)
     if not args.no_serving:
         out["serving"] = serving_leg(ctx, configs, args.serve_seconds, args.serve_rate)
+    if not args.no_problems and not args.quick:
+        out["problems"] = problem_rates(ctx, configs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(e)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]

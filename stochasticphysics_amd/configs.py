@@ -148,3 +148,22 @@ def uniaxialcubic(n=4096, problem="Neel", solver="Heun_Strong", n_steps=20_000, 
                     K_cubic=full(n, Kc), uniaxial_rotation=rot, axes=axes, y0=y0,
                     coord_transform=problem == "NeelSpherical", min_angle=math.pi / 36,
                     field_amplitude=(0.012, 0.012, 0.0), field_freq=(25.25e3, 24.5e3, 0.0))
+
+
+def bnes(n=65536, n_steps=200_000, oversampling=50, seed=77, solver="Heun_Strong", precision="fp64", anisotropy="uniaxial"):
+    """BrownAndNeelEulerSpherical with the coupled Brown+Neel ensemble's particles (r 12 nm, r_h 25 nm, K 5e3,
+    damping 0.1, 295 K, 12 mT 100 kHz sinusoidal z-field, dt 1e-11 s): uniformly random body orientations (Euler
+    angles) and moments, both chart rotations on at pi/36."""
+    rng = np.random.default_rng(seed)
+    eul = np.stack([rng.uniform(0, 2 * math.pi, n), np.arccos(rng.uniform(-1, 1, n)), rng.uniform(0, 2 * math.pi, n)], 1)
+    e = Ensemble(problem="BrownAndNeelEulerSpherical", solver=solver, field="sinusoidal", anisotropy=anisotropy,
+                 precision=precision, n_steps=n_steps, oversampling=oversampling, dt=1e-11, seed=seed,
+                 radius=full(n, 12e-9), Ms=full(n, 4.8e5), alpha=full(n, 0.1), gamma=full(n, GAMMA), T=full(n, 295.0),
+                 K=full(n, 5e3 if anisotropy != "cubic" else -1.1e4), r_h=full(n, 25e-9), viscosity=full(n, 1e-3),
+                 y0=np.concatenate([eul, to_spherical(unit_sphere(rng, n))], axis=1), coord_transform=True,
+                 min_angle=math.pi / 36, euler_transform=True, brown_min_angle=math.pi / 36,
+                 field_amplitude=(0.0, 0.0, 0.012), field_freq=(1e5, 0.0, 0.0))
+    if anisotropy == "uniaxialcubic":
+        e.K_cubic = full(n, -1.1e4)
+        e.uniaxial_rotation = rng.uniform([0, 0, 0], [2 * math.pi, math.pi, 2 * math.pi], (n, 3))
+    return e
