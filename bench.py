@@ -251,10 +251,8 @@ def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
             t0 = time.perf_counter()
             p.execute()
             dt = time.perf_counter() - t0
-            st = ctx.last_run_stats()
             p.release()
             row[prec] = e.particle_steps() / dt
-            row[prec + "_vgprs"] = st["vgprs"]
         log(f"bench: {name}: fp64 {row['fp64']:.3e}, mixed {row['mixed']:.3e} particle-steps/s")
         out[name] = row
     return dict(workload=f"{n} particles x {n_steps} steps per ensemble (inputs resident, mixed without its FP64 check)",
@@ -346,8 +344,7 @@ def main():
                roofline=dict(bound="valu", achieved=achieved, peak=PEAK_FP64_VALU_TFLOPS, unit="TFLOP/s",
                              frac=achieved / PEAK_FP64_VALU_TFLOPS, traffic=traffic_per_launch(),
                              kernel="k_plain<BNCfg<uniaxial,Heun_Strong,double>>", kernel_avg_ms=kern_avg_ms,
-                             launches_per_pass=launches // args.steps, flops_per_particle_step=FLOPS_PER_STEP,
-                             vgprs=st["vgprs"]),
+                             launches_per_pass=launches // args.steps, flops_per_particle_step=FLOPS_PER_STEP),
                reference_scheme=dict(nonfinite_particles=bad, finite_mean=bool(np.isfinite(ref["mean"]).all()),
                                      note="Heun_Strong as the reference writes it does not keep |m| = 1 for this "
                                           "problem; particles whose |m| runs away overflow, as in the reference"))
