@@ -858,14 +858,20 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             fc.ensure<MIDPOINT>(rq, n0, G);
             NormalsT<R>::philox(seed, p, ((n0 + G) * DW) >> 2, zn);
+            // the G outputs of the group are added first and enter the window's compensated sum as one term
+            // (the reference compensates every step, SingleParticleSimulator.h:148-153; the two differ by the
+            // rounding of G - 1 plain additions)
+            double gsum[DOUT];
 #pragma unroll
             for (int s = 0; s < G; ++s) {
                 Cfg::step(rq, L, y, cache, zc + s * DW, fc.at(n0 + s));
                 if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
                 double o[DOUT];
                 Cfg::output(L, y, cache, o);
-                accumulate(o);
+#pragma unroll
+                for (int c = 0; c < DOUT; ++c) gsum[c] = s == 0 ? o[c] : gsum[c] + o[c];
             }
+            accumulate(gsum);
             l += G;
             if (l == OS) { close_window(); l = 0; }
 #pragma unroll
