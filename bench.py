@@ -59,7 +59,7 @@ def cpu_baseline(e, budget_s=12.0):
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    solver = {"EulerMaruyama": 1, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7}[e.solver]
+    solver = {"EulerMaruyama": 1, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7, "Heun_NotConsistent": 8}[e.solver]
 
     def run(sub, steps):
         t0 = time.perf_counter()

@@ -48,7 +48,9 @@ enum { SP_PROBLEM_NEEL = 1, SP_PROBLEM_NEEL_SPHERICAL = 2, SP_PROBLEM_BROWN_AND_
        SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL = 4 };
 /* Settings::ISolver (Settings/SolverSettings.h:34-47) */
 enum { SP_SOLVER_EULER_MARUYAMA = 1, SP_SOLVER_EULER_MARUYAMA_NORMALIZED = 2, SP_SOLVER_IMPLICIT_MIDPOINT = 3,
-       SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7 };
+       SP_SOLVER_MILLSTEIN = 6, SP_SOLVER_HEUN_STRONG = 7,
+       SP_SOLVER_HEUN_NOT_CONSISTENT = 8 /* two noise draws per step (SDEFramework/Solver/Heun_NotConsistent.inl:34-45);
+                                            runs in FP64 whatever the precision asked for */ };
 /* Properties::IField (Fields/FieldList.h:48-59) */
 enum { SP_FIELD_ZERO = 1, SP_FIELD_CONSTANT = 2, SP_FIELD_SINUSOIDAL = 3, SP_FIELD_LISSAJOUS = 4,
        SP_FIELD_TRIANGULAR = 5, SP_FIELD_RECTANGULAR = 6, SP_FIELD_SINC = 7 };
@@ -110,7 +112,8 @@ typedef struct sp_request {
     double field_amplitude[3];
     double field_freq[3];
     double field_phase[3];
-    /* SP_NOISE_REPLAY only: standard normals z[k*n_particles + p], k = step*dim_W + j */
+    /* SP_NOISE_REPLAY only: standard normals z[k*n_particles + p], k = step*dim_W + j (k = step*2*dim_W + j for
+     * Heun_NotConsistent: dW then dW2) */
     const double* replay_normals;
     uint64_t replay_count;    /* number of k values available */
     double field_period;      /* triangular, rectangular, sinc: Periode [s] */

@@ -211,7 +211,7 @@ int or_simulate(const int64_t* ispec, const uint64_t* uspec, const double* dspec
     rs.newtonMaxIter = (int)ispec[4];
     const int nThreads = (int)ispec[5];
     if (rs.solver != SOLVER_EM && rs.solver != SOLVER_EM_NORMALIZED && rs.solver != SOLVER_IM && rs.solver != SOLVER_MILSTEIN &&
-        rs.solver != SOLVER_HEUN)
+        rs.solver != SOLVER_HEUN && rs.solver != SOLVER_HNC)
         return 3;
     rs.nParticles = uspec[0]; rs.nSteps = uspec[1]; rs.oversampling = uspec[2]; rs.seed = uspec[3];
     rs.replayCount = uspec[4]; rs.replay = replay;

@@ -21,7 +21,7 @@
 
 namespace oracle {
 
-enum { SOLVER_EM = 1, SOLVER_EM_NORMALIZED = 2, SOLVER_IM = 3, SOLVER_MILSTEIN = 6, SOLVER_HEUN = 7 };
+enum { SOLVER_EM = 1, SOLVER_EM_NORMALIZED = 2, SOLVER_IM = 3, SOLVER_MILSTEIN = 6, SOLVER_HEUN = 7, SOLVER_HNC = 8 };
 enum { PROB_NEEL = 1, PROB_SPHERICAL = 2, PROB_BN = 3, PROB_BNES = 4 };
 
 struct RunSpec {
@@ -104,6 +104,21 @@ template <class Prob> struct CartesianStepper {
         Y support; for (int i = 0; i < DY; ++i) support[i] = yi[i] + ayi[i] * dt + bdW[i];
         const Y a2 = P.det(support, xi) + P.drift(support);
         Y out; for (int i = 0; i < DY; ++i) out[i] = yi[i] + 0.5 * (a2[i] + ayi[i]) * dt + bdW[i];
+        return out;
+    }
+    // Heun_NotConsistent.inl:34-45: dW and dW2 drawn in this order; S at y_i and at the support on dW2
+    Y heunNC(const Y& yi, double time, NoiseSource& ns) {
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, DW> dW = noiseVec<Prob, DW>(ns, std::sqrt(dt));
+        const Vec<double, DW> dW2 = noiseVec<Prob, DW>(ns, std::sqrt(dt));
+        const Y ayi = P.det(yi, xi) + P.drift(yi);
+        const auto byi = P.sto(yi);
+        const Y bdW = matvec(byi, dW);
+        Y support; for (int i = 0; i < DY; ++i) support[i] = yi[i] + ayi[i] * dt + bdW[i];
+        const Y a2 = P.det(support, xi) + P.drift(support);
+        const Y b2 = matvec(P.sto(support), dW2), b1 = matvec(byi, dW2);
+        Y out; for (int i = 0; i < DY; ++i) out[i] = yi[i] + 0.5 * (a2[i] + ayi[i]) * dt + 0.5 * (b2[i] + b1[i]);
         return out;
     }
     Y milstein(const Y& yi, double time, NoiseSource& ns) {                            // Millstein.inl:34-44
@@ -200,6 +215,23 @@ template <class Aniso> struct SphericalStepper {
         P.finish(out, c0);
         return out;
     }
+    Y heunNC(const Y& yi, double time, NoiseSource& ns) {       // Heun_NotConsistent.inl:34-45, chart of y_i
+        Y res = yi; SphericalCache<Aniso, double> c0, c1;
+        P.prepare(res, c0);
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, 3> dW = noiseVec<Prob, 3>(ns, std::sqrt(dt));
+        const Vec<double, 3> dW2 = noiseVec<Prob, 3>(ns, std::sqrt(dt));
+        const Y ayi = P.det(c0, xi) + P.drift(c0);
+        const Y bdW = matvec(P.sto(c0), dW);
+        Y support; for (int i = 0; i < 2; ++i) support[i] = res[i] + ayi[i] * dt + bdW[i];
+        chartAt(support, c0.isRotated, c1);
+        const Y a2 = P.det(c1, xi) + P.drift(c1);
+        const Y b2 = matvec(P.sto(c1), dW2), b1 = matvec(P.sto(c0), dW2);
+        Y out; for (int i = 0; i < 2; ++i) out[i] = res[i] + 0.5 * (a2[i] + ayi[i]) * dt + 0.5 * (b2[i] + b1[i]);
+        P.finish(out, c0);
+        return out;
+    }
     Y milstein(const Y& yi, double time, NoiseSource& ns) {
         Y res = yi; SphericalCache<Aniso, double> c;
         P.prepare(res, c);
@@ -277,6 +309,23 @@ template <class Aniso> struct BNESStepper {
         P.finish(out, c0);
         return out;
     }
+    Y heunNC(const Y& yi, double time, NoiseSource& ns) {                              // Heun_NotConsistent.inl:34-45
+        Y res = yi; BNESCache c0, c1;
+        P.prepare(res, c0);
+        const Vec<double, 3> xi = rs.field.at(time);
+        const double dt = rs.dt;
+        const Vec<double, 6> dW = noiseVec<Prob, 6>(ns, std::sqrt(dt));
+        const Vec<double, 6> dW2 = noiseVec<Prob, 6>(ns, std::sqrt(dt));
+        const Y ayi = P.det(c0, xi) + P.drift(res, c0);
+        const Y bdW = matvec(P.sto(c0), dW);
+        Y support; for (int i = 0; i < 5; ++i) support[i] = res[i] + ayi[i] * dt + bdW[i];
+        P.chartAt(support, c0.brownRot, c0.neelRot, c1);
+        const Y a2 = P.det(c1, xi) + P.drift(support, c1);
+        const Y b2 = matvec(P.sto(c1), dW2), b1 = matvec(P.sto(c0), dW2);
+        Y out; for (int i = 0; i < 5; ++i) out[i] = res[i] + 0.5 * (a2[i] + ayi[i]) * dt + 0.5 * (b2[i] + b1[i]);
+        P.finish(out, c0);
+        return out;
+    }
     Y milstein(const Y& yi, double time, NoiseSource& ns) {                            // Millstein.inl:34-44
         Y res = yi; BNESCache c;
         P.prepare(res, c);
@@ -321,6 +370,7 @@ void simulateParticle(Prob& prob, Stepper& st, const RunSpec& rs, uint64_t p, co
                     prob.normalize(yi);
                     break;
                 case SOLVER_HEUN: yi = st.heun(yi, time, ns); break;
+                case SOLVER_HNC: yi = st.heunNC(yi, time, ns); break;
                 case SOLVER_MILSTEIN: yi = st.milstein(yi, time, ns); break;
                 default: yi = st.implicitMidpoint(yi, time, ns); break;
             }

@@ -27,8 +27,12 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
 
+# (source, extra flags, object name); a source compiled several times gets one object per flag set
 SOURCES = [
-    "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_uc.hip", "sp_kernels_bnes.hip",
+    "sp_kernels_neel.hip", "sp_kernels_sph.hip", "sp_kernels_bn.hip", "sp_kernels_uc.hip",
+    ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=1"], "sp_kernels_bnes_a1.o"),
+    ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=2"], "sp_kernels_bnes_a2.o"),
+    ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=4"], "sp_kernels_bnes_a4.o"),
     "sp_kernels_common.hip",
     "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
     "frontend/ini.cpp", "frontend/settings.cpp",
@@ -42,9 +46,10 @@ def hipcc():
     raise RuntimeError("hipcc not found: the gfx950 library cannot be built")
 
 
-def _compile(src):
-    obj = BUILD / (Path(src).stem + ".o")
-    cmd = [hipcc(), *HIP_FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+def _compile(entry):
+    src, extra, obj_name = (entry, [], Path(entry).stem + ".o") if isinstance(entry, str) else entry
+    obj = BUILD / obj_name
+    cmd = [hipcc(), *HIP_FLAGS, *extra, "-c", str(CSRC / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     # the host pass warns that it does not know the packed-fp32 target feature
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())

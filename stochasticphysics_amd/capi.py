@@ -12,7 +12,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libstophys_gpu.so"
 
 # enum values (identical to the reference's selector enums)
 PROBLEM = dict(Neel=1, NeelSpherical=2, BrownAndNeel=3, BrownAndNeelEulerSpherical=4)
-SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7)
+SOLVER = dict(EulerMaruyama=1, EulerMaruyama_Normalized=2, Implicit_Midpoint=3, Millstein=6, Heun_Strong=7,
+              Heun_NotConsistent=8)
 FIELD = dict(none=1, constant=2, sinusoidal=3, lissajous=4, triangular=5, rectangular=6, sinc=7)
 ANISO = dict(uniaxial=1, cubic=2, uniaxialcubic=4)
 PREC = dict(fp64=0, mixed=1)

@@ -153,7 +153,9 @@ __device__ __forceinline__ V3<R> aniso_field(const AnisoLane<R>& A, V3<R> m) {
     return (A.ku * dot(m, A.eu)) * A.eu + hc;
 }
 
-enum { S_EM = 1, S_IM = 3, S_MIL = 6, S_HEUN = 7 };
+enum { S_EM = 1, S_IM = 3, S_MIL = 6, S_HEUN = 7, S_HNC = 8 };
+// normals per step: dim_W, twice that for Heun_NotConsistent (dW and dW2, Heun_NotConsistent.inl:37-38)
+template <int SOLVER, int DW> constexpr int noise_per_step() { return SOLVER == S_HNC ? 2 * DW : DW; }
 
 // ================================================================ Neel (Cartesian)
 // Problems/NeelRelaxation.h: det = a m x H - b m x (m x H) (:117-128);
@@ -191,7 +193,8 @@ __device__ __forceinline__ V3<R> neel_s(const NeelLane<ANISO, R>& L, V3<R> m, V3
 }
 
 template <int ANISO, int SOLVER, class R> struct NeelCfg {
-    static constexpr int DY = 3, DW = 3, DOUT = 3, G = 4, NORM = 1;   // NORM: 3-blocks that normalize() rescales
+    static constexpr int DY = 3, DW = 3, DOUT = 3, NORM = 1;   // NORM: 3-blocks that normalize() rescales
+    static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = NeelLane<ANISO, R>;
     struct Cache {};
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { neel_load<ANISO, R>(rq, p, L); }
@@ -211,6 +214,12 @@ template <int ANISO, int SOLVER, class R> struct NeelCfg {
             const V3<R> sup = m + f0 + s0;
             const V3<R> f1 = neel_f<ANISO>(L, sup, h);
             inc = (R)0.5 * (f1 + f0) + s0;
+        } else if (SOLVER == S_HNC) {         // Heun_NotConsistent.inl:34-45: second stage with S at both points on dW2
+            const V3<R> z2 = {z[3], z[4], z[5]};
+            const V3<R> f0 = neel_f<ANISO>(L, m, h);
+            const V3<R> sup = m + f0 + neel_s<ANISO>(L, m, zz);
+            const V3<R> f1 = neel_f<ANISO>(L, sup, h);
+            inc = (R)0.5 * (f1 + f0) + (R)0.5 * (neel_s<ANISO>(L, sup, z2) + neel_s<ANISO>(L, m, z2));
         } else {                              // Millstein.inl:34-44: y + a dt + S dW + drift (dW.dW)
             const R zz2 = dot(zz, zz);
             inc = neel_det<ANISO>(L, m, h) + neel_s<ANISO>(L, m, zz) + (L.d * zz2) * m;
@@ -277,7 +286,8 @@ __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
 }
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
-    static constexpr int DY = 6, DW = 6, DOUT = 6, G = 2, NORM = 2;
+    static constexpr int DY = 6, DW = 6, DOUT = 6, NORM = 2;
+    static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNLane<ANISO, R>;
     struct Cache {};
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { bn_load<ANISO, R>(rq, p, L); }
@@ -303,6 +313,17 @@ template <int ANISO, int SOLVER, class R> struct BNCfg {
             bn_f<ANISO>(L, n + fn0 + sn, m + fm0 + sm, h, fn1, fm1, true);
             in = (R)0.5 * (fn1 + fn0) + sn;
             im = (R)0.5 * (fm1 + fm0) + sm;
+        } else if (SOLVER == S_HNC) {         // Heun_NotConsistent.inl:34-45
+            const V3<R> zF2 = {z[6], z[7], z[8]}, zH2 = {z[9], z[10], z[11]};
+            V3<R> fn0, fm0, sn, sm, fn1, fm1, sn1, sm1, sn0, sm0;
+            bn_f<ANISO>(L, n, m, h, fn0, fm0, true);
+            bn_s<ANISO>(L, n, m, zF, zH, sn, sm);
+            const V3<R> ns = n + fn0 + sn, ms = m + fm0 + sm;
+            bn_f<ANISO>(L, ns, ms, h, fn1, fm1, true);
+            bn_s<ANISO>(L, ns, ms, zF2, zH2, sn1, sm1);
+            bn_s<ANISO>(L, n, m, zF2, zH2, sn0, sm0);
+            in = (R)0.5 * (fn1 + fn0) + (R)0.5 * (sn1 + sn0);
+            im = (R)0.5 * (fm1 + fm0) + (R)0.5 * (sm1 + sm0);
         } else {   // Millstein: y + det dt + S dW + drift (dW.dW)
             V3<R> fn, fm, sn, sm, dn, dm;
             bn_f<ANISO>(L, n, m, h, fn, fm, false);
@@ -411,7 +432,8 @@ __device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Tri
 }
 
 template <int ANISO, int SOLVER, class R> struct SphCfg {
-    static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4, NORM = 0;
+    static constexpr int DY = 2, DW = 3, DOUT = 3, NORM = 0;
+    static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = SphLane<ANISO, R>;
     using Cache = Trig<R>;          // trig of the current (unrotated) state
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) {
@@ -468,7 +490,7 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         if (SOLVER == S_EM) {                         // EulerMaruyama.inl:41-72
             r[0] = Y[0] + (double)((det0.x + c0.drift0) * L.dt + s0);
             r[1] = Y[1] + (double)(det0.y * L.dt + s1);
-        } else if (SOLVER == S_HEUN) {                // chart of y_i kept for both stages
+        } else if (SOLVER == S_HEUN || SOLVER == S_HNC) {   // chart of y_i kept for both stages
             const R f00 = (det0.x + c0.drift0) * L.dt, f01 = det0.y * L.dt;
             const double sup0 = Y[0] + (double)(f00 + s0), sup1 = Y[1] + (double)(f01 + s1);
             Trig<R> T1;
@@ -477,8 +499,15 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
             Chart<R> c1; V3<R> det1;
             make_chart<ANISO>(L, T1, rot, h, c1, det1);
             const R f10 = (det1.x + c1.drift0) * L.dt, f11 = det1.y * L.dt;
-            r[0] = Y[0] + (double)((R)0.5 * (f10 + f00) + s0);
-            r[1] = Y[1] + (double)((R)0.5 * (f11 + f01) + s1);
+            if (SOLVER == S_HEUN) {
+                r[0] = Y[0] + (double)((R)0.5 * (f10 + f00) + s0);
+                r[1] = Y[1] + (double)((R)0.5 * (f11 + f01) + s1);
+            } else {                                  // Heun_NotConsistent.inl:34-45: S at both points on dW2
+                const V3<R> z2 = {z[3], z[4], z[5]};
+                const R t0 = L.sg * (dot(c1.P0, z2) + dot(c0.P0, z2)), t1 = L.sg * (dot(c1.P1, z2) + dot(c0.P1, z2));
+                r[0] = Y[0] + (double)((R)0.5 * (f10 + f00) + (R)0.5 * t0);
+                r[1] = Y[1] + (double)((R)0.5 * (f11 + f01) + (R)0.5 * t1);
+            }
         } else {                                      // Millstein: y + det dt + S dW + drift (dW.dW)
             const R dWdW = L.dt * dot(zz, zz);
             r[0] = Y[0] + (double)(det0.x * L.dt + s0 + c0.drift0 * dWdW);
@@ -717,7 +746,7 @@ template <int ANISO> struct IMSph {
 };
 
 template <class Prob> struct IMCfg {
-    static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G, NORM = Prob::NORM;
+    static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G, NORM = Prob::NORM, NZ = DW;
     using Lane = Prob;
     struct Cache { unsigned fails; };
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { L.load(rq, p); }
@@ -771,7 +800,7 @@ template <bool B> struct BoolTag { static constexpr bool value = B; };
 // (MeanSimulationResult::operator+=, :87-119, for the wave's 64 particles).
 template <class Cfg, class R, bool KAHAN, bool MIDPOINT>
 __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& tw) {
-    constexpr int DY = Cfg::DY, DW = Cfg::DW, DOUT = Cfg::DOUT, G = Cfg::G;
+    constexpr int DY = Cfg::DY, NZ = Cfg::NZ, DOUT = Cfg::DOUT, G = Cfg::G;   // NZ normals per step, 12 per group
     const int lane = threadIdx.x & 63;
     const uint64_t N = rq.n_particles;
     const uint64_t pr = (uint64_t)tw.tile * 64 + lane;
@@ -854,17 +883,17 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
         constexpr bool RN = decltype(renorm_tag)::value;
         const uint64_t seed = rq.seed;
         R zc[12], zn[12];
-        if (tw.step_begin < tw.step_end) NormalsT<R>::philox(seed, p, (tw.step_begin * DW) >> 2, zc);
+        if (tw.step_begin < tw.step_end) NormalsT<R>::philox(seed, p, (tw.step_begin * NZ) >> 2, zc);
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             fc.ensure<MIDPOINT>(rq, n0, G);
-            NormalsT<R>::philox(seed, p, ((n0 + G) * DW) >> 2, zn);
+            NormalsT<R>::philox(seed, p, ((n0 + G) * NZ) >> 2, zn);
             // the G outputs of the group are added first and enter the window's compensated sum as one term
             // (the reference compensates every step, SingleParticleSimulator.h:148-153; the two differ by the
             // rounding of G - 1 plain additions)
             double gsum[DOUT];
 #pragma unroll
             for (int s = 0; s < G; ++s) {
-                Cfg::step(rq, L, y, cache, zc + s * DW, fc.at(n0 + s));
+                Cfg::step(rq, L, y, cache, zc + s * NZ, fc.at(n0 + s));
                 if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
                 double o[DOUT];
                 Cfg::output(L, y, cache, o);
@@ -884,13 +913,13 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     } else {
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             R z[12];
-            normals12<R>(rq, p, n0 * DW, z);
+            normals12<R>(rq, p, n0 * NZ, z);
             fc.ensure<MIDPOINT>(rq, n0, G);
 #pragma unroll
             for (int s = 0; s < G; ++s) {
                 const uint64_t n = n0 + s;
                 if (n < tw.step_end) {
-                    Cfg::step(rq, L, y, cache, z + s * DW, fc.at(n));
+                    Cfg::step(rq, L, y, cache, z + s * NZ, fc.at(n));
                     if (renorm) normalize_blocks<Cfg::NORM>(y);
                     double o[DOUT];
                     Cfg::output(L, y, cache, o);

@@ -260,7 +260,8 @@ __device__ __forceinline__ void rotated_to_neel(const double* y, double* r) {
 }
 
 template <int ANISO, int SOLVER, class R> struct BNESCfg {
-    static constexpr int DY = 5, DW = 6, DOUT = 9, G = 2, NORM = 0;
+    static constexpr int DY = 5, DW = 6, DOUT = 9, NORM = 0;
+    static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNESLane<ANISO, R>;
     using Cache = Trig5<R>;      // trig of the current state (both charts unrotated)
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { bnes_load<ANISO, R>(rq, p, L); }
@@ -312,7 +313,7 @@ template <int ANISO, int SOLVER, class R> struct BNESCfg {
             bnes_rates(L, c0, T0, brot, nrot, h, true, f0);
 #pragma unroll
             for (int i = 0; i < 5; ++i) r[i] = Y[i] + (double)(f0[i] * L.dt + s0[i]);
-        } else if (SOLVER == S_HEUN) {                              // charts of y_i kept for both stages
+        } else if (SOLVER == S_HEUN || SOLVER == S_HNC) {           // charts of y_i kept for both stages
             bnes_rates(L, c0, T0, brot, nrot, h, true, f0);
             double sup[5];
 #pragma unroll
@@ -324,8 +325,17 @@ template <int ANISO, int SOLVER, class R> struct BNESCfg {
             bnes_chart(T1, brot, nrot, c1);
             R f1[5];
             bnes_rates(L, c1, T1, brot, nrot, h, true, f1);
+            if (SOLVER == S_HEUN) {
 #pragma unroll
-            for (int i = 0; i < 5; ++i) r[i] = Y[i] + (double)((R)0.5 * (f1[i] * L.dt + f0[i]) + s0[i]);
+                for (int i = 0; i < 5; ++i) r[i] = Y[i] + (double)((R)0.5 * (f1[i] * L.dt + f0[i]) + s0[i]);
+            } else {                                                // Heun_NotConsistent.inl:34-45: S at both points on dW2
+                const V3<R> zF2 = {z[6], z[7], z[8]}, zH2 = {z[9], z[10], z[11]};
+                R t1[5], t0[5];
+                bnes_noise(L, c1, zF2, zH2, t1);
+                bnes_noise(L, c0, zF2, zH2, t0);
+#pragma unroll
+                for (int i = 0; i < 5; ++i) r[i] = Y[i] + (double)((R)0.5 * (f1[i] * L.dt + f0[i]) + (R)0.5 * (t1[i] + t0[i]));
+            }
         } else {                                                    // Millstein: y + det dt + S dW + drift (dW.dW)
             bnes_rates(L, c0, T0, brot, nrot, h, false, f0);
             R dr[4];

@@ -51,12 +51,18 @@ struct HostRequest {
 inline uint64_t gcd64(uint64_t a, uint64_t b) { while (b) { const uint64_t t = a % b; a = b; b = t; } return a; }
 inline uint64_t lcm64(uint64_t a, uint64_t b) { return a / gcd64(a, b) * b; }
 
+// normals drawn per step: dim_W, or dW and dW2 for Heun_NotConsistent (Heun_NotConsistent.inl:37-38)
+inline uint32_t normals_per_step(const sp_request& rq, uint32_t dw) {
+    return rq.solver == SP_SOLVER_HEUN_NOT_CONSISTENT ? 2 * dw : dw;
+}
+
 inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (!rq || !pt) return set_error(SP_ERR_INVALID_ARG, "request or particles is NULL");
     uint32_t dy, dw, dout;
     if (request_dims(rq->problem, &dy, &dw, &dout)) return set_error(SP_ERR_UNSUPPORTED, "unknown problem type");
     if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_EULER_MARUYAMA_NORMALIZED &&
-        rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT)
+        rq->solver != SP_SOLVER_HEUN_STRONG && rq->solver != SP_SOLVER_MILLSTEIN && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT &&
+        rq->solver != SP_SOLVER_HEUN_NOT_CONSISTENT)
         return set_error(SP_ERR_UNSUPPORTED, "solver not available on the GPU backend");
     if (rq->field < SP_FIELD_ZERO || rq->field > SP_FIELD_SINC)
         return set_error(SP_ERR_UNSUPPORTED, "field type not available on the GPU backend");
@@ -97,7 +103,7 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (rq->solver == SP_SOLVER_IMPLICIT_MIDPOINT && (rq->newton_max_iter == 0 || !(rq->newton_tol > 0.0)))
         return set_error(SP_ERR_INVALID_ARG, "Implicit_Midpoint needs Max_Iterations > 0 and Accuracy_Goal > 0");
     const uint64_t total = (rq->n_steps / rq->oversampling - 1) * rq->oversampling;
-    if (rq->noise == SP_NOISE_REPLAY && (!rq->replay_normals || rq->replay_count < total * dw))
+    if (rq->noise == SP_NOISE_REPLAY && (!rq->replay_normals || rq->replay_count < total * normals_per_step(*rq, dw)))
         return set_error(SP_ERR_INVALID_ARG, "replay needs n_particles*steps*dim_W normals");
     return SP_OK;
 }
@@ -123,7 +129,8 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.n_out = rq->n_steps / rq->oversampling;
     hr.total_steps = (hr.n_out - 1) * rq->oversampling;
     hr.n_tiles = (uint32_t)((hr.N + 63) / 64);
-    hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT;
+    hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT &&
+               rq->solver != SP_SOLVER_HEUN_NOT_CONSISTENT;     // both run in FP64 only
     hr.kind = kind_for(*rq, hr.mixed);
     hr.params.assign((size_t)NPARAM * hr.N, 0.0);
     compute_params(*rq, *pt, hr.params.data(), hr.N);
@@ -131,7 +138,7 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     for (uint64_t p = 0; p < hr.N; ++p)
         for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
     if (rq->noise == SP_NOISE_REPLAY) {
-        const uint64_t K = hr.total_steps * hr.dw;
+        const uint64_t K = hr.total_steps * normals_per_step(*rq, hr.dw);
         hr.replay.assign(rq->replay_normals, rq->replay_normals + K * hr.N);
         hr.rq.replay_count = K;
     }

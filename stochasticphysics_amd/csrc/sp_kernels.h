@@ -33,9 +33,15 @@ bool info_bn(uint32_t, KernelInfo*);
 bool launch_plain_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool info_uc(uint32_t, KernelInfo*);
-bool launch_plain_bnes(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
-bool launch_batch_bnes(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
-bool info_bnes(uint32_t, KernelInfo*);
+#define SP_BNES_DECL(A)                                                                                               \
+    bool launch_plain_bnes_a##A(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t); \
+    bool launch_batch_bnes_a##A(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t,   \
+                                uint32_t*);                                                                               \
+    bool info_bnes_a##A(uint32_t, KernelInfo*);
+SP_BNES_DECL(1)
+SP_BNES_DECL(2)
+SP_BNES_DECL(4)
+#undef SP_BNES_DECL
 
 // deterministic second-stage reduction; one block per (entry, row)
 void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows);

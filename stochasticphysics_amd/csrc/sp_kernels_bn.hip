@@ -1,5 +1,6 @@
 // sp_kernels_bn.hip -- kernel classes of problem 3 (BNCfg): EulerMaruyama, Heun_Strong and
-// Millstein in FP64 and mixed precision, ImplicitMidpoint in FP64, uniaxial and cubic anisotropy.
+// Millstein in FP64 and mixed precision, Heun_NotConsistent and ImplicitMidpoint in FP64, uniaxial and
+// cubic anisotropy.
 #include "sp_kernels_impl.h"
 
 namespace sp {
@@ -17,6 +18,8 @@ namespace sp {
     X(kernel_kind(3, S_HEUN, 2, 1), BNCfg<2 SP_COMMA S_HEUN SP_COMMA float>, float, false, false)     \
     X(kernel_kind(3, S_MIL, 1, 1), BNCfg<1 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
     X(kernel_kind(3, S_MIL, 2, 1), BNCfg<2 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
+    X(kernel_kind(3, S_HNC, 1, 0), BNCfg<1 SP_COMMA S_HNC SP_COMMA double>, double, true, false)      \
+    X(kernel_kind(3, S_HNC, 2, 0), BNCfg<2 SP_COMMA S_HNC SP_COMMA double>, double, true, false)      \
     X(kernel_kind(3, S_IM, 1, 0), IMCfg<IMBN<1>>, double, true, true)                                \
     X(kernel_kind(3, S_IM, 2, 0), IMCfg<IMBN<2>>, double, true, true)
 #define SP_COMMA ,

@@ -10,7 +10,13 @@ bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
         case 1: return launch_plain_neel(kind, grid, block, s, rq, sb, se, partial, pstride);
         case 2: return launch_plain_sph(kind, grid, block, s, rq, sb, se, partial, pstride);
         case 3: return launch_plain_bn(kind, grid, block, s, rq, sb, se, partial, pstride);
-        case 4: return launch_plain_bnes(kind, grid, block, s, rq, sb, se, partial, pstride);
+        case 4:
+            switch ((kind >> 1) & 3u) {   // anisotropy slot
+                case 0: return launch_plain_bnes_a1(kind, grid, block, s, rq, sb, se, partial, pstride);
+                case 1: return launch_plain_bnes_a2(kind, grid, block, s, rq, sb, se, partial, pstride);
+                case 2: return launch_plain_bnes_a4(kind, grid, block, s, rq, sb, se, partial, pstride);
+            }
+            return false;
     }
     return false;
 }
@@ -21,7 +27,13 @@ bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
         case 1: return launch_batch_neel(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
         case 2: return launch_batch_sph(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
         case 3: return launch_batch_bn(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
-        case 4: return launch_batch_bnes(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+        case 4:
+            switch ((kind >> 1) & 3u) {
+                case 0: return launch_batch_bnes_a1(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+                case 1: return launch_batch_bnes_a2(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+                case 2: return launch_batch_bnes_a4(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);
+            }
+            return false;
     }
     return false;
 }
@@ -31,7 +43,13 @@ bool kernel_info(uint32_t kind, KernelInfo* info) {
         case 1: return info_neel(kind, info);
         case 2: return info_sph(kind, info);
         case 3: return info_bn(kind, info);
-        case 4: return info_bnes(kind, info);
+        case 4:
+            switch ((kind >> 1) & 3u) {
+                case 0: return info_bnes_a1(kind, info);
+                case 1: return info_bnes_a2(kind, info);
+                case 2: return info_bnes_a4(kind, info);
+            }
+            return false;
     }
     return false;
 }

@@ -1,5 +1,6 @@
 // sp_kernels_neel.hip -- kernel classes of problem 1 (NeelCfg): EulerMaruyama, Heun_Strong and
-// Millstein in FP64 and mixed precision, ImplicitMidpoint in FP64, uniaxial and cubic anisotropy.
+// Millstein in FP64 and mixed precision, Heun_NotConsistent and ImplicitMidpoint in FP64, uniaxial and
+// cubic anisotropy.
 #include "sp_kernels_impl.h"
 
 namespace sp {
@@ -17,6 +18,8 @@ namespace sp {
     X(kernel_kind(1, S_HEUN, 2, 1), NeelCfg<2 SP_COMMA S_HEUN SP_COMMA float>, float, false, false)     \
     X(kernel_kind(1, S_MIL, 1, 1), NeelCfg<1 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
     X(kernel_kind(1, S_MIL, 2, 1), NeelCfg<2 SP_COMMA S_MIL SP_COMMA float>, float, false, false)       \
+    X(kernel_kind(1, S_HNC, 1, 0), NeelCfg<1 SP_COMMA S_HNC SP_COMMA double>, double, true, false)      \
+    X(kernel_kind(1, S_HNC, 2, 0), NeelCfg<2 SP_COMMA S_HNC SP_COMMA double>, double, true, false)      \
     X(kernel_kind(1, S_IM, 1, 0), IMCfg<IMNeel<1>>, double, true, true)                                \
     X(kernel_kind(1, S_IM, 2, 0), IMCfg<IMNeel<2>>, double, true, true)
 #define SP_COMMA ,

@@ -194,5 +194,7 @@ def test_request_validation_errors():
     assert (n_out.value, dy.value, dw.value, do.value) == (10, 3, 3, 3)
     out = np.zeros(capi.NPARAM * 8)
     n_par = C.c_uint32()
-    r.solver = 8      # Heun_NotConsistent: not built for the GPU backend
+    r.solver = 9      # WeakTest: not built for the GPU backend
     assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), C.byref(n_par)) == 2
+    r.solver = 8      # Heun_NotConsistent
+    assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), C.byref(n_par)) == 0

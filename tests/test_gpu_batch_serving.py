@@ -12,7 +12,7 @@ import pytest
 from stochasticphysics_amd import Context, configs
 from stochasticphysics_amd.capi import SP_OK, SpError
 
-from test_gpu_parity import bn_ensemble, neel_ensemble, run_oracle, sph_ensemble
+from test_gpu_parity import bn_ensemble, bnes_ensemble, neel_ensemble, run_oracle, sph_ensemble
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +29,9 @@ def mixed_bag():
             neel_ensemble(n=70, n_steps=5000, solver="Heun_Strong", aniso="cubic", seed=22),
             bn_ensemble(n=130, n_steps=1500, solver="Heun_Strong", seed=23),
             sph_ensemble(n=150, n_steps=2500, solver="Millstein", seed=24),
-            sph_ensemble(n=64, n_steps=1000, solver="Implicit_Midpoint", seed=25)]
+            sph_ensemble(n=64, n_steps=1000, solver="Implicit_Midpoint", seed=25),
+            bn_ensemble(n=90, n_steps=1200, solver="Heun_NotConsistent", seed=26),
+            bnes_ensemble(n=100, n_steps=1500, solver="Millstein", aniso="cubic", seed=27)]
 
 
 def same(a, b):
@@ -102,7 +104,7 @@ def test_server_alone_vs_mix(ctx):
     rid = ctx.submit(target)
     assert ctx.wait(rid, 120_000) == SP_OK
     alone = ctx.result(rid, target)
-    ids = [ctx.submit(e) for e in bag]        # the target again, now among four other requests
+    ids = [ctx.submit(e) for e in bag]        # the target again, now among six other requests
     outs = []
     for rid, e in zip(ids, bag):
         assert ctx.wait(rid, 120_000) == SP_OK
@@ -112,7 +114,7 @@ def test_server_alone_vs_mix(ctx):
     for p, o in zip(plain, outs):
         same(p, o)
     st = ctx.server_stats()
-    assert st["completed"] >= 6 and st["iterations"] >= 1
+    assert st["completed"] >= 8 and st["iterations"] >= 1
 
 
 def test_server_admission_by_memory(ctx):

@@ -29,7 +29,8 @@ def ctx():
 
 def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False):
     return oracle.simulate(e.pid, {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint": 3, "Millstein": 6,
-                                   "Heun_Strong": 7}[e.solver],
+                                   "Heun_Strong": 7,
+                                   "Heun_NotConsistent": 8}[e.solver],
                            {"uniaxial": 1, "cubic": 2, "uniaxialcubic": 4}[e.anisotropy], e.n_steps, e.oversampling,
                            e.dt, e.seed,
                            e.oracle_props(), e.y0, axes=e.axes, field=e.field_dict(), bn_simple=e.bn_simple,
@@ -297,6 +298,53 @@ def test_implicit_midpoint_parity(ctx, problem):
     assert g["newton_failures"] == o["newton_failures"]
     compare(g, o, 1e-10, spherical=problem == "NeelSpherical")
 
+
+
+def hnc_replay(e: Ensemble, seed=99):
+    """Replayed noise for Heun_NotConsistent: two dim_W blocks of normals per step (dW1, then dW2)."""
+    rng = np.random.default_rng(seed)
+    e.replay = rng.standard_normal((e.total_steps * 2 * DIM_W[e.pid], e.n))
+    return e
+
+
+HNC = "Heun_NotConsistent"
+HNC_CASES = {"neel_uniaxial": lambda: neel_ensemble(solver=HNC, field="sinusoidal", n=130),
+             "neel_cubic_replay": lambda: hnc_replay(neel_ensemble(solver=HNC, aniso="cubic", field="lissajous")),
+             "spherical_cubic": lambda: sph_ensemble(solver=HNC),
+             "spherical_uniaxial": lambda: sph_ensemble(solver=HNC, aniso="uniaxial"),
+             "brown_and_neel": lambda: bn_ensemble(solver=HNC),
+             "brown_and_neel_simple": lambda: bn_ensemble(solver=HNC, simple=True),
+             "brown_and_neel_replay": lambda: hnc_replay(bn_ensemble(solver=HNC, n=70)),
+             "euler_spherical_uniaxial": lambda: bnes_ensemble(solver=HNC),
+             "euler_spherical_uniaxialcubic": lambda: bnes_ensemble(solver=HNC, aniso="uniaxialcubic"),
+             "uniaxialcubic_neel": lambda: configs.uniaxialcubic(n=130, problem="Neel", solver=HNC, n_steps=2000,
+                                                                 oversampling=10),
+             "uniaxialcubic_spherical": lambda: configs.uniaxialcubic(n=96, problem="NeelSpherical", solver=HNC,
+                                                                      n_steps=2000, oversampling=10)}
+
+
+@pytest.mark.parametrize("case", list(HNC_CASES))
+def test_heun_not_consistent_parity(ctx, case):
+    """Heun_NotConsistent (Heun_NotConsistent.inl:34-45): predictor on dW1, corrector with the stochastic matrix
+    at both points on an independent dW2; every problem class, Philox and replayed noise.  An FMA-contracted oracle
+    build differs from the plain one by at most 1.1e-11 (final state) and 1.6e-13 (mean) on these cases."""
+    e = HNC_CASES[case]()
+    g, o = ctx.run(e, keep_final=True), run_oracle(e)
+    if e.problem == "BrownAndNeelEulerSpherical":
+        compare_bnes(g, o, 1e-10)
+    else:
+        compare(g, o, 1e-10, spherical=e.problem == "NeelSpherical")
+
+
+def test_heun_not_consistent_mixed_request_runs_fp64(ctx):
+    """Heun_NotConsistent has no mixed-precision kernels: a mixed request runs the FP64 ones and equals the FP64
+    run bit for bit."""
+    e = neel_ensemble(solver=HNC, field="sinusoidal", n=130)
+    a = ctx.run(e, keep_final=True)
+    e.precision = "mixed"
+    b = ctx.run(e, keep_final=True)
+    assert b["precision_used"] == 0 and not b["used_fallback"]
+    assert np.array_equal(a["mean"], b["mean"]) and np.array_equal(a["final_state"], b["final_state"])
 
 UC_CASES = {"neel_em": ("Neel", "EulerMaruyama", 130, 2000), "neel_heun": ("Neel", "Heun_Strong", 130, 2000),
             "neel_millstein": ("Neel", "Millstein", 130, 2000), "neel_im": ("Neel", "Implicit_Midpoint", 64, 500),
