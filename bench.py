@@ -15,7 +15,10 @@ vector-ALU roofline of its kernel, a CPU baseline on host cores, and
     precision (with its FP64 check), and the <m> error of mixed against FP64;
   * "serving": Poisson arrivals of spherical-cubic requests (Implicit_Midpoint
     for damping >= 0.3) through the continuous-batching server, p50/p99
-    end-to-end latency and throughput.
+    end-to-end latency and throughput;
+  * "problems": FP64 and mixed throughput of every problem class;
+  * "provider": the particles of a parameter file drawn on the device vs
+    materialised on the host and uploaded.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S]
 """
@@ -263,6 +266,106 @@ def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
                 unit="particle-steps/s", results=out)
 
 
+PROVIDER_PARTICLE = """[Particle_Properties]
+Temperature = 300
+[Particle_Properties.Magnetic_Properties]
+Magnetic_radius = {radius}
+Saturation_magnetisation = 4.8e5
+Damping_constant = 0.2
+Gyromagnetic_ratio = 1.76e11
+Type_of_anisotropy = 'uniaxial'
+[Particle_Properties.Magnetic_Properties.Anisotropy_Properties]
+K_uniaxial = 5e3
+[Particle_Properties.Hydrodynamic_Properties]
+Hydrodynamic_Radius = {r_h}
+Viscosity = 1.0e-03
+[Particle_Simulation_Initialization]
+Use_random_initial_particle_orientation = TRUE
+Use_random_initial_particle_magnetisation = TRUE
+[Particle_Simulation_Settings]
+Magnetic_radius_distribution_type = 'lognormal'
+Magnetic_radius_distribution_width = 0.1
+Use_relative_distribution_width_for_magnetic_radius = TRUE
+Hydrodynamic_shell_distribution_type = 'normal'
+Hydrodynamic_shell_distribution_width = 0.05
+Use_relative_distribution_width_for_hydrodynamic_shell = TRUE
+Distribution_type = 'normal'
+Anisotropy_constants_distribution_width = 0.05
+"""
+
+PROVIDER_RUN = """[Solver_Settings]
+Solver = 'Heun_Strong'
+[Simulation_Settings]
+Timestep = 1e-11
+Number_of_steps = 200000
+Oversampling = 50
+Number_of_simulations = {n}
+Seed = 7
+[Field_Parameters]
+Type_of_field = 'sinusoidal'
+Offset_Field = {{0.0, 0.0, 0.0}}
+Amplitude = {{0.0, 0.0, 0.012}}
+Frequency = 1e5
+Phase = 0.0
+[Problem_Settings]
+Type_of_problem = 'BrownAndNeel'
+[Particle_Provider]
+Use_discrete_distribution_to_select_particle = TRUE
+[Particle_Provider.Particle_List]
+Small = 'small.ini'
+Large = 'large.ini'
+[Particle_Provider.Particle_Numbers]
+Small = 3
+Large = 1
+"""
+
+
+def provider_leg(ctx, n_device=1 << 24, n_host=1 << 20):
+    """Particles of a two-type BrownAndNeel parameter file (discrete type selection, lognormal radii, normal shells
+    and K, random orientations and magnetisations): drawn by the provider kernel straight into HBM
+    (sp_settings_prepare) vs materialised on the host, run through the parameter calculators and uploaded
+    (sp_settings_request + sp_prepare), the path a host-side provider takes."""
+    import ctypes as C
+    import tempfile
+    from stochasticphysics_amd import Settings, capi
+    from stochasticphysics_amd.capi import Particles, Request, lib
+    with tempfile.TemporaryDirectory() as d:
+        d = Path(d)
+        (d / "small.ini").write_text(PROVIDER_PARTICLE.format(radius=1.0e-8, r_h=2.2e-8))
+        (d / "large.ini").write_text(PROVIDER_PARTICLE.format(radius=1.4e-8, r_h=2.5e-8))
+        (d / "dev.ini").write_text(PROVIDER_RUN.format(n=n_device))
+        (d / "host.ini").write_text(PROVIDER_RUN.format(n=n_host))
+        s = Settings(d / "dev.ini")
+        ctx.prepare_settings(s).release()                     # first use: allocation pools, code objects
+        t0 = time.perf_counter()
+        h = ctx.prepare_settings(s)
+        dev_s = time.perf_counter() - t0
+        h.release()
+        t0 = time.perf_counter()
+        hs = C.c_void_p()
+        capi.check(lib().sp_settings_load(str(d / "host.ini").encode(), C.byref(hs)))
+        r, p = Request(), Particles()
+        capi.check(lib().sp_settings_request(hs, C.byref(r), C.byref(p)))
+        t1 = time.perf_counter()
+        hp = C.c_void_p()
+        capi.check(lib().sp_prepare(ctx._ctx, C.byref(r), C.byref(p), C.byref(hp)))
+        host_s = time.perf_counter() - t0
+        lib().sp_release(hp)
+        lib().sp_settings_free(hs)
+    bytes_per_particle = (capi.NPARAM + 6) * 8
+    out = dict(workload="BrownAndNeel parameter file, two particle types (discrete selection), lognormal radius, normal "
+                        "shell and K, random orientation and magnetisation",
+               device_particles=n_device, device_prepare_ms=1e3 * dev_s, device_particles_per_s=n_device / dev_s,
+               host_particles=n_host, host_prepare_ms=1e3 * host_s, host_draw_ms=1e3 * (t1 - t0),
+               host_particles_per_s=n_host / host_s, speedup=(n_device / dev_s) / (n_host / host_s),
+               bytes_written_per_particle=bytes_per_particle,
+               note="device: parameter block and start state drawn into HBM (alloc + kernel + per-group sums); "
+                    "host: .ini draw, parameter calculators, upload")
+    log(f"bench: provider: device {out['device_particles_per_s']:.3e} particles/s ({n_device} in "
+        f"{out['device_prepare_ms']:.1f} ms), host {out['host_particles_per_s']:.3e} particles/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -275,11 +378,13 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-sets", type=int, default=48, help="uneven sets in the mixed/batched leg")
-    ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems"), default="headline",
+    ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems", "provider"), default="headline",
                     help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation; "
-                         "problems: every problem class on a 65,536-particle ensemble (each prints its own JSON line "
-                         "instead of the headline one)")
+                         "problems: every problem class on a 65,536-particle ensemble; provider: particles of a "
+                         "parameter file drawn on the device vs on the host (each prints its own JSON line instead "
+                         "of the headline one)")
     ap.add_argument("--no-problems", action="store_true", help="headline run without the per-problem leg")
+    ap.add_argument("--no-provider", action="store_true", help="headline run without the particle-provider leg")
     ap.add_argument("--particles", type=int, default=1_048_576, help="precision study: particles")
     ap.add_argument("--study-steps", type=int, default=500_000, help="precision study: steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -309,6 +414,8 @@ def main():
             res = precision_study(ctx, configs, args.particles, args.study_steps)
         elif args.workload == "problems":
             res = problem_rates(ctx, configs)
+        elif args.workload == "provider":
+            res = provider_leg(ctx)
         else:
             res = neel_relaxation(ctx, configs)
         ctx.close()
@@ -358,6 +465,8 @@ def main():
         out["serving"] = serving_leg(ctx, configs, args.serve_seconds, args.serve_rate)
     if not args.no_problems and not args.quick:
         out["problems"] = problem_rates(ctx, configs)
+    if not args.no_provider:
+        out["provider"] = provider_leg(ctx, *((1 << 16, 1 << 12) if args.quick else ()))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(e)
         out["vs_cpu"] = value / out["cpu_baseline"]["value"]

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 5
+#define SP_ABI_VERSION 6
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -182,6 +182,10 @@ typedef struct sp_result {
     uint32_t used_fallback;    /* 1 if the mixed result failed the check and was recomputed in FP64 */
     uint32_t precision_used;   /* SP_PREC_* of the returned result */
     uint64_t newton_failures;  /* ImplicitMidpoint steps that hit Max_Iterations */
+    double particle_means[8];  /* means over the ensemble's particles of Temperature, Magnetic_radius,
+                                  Saturation_magnetisation, Damping_constant, Gyromagnetic_ratio,
+                                  Anisotropy_constant, Hydrodynamic_Radius, Viscosity: the Mean_Properties of
+                                  Mean_Results (MeanSimulationResult.h:131-141) */
 } sp_result;
 
 /* execution knobs chosen by the autotuner (or by hand) */
@@ -257,7 +261,8 @@ int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending,
  *      Provider/ParticleProvider.h:230-295) read into one request; CPU only ---- */
 typedef struct sp_settings sp_settings;
 int sp_settings_load(const char* ini_path, sp_settings** out);
-/* fills req and points parts at arrays owned by the settings object */
+/* fills req and, when parts is not NULL, points parts at per-particle arrays owned by the settings object
+ * (materialised on the host at the first such call) */
 int sp_settings_request(const sp_settings* s, sp_request* req, sp_particles* parts);
 const char* sp_settings_savefile(const sp_settings* s);
 /* writes Mean_Results {Time, Field, Results, Weight, Mean_Properties} as JSON
@@ -270,6 +275,21 @@ const char* sp_settings_single_savefile(const sp_settings* s);
  * SimulationResultManager::addSingleResult, Results/SimulationResultManager.h:115-149) */
 int sp_settings_write_single_results(const sp_settings* s, const sp_result* res, const char* path);
 void sp_settings_free(sp_settings* s);
+
+/* ---- device-side particle provider: the particles of a parameter file drawn on the GPU ----
+ * Replace ParticleProvider::getProvidedObject and ParticleSimulationSettings::applySettingsToParticleProperties
+ * (Provider/ParticleProvider.h:230-295, Settings/ParticleSimulationSettings.h:51-152) with the start states and
+ * parameter calculators after them, for ensembles too large to materialise on the host: one kernel writes
+ * every particle's parameters and initial state straight into HBM when the request's buffers are allocated.
+ * The particles are those sp_settings_request materialises on the host (same type selection, same Philox
+ * draws; values equal up to the last-bit rounding of exp, sin, cos, acos and atan2), and no per-particle array
+ * is built on the host.  sp_settings_prepare gives an sp_prepared for sp_execute; sp_settings_submit queues the
+ * ensemble on the request server like sp_submit. */
+int sp_settings_prepare(sp_ctx* ctx, const sp_settings* s, sp_prepared** out);
+int sp_settings_submit(sp_ctx* ctx, const sp_settings* s, uint64_t* id);
+/* copies a prepared request's resident inputs to the host: params [NPARAM = 23][n_particles] (param-major,
+ * the rows of sp_compute_particle_params), initial_state [n_particles][dim_y]; either may be NULL */
+int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_state);
 
 /* ---- autotuner: times candidate tunings on a short synthetic run ---- */
 int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts,

@@ -5,8 +5,9 @@
 // (Particle_Simulation/Simulator/SimulationManager.h:545-615): constructor from
 // the parameters, StartSimulationManager, waitUntilFinsihed (sic), isFinished,
 // abort and the static progress accessors.  The whole ensemble runs on one GPU
-// through libstophys_gpu.so; results are written like the reference's
-// Mean_Results when the manager finishes.  Errors are rethrown as
+// through libstophys_gpu.so, its particles drawn on the device by the provider
+// kernel (no per-particle array on the host); results are written like the
+// reference's Mean_Results when the manager finishes.  Errors are rethrown as
 // std::runtime_error, as the reference does.
 #pragma once
 #include <atomic>
@@ -26,7 +27,7 @@ public:
     explicit SimulationManagerGFX950(const Parameters& parameter_file, int device = 0, std::string result_file = {})
         : result_file_(std::move(result_file)) {
         check(sp_settings_load(parameter_file.c_str(), &settings_));
-        check(sp_settings_request(settings_, &req_, &parts_));
+        check(sp_settings_request(settings_, &req_, nullptr));
         if (result_file_.empty()) {
             const char* s = sp_settings_savefile(settings_);
             result_file_ = s && *s ? std::string(s) : std::string("Mean_Results");
@@ -46,7 +47,7 @@ public:
     // SimulationManager.h:549-574: starts the ensemble (asynchronously here)
     void StartSimulationManager() {
         if (started_) return;
-        check(sp_submit(ctx_, &req_, &parts_, &id_));
+        check(sp_settings_submit(ctx_, settings_, &id_));
         started_ = true;
     }
     // SimulationManager.h:577-584
@@ -121,7 +122,6 @@ private:
     sp_settings* settings_ = nullptr;
     sp_ctx* ctx_ = nullptr;
     sp_request req_{};
-    sp_particles parts_{};
     uint64_t id_ = 0;
     bool started_ = false, collected_ = false;
     std::string result_file_, single_file_;

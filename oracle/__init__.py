@@ -51,6 +51,8 @@ def lib(native=False):
         L.or_newton_testfn.argtypes = [C.c_int, d, C.c_double, C.c_int, d]
         L.or_newton_testfn.restype = C.c_int
         L.or_simulate.argtypes = [i64, C.POINTER(C.c_uint64), d, d, d, d, d, d, d, d, d, d, d, d]
+        L.or_provide.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, u64, u64, d, d, d, d, i64]
+        L.or_provide.restype = None
         L.or_simulate.restype = C.c_int
         _LIBS[name] = L
     return _LIBS[name]
@@ -210,3 +212,30 @@ def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0,
     if per_particle:
         res["per_particle"] = pp.reshape(N, n_out, dout)
     return res
+
+
+DIST_CODE = {"delta": 0, "normal": 1, "lognormal": 2}
+
+
+def provide(problem, aniso, types, seed, n, discrete=False):
+    """The particles of an ensemble from its particle types (or_provide).  types: list of dicts with count, T,
+    radius, Ms, alpha, gamma, K, r_h, viscosity and optionally K_cubic, euler, mag, urot, rand_orient, rand_mag,
+    rand_urot and distributions rad / shell / K / K_cubic given as (type, width, relative).  Returns props [n, 12]
+    (the layout of simulate), axes [n, 9], y0 [n, dim_y] and the type index of every particle."""
+    rows = []
+    for t in types:
+        row = [t["count"], t["T"], t["radius"], t["Ms"], t["alpha"], t["gamma"], t["K"], t.get("r_h", 0.0),
+               t.get("viscosity", 0.0), t.get("K_cubic", 0.0), *t.get("euler", (0.0, 0.0, 0.0)),
+               *t.get("mag", (0.0, 0.0, 1.0)), *t.get("urot", (0.0, 0.0, 0.0)), float(t.get("rand_orient", False)),
+               float(t.get("rand_mag", False)), float(t.get("rand_urot", False))]
+        for key in ("rad", "shell", "K_dist", "K_cubic_dist"):
+            kind, width, rel = t.get(key, ("delta", 0.0, True))
+            row += [DIST_CODE[kind], width, float(rel)]
+        rows.append(row)
+    tab = np.ascontiguousarray(rows, dtype=np.float64)
+    assert tab.shape[1] == 34
+    props, axes = np.zeros((n, 12)), np.zeros((n, 9))
+    y0, typ = np.zeros((n, DIM_Y[problem])), np.zeros(n, dtype=np.int64)
+    lib().or_provide(problem, aniso, len(types), int(discrete), seed, n, _p(tab), _p(props), _p(axes), _p(y0),
+                     _p(typ, C.c_int64))
+    return dict(props=props, axes=axes, y0=y0, type=typ)

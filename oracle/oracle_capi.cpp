@@ -90,6 +90,106 @@ void or_euler_axes(double phi, double theta, double psi, double* out) {
     for (int i = 0; i < 3; ++i) { out[i] = a.x[i]; out[3 + i] = a.y[i]; out[6 + i] = a.z[i]; }
 }
 
+// The particles of an ensemble from its particle types, restated from the engine's definition of its
+// provider (the reference draws them from std::random_device with MyCEL distributions, which cannot be
+// reproduced): draw k of particle i is Philox block (k, i) of stream tag 1 under the seed, a uniform is
+// u(w0) and a normal the first Box-Muller normal of the block.  Per particle, in this order: the type
+// (discrete: u * total count against the running counts; else the list order, each type for its count),
+// the magnetic radius, the hydrodynamic shell (r_h = radius + shell) and K from their distributions
+// (delta; normal mean + sd z; lognormal mean exp(s z - s^2 / 2) with s^2 = ln(1 + sd^2 / mean^2); sd is
+// width * mean when the width is relative); uniaxialcubic: K_cubic, then the three rotation angles
+// (U(0, 2 pi), U(0, pi), U(0, 2 pi)) when random; the Euler angles when the orientation is random
+// (U(0, 2 pi) each; theta U(0, pi) for BrownAndNeelEulerSpherical, ParticleStateInitializer.h:42-62); the
+// magnetisation when random (BrownAndNeelEulerSpherical: phi U(0, 2 pi), theta U(0, pi),
+// ParticleStateInitializer.h:107-130; otherwise three normals, normalised); BrownAndNeel with a random
+// orientation: three more normals for the axis n (BrownAndNeelRelaxation.inl:277-324).
+// types: [n_types][34] = count, T, radius, Ms, alpha, gamma, K, r_h, viscosity, K_cubic, euler(3), mag(3),
+// urot(3), random orientation, random magnetisation, random rotation, then four distributions (radius,
+// shell, K, K_cubic) of (type 0 delta / 1 normal / 2 lognormal, width, relative).
+// Outputs: props [N][12] (the layout of or_simulate), axes [N][9], y0 [N][dim_y], type [N].
+void or_provide(int problem, int aniso, int n_types, int discrete, uint64_t seed, uint64_t N, const double* types,
+                double* props, double* axes_out, double* y0, int64_t* type_out) {
+    const int TS = 34;
+    auto draw = [&](uint64_t i, uint64_t& k, bool normal) {
+        double z[4];
+        uint32_t c[4] = {(uint32_t)k, (uint32_t)(k >> 32), (uint32_t)i, 1u};
+        ++k;
+        if (normal) { normals4(seed, i, k - 1, 1u, z); return z[0]; }
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        return u01(c[0]);
+    };
+    auto sample = [&](const double* dist, double mean, uint64_t i, uint64_t& k) {
+        const int kind = (int)dist[0];
+        const double width = dist[1];
+        if (kind == 0 || width == 0.0) return mean;
+        const double sd = dist[2] != 0.0 ? width * mean : width;
+        if (kind == 1) return mean + sd * draw(i, k, true);
+        const double s2 = std::log1p(sd * sd / (mean * mean));
+        return mean * std::exp(std::sqrt(s2) * draw(i, k, true) - 0.5 * s2);
+    };
+    const bool bnes = problem == PROB_BNES;
+    const int dy = problem == PROB_NEEL ? 3 : (problem == PROB_SPHERICAL ? 2 : (bnes ? 5 : 6));
+    double total = 0.0;
+    for (int t = 0; t < n_types; ++t) total += types[t * TS];
+    for (uint64_t i = 0; i < N; ++i) {
+        uint64_t k = 0;
+        int sel = 0;
+        if (discrete && total > 0.0) {
+            const double u = draw(i, k, false) * total;
+            double run = 0.0;
+            for (sel = 0; sel < n_types - 1; ++sel) { run += types[sel * TS]; if (u < run) break; }
+        } else {
+            uint64_t run = 0;
+            for (sel = 0; sel < n_types - 1; ++sel) { run += (uint64_t)types[sel * TS]; if (i < run) break; }
+        }
+        const double* t = types + sel * TS;
+        type_out[i] = sel;
+        double* pr = props + 12 * i;
+        const double radius = sample(t + 22, t[2], i, k);
+        const double shell = sample(t + 25, t[7] - t[2], i, k);
+        pr[0] = radius; pr[1] = t[3]; pr[2] = t[4]; pr[3] = t[5]; pr[4] = t[1];
+        pr[5] = sample(t + 28, t[6], i, k);
+        pr[6] = radius + shell; pr[7] = t[8];
+        pr[8] = 0.0; pr[9] = pr[10] = pr[11] = 0.0;
+        if (aniso == 4) {
+            pr[8] = sample(t + 31, t[9], i, k);
+            for (int a = 0; a < 3; ++a) pr[9 + a] = t[21] != 0.0 ? draw(i, k, false) * (a == 1 ? PI : TWO_PI) : t[16 + a];
+        }
+        double e[3] = {t[10], t[11], t[12]};
+        if (t[19] != 0.0)
+            for (int a = 0; a < 3; ++a) e[a] = draw(i, k, false) * (bnes && a == 1 ? PI : TWO_PI);
+        or_euler_axes(e[0], e[1], e[2], axes_out + 9 * i);
+        double m[3] = {t[13], t[14], t[15]};
+        if (t[20] != 0.0 && bnes) {
+            const double ph = draw(i, k, false) * TWO_PI, th = draw(i, k, false) * PI;
+            m[0] = std::cos(ph) * std::sin(th); m[1] = std::sin(ph) * std::sin(th); m[2] = std::cos(th);
+        } else if (t[20] != 0.0) {
+            for (double& x : m) x = draw(i, k, true);
+        }
+        const double mn = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+        for (double& x : m) x /= mn;
+        double* y = y0 + (uint64_t)dy * i;
+        if (bnes) {
+            y[0] = e[0]; y[1] = e[1]; y[2] = e[2]; y[3] = std::acos(m[2]); y[4] = std::atan2(m[1], m[0]);
+        } else if (problem == PROB_NEEL) {
+            y[0] = m[0]; y[1] = m[1]; y[2] = m[2];
+        } else if (problem == PROB_SPHERICAL) {
+            y[0] = std::acos(m[2]); y[1] = std::atan2(m[1], m[0]);
+        } else {
+            double n[3];
+            if (t[19] != 0.0) {
+                for (double& x : n) x = draw(i, k, true);
+            } else {   // BrownAndNeelRelaxation.inl:297-307: the axis from the Euler angles
+                n[0] = std::cos(e[0]) * std::cos(e[2]) - std::sin(e[0]) * std::cos(e[1]) * std::sin(e[2]);
+                n[1] = -std::cos(e[0]) * std::sin(e[2]) - std::sin(e[0]) * std::cos(e[1]) * std::cos(e[2]);
+                n[2] = std::sin(e[0]) * std::sin(e[1]);
+            }
+            const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            for (int a = 0; a < 3; ++a) { y[a] = n[a] / nn; y[3 + a] = m[a]; }
+        }
+    }
+}
+
 // Evaluate one problem term at state y with field h (golden-vector checks).
 // what: 0 det, 1 drift, 2 sto (row-major), 3 Neel reference JacDet (dt), 4 Neel reference
 // JacSto (h = dW), 5 NeelSpherical prepare+finish round trip, 6 NeelSpherical isRotated

@@ -4,6 +4,6 @@ The compute path is libstophys_gpu.so (hand-written HIP kernels behind the C ABI
 include/stophys_gpu.h); this package binds it with ctypes and builds workloads.
 """
 from . import capi, configs
-from .engine import Context, Ensemble, compute_params, host_box_muller, host_normals
+from .engine import Context, Ensemble, Settings, compute_params, host_box_muller, host_normals
 
-__all__ = ["Context", "Ensemble", "capi", "configs", "compute_params", "host_box_muller", "host_normals"]
+__all__ = ["Context", "Ensemble", "Settings", "capi", "configs", "compute_params", "host_box_muller", "host_normals"]

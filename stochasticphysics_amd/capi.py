@@ -25,6 +25,8 @@ DIM_Y = {1: 3, 2: 2, 3: 6, 4: 5}
 DIM_W = {1: 3, 2: 3, 3: 6, 4: 6}
 DIM_OUT = {1: 3, 2: 3, 3: 6, 4: 9}
 NPARAM = 23
+# rows of the parameter block (sp_types.h ParamIndex)
+P_A, P_B, P_SIG, P_D, P_MU, P_K1, P_PT, P_C, P_CF, P_DD, P_AX, P_KU, P_U = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 19, 20
 
 SP_OK, SP_ERR_NOT_CONVERGED, SP_ERR_TIMEOUT, SP_ERR_NOT_READY = 0, 5, 7, 9
 
@@ -62,7 +64,7 @@ class Result(C.Structure):
                 ("device_ms", C.c_double), ("latency_ms", C.c_double), ("particle_steps", C.c_double),
                 ("check_error", C.c_double), ("check_noise", C.c_double), ("used_fallback", C.c_uint32),
                 ("precision_used", C.c_uint32),
-                ("newton_failures", C.c_uint64)]
+                ("newton_failures", C.c_uint64), ("particle_means", C.c_double * 8)]
 
 
 class Tuning(C.Structure):
@@ -77,7 +79,7 @@ EXPORTS = [
     "sp_result_get", "sp_cancel", "sp_server_stats", "sp_autotune", "sp_dev_philox", "sp_dev_normals", "sp_dev_box_muller", "sp_host_box_muller",
     "sp_host_normals", "sp_prepare", "sp_execute", "sp_release", "sp_settings_load", "sp_settings_request",
     "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free", "sp_settings_single_savefile",
-    "sp_settings_write_single_results",
+    "sp_settings_write_single_results", "sp_settings_prepare", "sp_settings_submit", "sp_prepared_inputs",
 ]
 
 _LIB = None
@@ -125,6 +127,9 @@ def lib():
         L.sp_settings_single_savefile.argtypes = [C.c_void_p]
         L.sp_settings_single_savefile.restype = C.c_char_p
         L.sp_settings_write_single_results.argtypes = [C.c_void_p, P(Result), C.c_char_p]
+        L.sp_settings_prepare.argtypes = [C.c_void_p, C.c_void_p, P(C.c_void_p)]
+        L.sp_settings_submit.argtypes = [C.c_void_p, C.c_void_p, P(C.c_uint64)]
+        L.sp_prepared_inputs.argtypes = [C.c_void_p, d, d]
         L.sp_settings_free.argtypes = [C.c_void_p]
         L.sp_settings_free.restype = None
         L.sp_host_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, d]

@@ -7,12 +7,11 @@
 // (SimulationManager.h:373-478).  Key names follow the serialize() functions of
 // Settings/*.h and Properties/*.h; a few older keys of Examples/*.ini are accepted too.
 //
-// Choices where the reference draws from std::random_device (unreproducible):
-// random initial orientations/magnetisations and the discrete particle-type
-// selection use Philox stream 1 under the request seed.  Radius, shell and
-// anisotropy distributions are delta, normal or lognormal with the given mean and
-// width (for lognormal: the normal parameters that give that mean and standard
-// deviation); MyCEL's own parameterisation is not available to compare with.
+// The particle files become a table of particle types; particle i of the ensemble is
+// provide_particle(i) of sp_provider.h, run here on the host when the arrays are asked
+// for (sp_settings_request) or by the provider kernel on the device
+// (sp_settings_prepare, sp_settings_submit).  MyCEL's own parameterisation of the
+// lognormal distribution is not available to compare with.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -23,7 +22,9 @@
 #include <string>
 #include <vector>
 
+#include "../sp_engine.h"
 #include "../sp_host.h"
+#include "../sp_provider.h"
 #include "../sp_rng.h"
 #include "ini.h"
 
@@ -52,7 +53,9 @@ struct Settings {
     bool discrete = false;
     std::vector<ParticleFile> particles;
     std::string savefile, single_savefile, single_prefix = "Single";
-    // materialised arrays
+    std::shared_ptr<const ProvSpec> spec;   // the particle types as the provider draws them
+    // host arrays, materialised on first use
+    bool materialised = false;
     std::vector<double> radius, Ms, alpha, gamma, T, K, r_h, visc, axes, y0, K2, urot;
 };
 
@@ -138,30 +141,88 @@ static ParticleFile load_particle(const std::string& path) {
     return p;
 }
 
-// ---- seeded draws (Philox stream tag 1: particle p, draw index k)
-struct Draw {
-    uint64_t seed, p; uint64_t k = 0;
-    double uniform() {   // (0,1)
-        const U4 w = noise_block(seed, p, k++, 1u);
-        return u01_f64(w.x);
-    }
-    double normal() {
-        const U4 w = noise_block(seed, p, k++, 1u);
-        double z[4];
-        box_muller4_f64(w, z);
-        return z[0];
-    }
-};
-
-static double draw_dist(const Dist& d, double mean, Draw& rng) {
-    if (d.type == "delta" || d.width == 0.0) return mean;
+// ---- particle types for the provider
+static ProvDist prov_dist(const Dist& d, double mean) {
+    ProvDist p{};
+    p.mean = mean;
+    if (d.type == "delta" || d.width == 0.0) { p.type = PD_DELTA; return p; }
     const double sd = d.relative ? d.width * mean : d.width;
-    if (d.type == "normal") return mean + sd * rng.normal();
+    if (d.type == "normal") { p.type = PD_NORMAL; p.sd = sd; return p; }
     if (d.type == "lognormal") {
         const double s2 = std::log1p((sd * sd) / (mean * mean));
-        return mean * std::exp(std::sqrt(s2) * rng.normal() - 0.5 * s2);
+        p.type = PD_LOGNORMAL;
+        p.sigma = std::sqrt(s2);
+        p.half_s2 = 0.5 * s2;
+        return p;
     }
     throw std::runtime_error("unknown distribution type '" + d.type + "'");
+}
+
+static std::shared_ptr<const ProvSpec> build_spec(const Settings& s) {
+    auto spec = std::make_shared<ProvSpec>();
+    ProvDesc& d = spec->base;
+    d = ProvDesc{};
+    d.seed = s.rq.seed;
+    d.N = s.rq.n_particles;
+    d.problem = s.rq.problem;
+    d.aniso = s.particles[0].aniso;
+    d.n_types = (uint32_t)s.particles.size();
+    d.discrete = s.discrete ? 1u : 0u;
+    double acc = 0.0;
+    uint64_t acc_n = 0;
+    for (const ParticleFile& pf : s.particles) {
+        if (pf.aniso != d.aniso) throw std::runtime_error("all particle types of one run must share the anisotropy type");
+        ProvType t{};
+        t.rad = prov_dist(pf.rad, pf.radius);
+        t.shell = prov_dist(pf.shell, pf.r_h - pf.radius);
+        t.K = prov_dist(pf.aniso_dist, pf.K);
+        t.K2 = prov_dist(pf.aniso_dist2, pf.K2);
+        t.Ms = pf.Ms; t.alpha = pf.alpha; t.gamma = pf.gamma; t.T = pf.T; t.visc = pf.visc;
+        for (int k = 0; k < 3; ++k) { t.euler[k] = pf.euler[k]; t.mag[k] = pf.mag[k]; t.urot[k] = pf.urot[k]; }
+        t.rand_orient = pf.rand_orient; t.rand_mag = pf.rand_mag; t.rand_urot = pf.rand_urot;
+        acc += (double)pf.count;
+        acc_n += pf.count;
+        t.cum = acc;
+        t.cum_count = acc_n;
+        spec->types.push_back(t);
+    }
+    d.total_count = acc;
+    return spec;
+}
+
+// the N particles as host arrays (Provider::getProvidedObject, ParticleSimulationSettings::applySettingsToParticleProperties)
+static void materialise(Settings& s) {
+    if (s.materialised) return;
+    const uint64_t N = s.rq.n_particles;
+    uint32_t dy = 0, dw = 0, dout = 0;
+    request_dims(s.rq.problem, &dy, &dw, &dout);
+    s.radius.resize(N); s.Ms.resize(N); s.alpha.resize(N); s.gamma.resize(N); s.T.resize(N); s.K.resize(N);
+    s.r_h.resize(N); s.visc.resize(N); s.axes.resize(N * 9); s.y0.resize(N * dy);
+    const bool uc = s.spec->base.aniso == SP_ANISO_UNIAXIALCUBIC;
+    if (uc) { s.K2.resize(N); s.urot.resize(N * 3); }
+    ProvDesc d = s.spec->base;
+    ParticleProps q;
+    for (uint64_t i = 0; i < N; ++i) {
+        provide_particle(d, s.spec->types.data(), i, q);
+        s.radius[i] = q.radius; s.Ms[i] = q.Ms; s.alpha[i] = q.alpha; s.gamma[i] = q.gamma; s.T[i] = q.T;
+        s.K[i] = q.K; s.r_h[i] = q.r_h; s.visc[i] = q.visc;
+        if (uc) { s.K2[i] = q.K2; for (int k = 0; k < 3; ++k) s.urot[i * 3 + k] = q.urot[k]; }
+        for (int k = 0; k < 9; ++k) s.axes[i * 9 + k] = q.axes[k];
+        for (uint32_t k = 0; k < dy; ++k) s.y0[i * dy + k] = q.y0[k];
+    }
+    s.materialised = true;
+}
+
+// particle p's properties: from the host arrays when they exist, else drawn on the host for this particle
+static ParticleProps particle_at(const Settings& s, uint64_t p) {
+    ParticleProps q{};
+    if (s.materialised) {
+        q.radius = s.radius[p]; q.Ms = s.Ms[p]; q.alpha = s.alpha[p]; q.gamma = s.gamma[p]; q.T = s.T[p];
+        q.K = s.K[p]; q.r_h = s.r_h[p]; q.visc = s.visc[p];
+    } else {
+        provide_particle(s.spec->base, s.spec->types.data(), p, q);
+    }
+    return q;
 }
 
 Settings* load(const std::string& path) {
@@ -252,82 +313,10 @@ Settings* load(const std::string& path) {
         s->particles.push_back(pf);
     }
     if (s->particles.empty()) throw std::runtime_error("empty particle list");
-    // materialise N particles (Provider::getProvidedObject, ParticleSimulationSettings::applySettingsToParticleProperties)
-    const uint64_t N = r.n_particles;
     uint32_t dy = 0, dw = 0, dout = 0;
     if (request_dims(r.problem, &dy, &dw, &dout)) throw std::runtime_error("problem type not available on the GPU backend");
-    s->radius.resize(N); s->Ms.resize(N); s->alpha.resize(N); s->gamma.resize(N); s->T.resize(N); s->K.resize(N);
-    s->r_h.resize(N); s->visc.resize(N); s->axes.resize(N * 9); s->y0.resize(N * dy);
-    if (s->particles[0].aniso == SP_ANISO_UNIAXIALCUBIC) { s->K2.resize(N); s->urot.resize(N * 3); }
-    std::vector<uint64_t> used(s->particles.size(), 0);
-    uint64_t total_count = 0;
-    for (auto& p : s->particles) total_count += p.count;
-    const int aniso0 = s->particles[0].aniso;
-    for (uint64_t i = 0; i < N; ++i) {
-        Draw rng{r.seed, i};
-        size_t sel = 0;
-        if (s->discrete && total_count) {
-            double u = rng.uniform() * (double)total_count, acc = 0;
-            for (sel = 0; sel + 1 < s->particles.size(); ++sel) { acc += (double)s->particles[sel].count; if (u < acc) break; }
-        } else {
-            for (sel = 0; sel + 1 < s->particles.size() && used[sel] >= s->particles[sel].count; ++sel) {}
-        }
-        ++used[sel];
-        const ParticleFile& pf = s->particles[sel];
-        if (pf.aniso != aniso0) throw std::runtime_error("all particle types of one run must share the anisotropy type");
-        const double rad = draw_dist(pf.rad, pf.radius, rng);
-        const double shell = draw_dist(pf.shell, pf.r_h - pf.radius, rng);
-        s->radius[i] = rad; s->Ms[i] = pf.Ms; s->alpha[i] = pf.alpha; s->gamma[i] = pf.gamma; s->T[i] = pf.T;
-        s->K[i] = draw_dist(pf.aniso_dist, pf.K, rng);
-        s->r_h[i] = shell + rad; s->visc[i] = pf.visc;
-        if (pf.aniso == SP_ANISO_UNIAXIALCUBIC) {      // UniaxialCubic_Distribution::applyDistribution (:63-76)
-            s->K2[i] = draw_dist(pf.aniso_dist2, pf.K2, rng);
-            const double scale[3] = {6.283185307179586, 3.141592653589793, 6.283185307179586};
-            for (int k = 0; k < 3; ++k) s->urot[i * 3 + k] = pf.rand_urot ? rng.uniform() * scale[k] : pf.urot[k];
-        }
-        // axes from Euler angles (NeelRelaxation.h:223-261); random: each angle U(0, 2 pi), or for
-        // BrownAndNeelEulerSpherical phi, psi U(0, 2 pi) and theta U(0, pi) (ParticleStateInitializer.h:42-62)
-        const bool euler_state = r.problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL;
-        double e[3] = {pf.euler[0], pf.euler[1], pf.euler[2]};
-        if (pf.rand_orient)
-            for (int k = 0; k < 3; ++k) e[k] = rng.uniform() * (euler_state && k == 1 ? 3.141592653589793 : 6.283185307179586);
-        const double cphi = std::cos(e[0]), cth = std::cos(e[1]), cpsi = std::cos(e[2]);
-        const double sphi = std::sin(e[0]), sth = std::sin(e[1]), spsi = std::sin(e[2]);
-        const double ax[9] = {cphi * cpsi - cth * sphi * spsi, -sphi * cpsi - cth * cphi * spsi, sth * spsi,
-                              cth * sphi * cpsi + cphi * spsi, cth * cphi * cpsi - sphi * spsi, -sth * cpsi,
-                              sth * sphi, sth * cphi, cth};
-        for (int k = 0; k < 9; ++k) s->axes[i * 9 + k] = ax[k];
-        double m[3] = {pf.mag[0], pf.mag[1], pf.mag[2]};
-        if (pf.rand_mag && euler_state) {   // ParticleStateInitializer.h:107-130: phi U(0, 2 pi), theta U(0, pi)
-            const double ph = rng.uniform() * 6.283185307179586, th = rng.uniform() * 3.141592653589793;
-            m[0] = std::cos(ph) * std::sin(th); m[1] = std::sin(ph) * std::sin(th); m[2] = std::cos(th);
-        } else if (pf.rand_mag) {
-            for (double& x : m) x = rng.normal();
-        }
-        const double mn = std::sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
-        for (double& x : m) x /= mn;
-        if (euler_state) {                  // getStart (BrownAndNeelRelaxationEulerSpherical.h:741-753)
-            for (int k = 0; k < 3; ++k) s->y0[i * 5 + k] = e[k];
-            s->y0[i * 5 + 3] = std::acos(m[2]);
-            s->y0[i * 5 + 4] = std::atan2(m[1], m[0]);
-        } else if (r.problem == SP_PROBLEM_NEEL) { for (int k = 0; k < 3; ++k) s->y0[i * 3 + k] = m[k]; }
-        else if (r.problem == SP_PROBLEM_NEEL_SPHERICAL) {   // ParticleStateInitializer.h:132-144
-            s->y0[i * 2 + 0] = std::acos(m[2]);
-            s->y0[i * 2 + 1] = std::atan2(m[1], m[0]);
-        } else {                                             // BrownAndNeelRelaxation.inl:277-324
-            double n[3];
-            if (pf.rand_orient) { for (double& x : n) x = rng.normal(); }
-            else {
-                const double a = e[0], b = e[1], g = e[2];
-                n[0] = std::cos(a) * std::cos(g) - std::sin(a) * std::cos(b) * std::sin(g);
-                n[1] = -std::cos(a) * std::sin(g) - std::sin(a) * std::cos(b) * std::cos(g);
-                n[2] = std::sin(a) * std::sin(b);
-            }
-            const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-            for (int k = 0; k < 3; ++k) { s->y0[i * 6 + k] = n[k] / nn; s->y0[i * 6 + 3 + k] = m[k]; }
-        }
-    }
-    r.anisotropy = aniso0;
+    s->spec = build_spec(*s);
+    r.anisotropy = s->spec->base.aniso;
     return s.release();
 }
 
@@ -350,13 +339,13 @@ int write_result(const Settings& s, const sp_result& res, const std::string& pat
     for (uint64_t k = 0; k < res.n_out; ++k) { if (k) o << ','; write_array(o, res.mean + res.dim_out * k, res.dim_out); }
     o << "],\"Weight\":";
     write_array(o, res.weight, res.dim_out);
-    auto mean = [](const std::vector<double>& v) { double a = 0; for (double x : v) a += x; return v.empty() ? 0.0 : a / (double)v.size(); };
     char buf[512];
     std::snprintf(buf, sizeof buf,
                   ",\"Mean_Properties\":{\"Temperature\":%.17g,\"Magnetic_radius\":%.17g,\"Saturation_magnetisation\":%.17g,"
                   "\"Damping_constant\":%.17g,\"Gyromagnetic_ratio\":%.17g,\"Anisotropy_constant\":%.17g,"
                   "\"Hydrodynamic_Radius\":%.17g,\"Viscosity\":%.17g}}",
-                  mean(s.T), mean(s.radius), mean(s.Ms), mean(s.alpha), mean(s.gamma), mean(s.K), mean(s.r_h), mean(s.visc));
+                  res.particle_means[0], res.particle_means[1], res.particle_means[2], res.particle_means[3],
+                  res.particle_means[4], res.particle_means[5], res.particle_means[6], res.particle_means[7]);
     o << buf;
     std::snprintf(buf, sizeof buf,
                   ",\"Engine\":{\"device_ms\":%.6f,\"particle_steps\":%.17g,\"precision_used\":%u,\"check_error\":%.6g,"
@@ -380,14 +369,15 @@ int write_single_results(const Settings& s, const sp_result& res, const std::str
     std::vector<double> row(D);
     for (uint64_t j = 0; j < S; ++j) {
         const uint64_t p = (j + 1) * k - 1;
-        const double V = 4.0 / 3.0 * 3.14159265358979323846 * s.radius[p] * s.radius[p] * s.radius[p];
-        const double mu = s.Ms[p] * V;
+        const ParticleProps q = particle_at(s, p);
+        const double V = 4.0 / 3.0 * 3.14159265358979323846 * q.radius * q.radius * q.radius;
+        const double mu = q.Ms * V;
         std::snprintf(buf, sizeof buf,
                       "%s\"%s_%llu\":{\"UsedProperties\":{\"Temperature\":%.17g,\"Magnetic_radius\":%.17g,"
                       "\"Saturation_magnetisation\":%.17g,\"Damping_constant\":%.17g,\"Gyromagnetic_ratio\":%.17g,"
                       "\"Anisotropy_constant\":%.17g,\"Hydrodynamic_Radius\":%.17g,\"Viscosity\":%.17g},\"Time\":",
-                      j ? "," : "", s.single_prefix.c_str(), (unsigned long long)(p + 1), s.T[p], s.radius[p], s.Ms[p],
-                      s.alpha[p], s.gamma[p], s.K[p], s.r_h[p], s.visc[p]);
+                      j ? "," : "", s.single_prefix.c_str(), (unsigned long long)(p + 1), q.T, q.radius, q.Ms,
+                      q.alpha, q.gamma, q.K, q.r_h, q.visc);
         o << buf;
         write_array(o, res.time, res.n_out);
         for (uint64_t c = 0; c < D; ++c) row[c] = (D == 6 && c < 3) ? 1.0 : mu;   // getWeighting of the problem
@@ -429,9 +419,10 @@ int sp_settings_load(const char* ini_path, sp_settings** out) {
 
 int sp_settings_request(const sp_settings* h, sp_request* req, sp_particles* parts) {
     if (!h || !h->s) return sp::set_error(SP_ERR_INVALID_ARG, "NULL settings");
-    const auto& s = *h->s;
+    auto& s = *h->s;
     if (req) *req = s.rq;
     if (parts) {
+        sp::frontend::materialise(s);
         parts->magnetic_radius = s.radius.data();
         parts->saturation_magnetisation = s.Ms.data();
         parts->damping = s.alpha.data();
@@ -446,6 +437,29 @@ int sp_settings_request(const sp_settings* h, sp_request* req, sp_particles* par
         parts->uniaxial_rotation = s.urot.empty() ? nullptr : s.urot.data();
     }
     return SP_OK;
+}
+
+int sp_settings_prepare(sp_ctx* ctx, const sp_settings* h, sp_prepared** out) {
+    if (!ctx || !h || !h->s || !out) return sp::set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    const auto& s = *h->s;
+    std::unique_ptr<sp_prepared> p(new sp_prepared());
+    p->ctx = ctx;
+    if (int e = sp::prepare_provided(&s.rq, s.spec, p->pr.hr)) return e;
+    p->pr.keep_final = (s.rq.flags & SP_FLAG_KEEP_FINAL_STATE) != 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->run_mu);
+        if (int e = sp::prepare_plain(ctx, p->pr)) return e;
+    }
+    *out = p.release();
+    return SP_OK;
+}
+
+int sp_settings_submit(sp_ctx* ctx, const sp_settings* h, uint64_t* id) {
+    if (!ctx || !h || !h->s || !id) return sp::set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    const auto& s = *h->s;
+    sp::HostRequest hr;
+    if (int e = sp::prepare_provided(&s.rq, s.spec, hr)) return e;
+    return sp::submit_request(ctx, std::move(hr), (s.rq.flags & SP_FLAG_KEEP_FINAL_STATE) != 0, id);
 }
 
 const char* sp_settings_single_savefile(const sp_settings* h) { return (h && h->s) ? h->s->single_savefile.c_str() : nullptr; }

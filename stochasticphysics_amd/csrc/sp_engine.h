@@ -18,6 +18,7 @@
 
 #include "sp_host.h"
 #include "sp_kernels.h"
+#include "sp_provider.h"
 #include "sp_rng.h"
 
 namespace sp {
@@ -34,6 +35,12 @@ inline double ms_since(Clock::time_point t0, Clock::time_point t1) {
 }
 
 // ------------------------------------------------------------------ validated request
+// particles drawn on the device by the provider kernel instead of uploaded from host arrays
+struct ProvSpec {
+    ProvDesc base;                 // seed, N, problem, anisotropy and selection of the ensemble (no pointers)
+    std::vector<ProvType> types;
+};
+
 struct HostRequest {
     sp_request rq{};
     uint32_t dy = 0, dw = 0, dout = 0, kind = 0;
@@ -43,7 +50,9 @@ struct HostRequest {
     std::vector<double> params;   // [NPARAM][N]
     std::vector<double> y0;       // [dy][N]
     std::vector<double> replay;   // [K][N]
+    std::shared_ptr<const ProvSpec> prov;   // set: params and y0 are drawn on the device (both vectors empty)
     double wsum[9] = {0}, wsum_check[9] = {0};
+    double prop_sum[8] = {0};     // sums of T, radius, Ms, alpha, gamma, K, r_h, viscosity over the particles
     uint32_t n_check_tiles = 0;
     uint64_t n_saved = 0;         // particles with single-simulation outputs (save_interval)
 };
@@ -56,8 +65,9 @@ inline uint32_t normals_per_step(const sp_request& rq, uint32_t dw) {
     return rq.solver == SP_SOLVER_HEUN_NOT_CONSISTENT ? 2 * dw : dw;
 }
 
-inline int validate(const sp_request* rq, const sp_particles* pt) {
-    if (!rq || !pt) return set_error(SP_ERR_INVALID_ARG, "request or particles is NULL");
+// pt == nullptr: the particles are drawn on the device (only the request is checked)
+inline int validate(const sp_request* rq, const sp_particles* pt, bool provided = false) {
+    if (!rq || (!pt && !provided)) return set_error(SP_ERR_INVALID_ARG, "request or particles is NULL");
     uint32_t dy, dw, dout;
     if (request_dims(rq->problem, &dy, &dw, &dout)) return set_error(SP_ERR_UNSUPPORTED, "unknown problem type");
     if (rq->solver != SP_SOLVER_EULER_MARUYAMA && rq->solver != SP_SOLVER_EULER_MARUYAMA_NORMALIZED &&
@@ -84,8 +94,8 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
         // BrownAndNeelRelaxationEulerSpherical.h:606-671)
         return set_error(SP_ERR_UNSUPPORTED, "Implicit_Midpoint is not available for BrownAndNeelEulerSpherical");
     }
-    if (rq->anisotropy == SP_ANISO_UNIAXIALCUBIC) {
-        if (!pt || !pt->anisotropy_constant_cubic || !pt->uniaxial_rotation)
+    if (rq->anisotropy == SP_ANISO_UNIAXIALCUBIC && pt) {
+        if (!pt->anisotropy_constant_cubic || !pt->uniaxial_rotation)
             return set_error(SP_ERR_INVALID_ARG, "uniaxialcubic needs anisotropy_constant_cubic and uniaxial_rotation");
     }
     if (rq->precision != SP_PREC_FP64 && rq->precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
@@ -94,10 +104,10 @@ inline int validate(const sp_request* rq, const sp_particles* pt) {
     if (rq->oversampling == 0) return set_error(SP_ERR_INVALID_ARG, "oversampling must be positive");
     if (rq->n_steps < rq->oversampling) return set_error(SP_ERR_INVALID_ARG, "n_steps must be >= oversampling");
     if (!(rq->dt > 0.0) || !std::isfinite(rq->dt)) return set_error(SP_ERR_INVALID_ARG, "dt must be positive");
-    if (!pt->magnetic_radius || !pt->saturation_magnetisation || !pt->damping || !pt->gyromagnetic_ratio ||
-        !pt->temperature || !pt->anisotropy_constant || !pt->initial_state)
+    if (pt && (!pt->magnetic_radius || !pt->saturation_magnetisation || !pt->damping || !pt->gyromagnetic_ratio ||
+               !pt->temperature || !pt->anisotropy_constant || !pt->initial_state))
         return set_error(SP_ERR_INVALID_ARG, "missing particle property array");
-    if ((rq->problem == SP_PROBLEM_BROWN_AND_NEEL || rq->problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL) &&
+    if ((rq->problem == SP_PROBLEM_BROWN_AND_NEEL || rq->problem == SP_PROBLEM_BROWN_AND_NEEL_EULER_SPHERICAL) && pt &&
         (!pt->hydrodynamic_radius || !pt->viscosity))
         return set_error(SP_ERR_INVALID_ARG, "BrownAndNeel problems need hydrodynamic_radius and viscosity");
     if (rq->solver == SP_SOLVER_IMPLICIT_MIDPOINT && (rq->newton_max_iter == 0 || !(rq->newton_tol > 0.0)))
@@ -121,8 +131,8 @@ inline uint32_t check_tiles_for(uint32_t n_tiles) {
     return std::min<uint32_t>(n_tiles, std::max<uint32_t>(8u, std::min<uint32_t>(64u, n_tiles / 8)));
 }
 
-inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostRequest& hr) {
-    if (int e = validate(rq, pt)) return e;
+// the request-level fields of a HostRequest (shapes, kernel class, replayed noise, check tiles)
+inline void prepare_shape(const sp_request* rq, HostRequest& hr) {
     hr.rq = *rq;
     request_dims(rq->problem, &hr.dy, &hr.dw, &hr.dout);
     hr.N = rq->n_particles;
@@ -132,11 +142,6 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.mixed = rq->precision == SP_PREC_MIXED && rq->solver != SP_SOLVER_IMPLICIT_MIDPOINT &&
                rq->solver != SP_SOLVER_HEUN_NOT_CONSISTENT;     // both run in FP64 only
     hr.kind = kind_for(*rq, hr.mixed);
-    hr.params.assign((size_t)NPARAM * hr.N, 0.0);
-    compute_params(*rq, *pt, hr.params.data(), hr.N);
-    hr.y0.resize((size_t)hr.dy * hr.N);
-    for (uint64_t p = 0; p < hr.N; ++p)
-        for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
     if (rq->noise == SP_NOISE_REPLAY) {
         const uint64_t K = hr.total_steps * normals_per_step(*rq, hr.dw);
         hr.replay.assign(rq->replay_normals, rq->replay_normals + K * hr.N);
@@ -145,6 +150,16 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.rq.replay_normals = nullptr;
     hr.n_saved = rq->save_interval ? hr.N / rq->save_interval : 0;
     hr.n_check_tiles = check_tiles_for(hr.n_tiles);
+}
+
+inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostRequest& hr) {
+    if (int e = validate(rq, pt)) return e;
+    prepare_shape(rq, hr);
+    hr.params.assign((size_t)NPARAM * hr.N, 0.0);
+    compute_params(*rq, *pt, hr.params.data(), hr.N);
+    hr.y0.resize((size_t)hr.dy * hr.N);
+    for (uint64_t p = 0; p < hr.N; ++p)
+        for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
     const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
     for (uint32_t c = 0; c < hr.dout; ++c) { hr.wsum[c] = 0.0; hr.wsum_check[c] = 0.0; }
     for (uint64_t p = 0; p < hr.N; ++p) {
@@ -155,6 +170,24 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
             if (p < n_check) hr.wsum_check[c] += w;
         }
     }
+    const double* props[8] = {pt->temperature, pt->magnetic_radius, pt->saturation_magnetisation, pt->damping,
+                              pt->gyromagnetic_ratio, pt->anisotropy_constant, pt->hydrodynamic_radius, pt->viscosity};
+    for (int k = 0; k < 8; ++k) {
+        hr.prop_sum[k] = 0.0;
+        if (props[k]) for (uint64_t p = 0; p < hr.N; ++p) hr.prop_sum[k] += props[k][p];
+    }
+    return SP_OK;
+}
+
+// a request whose particles the provider kernel draws on the device when its buffers are allocated
+inline int prepare_provided(const sp_request* rq, std::shared_ptr<const ProvSpec> spec, HostRequest& hr) {
+    if (int e = validate(rq, nullptr, true)) return e;
+    if (!spec || spec->types.empty()) return set_error(SP_ERR_INVALID_ARG, "no particle types");
+    if (spec->base.N != rq->n_particles || spec->base.seed != rq->seed || spec->base.problem != rq->problem ||
+        spec->base.aniso != rq->anisotropy)
+        return set_error(SP_ERR_INVALID_ARG, "particle types do not match the request");
+    prepare_shape(rq, hr);
+    hr.prov = std::move(spec);
     return SP_OK;
 }
 
@@ -195,15 +228,19 @@ inline void free_dev(ReqDev& d) {
     d = ReqDev{};
 }
 
-inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
+int run_provider(HostRequest& hr, ReqDev& d, hipStream_t s);   // sp_runtime.cpp
+
+// device buffers of a request; its parameters and initial state are uploaded, or drawn on the device by
+// the provider kernel (which also fills the weight and property sums of hr)
+inline int alloc_dev(HostRequest& hr, ReqDev& d, hipStream_t s) {
     auto fail = [&](const char* what, hipError_t e) {
         free_dev(d);
         return set_error(e == hipErrorOutOfMemory ? SP_ERR_OUT_OF_MEMORY : SP_ERR_DEVICE,
                          std::string("allocating ") + what + ": " + hipGetErrorString(e));
     };
     hipError_t e;
-    if ((e = hipMalloc(&d.params, hr.params.size() * 8)) != hipSuccess) return fail("params", e);
-    if ((e = hipMalloc(&d.state, hr.y0.size() * 8)) != hipSuccess) return fail("state", e);
+    if ((e = hipMalloc(&d.params, (size_t)NPARAM * hr.N * 8)) != hipSuccess) return fail("params", e);
+    if ((e = hipMalloc(&d.state, (size_t)hr.dy * hr.N * 8)) != hipSuccess) return fail("state", e);
     if (!hr.replay.empty() && (e = hipMalloc(&d.replay, hr.replay.size() * 8)) != hipSuccess) return fail("replay", e);
     if ((e = hipMalloc(&d.out, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("out", e);
     if ((e = hipMalloc(&d.check, check_bytes(hr))) != hipSuccess) return fail("check", e);
@@ -211,8 +248,12 @@ inline int alloc_dev(const HostRequest& hr, ReqDev& d, hipStream_t s) {
     if ((e = hipMalloc(&d.dreq, sizeof(DevRequest))) != hipSuccess) return fail("descriptor", e);
     if (traj_bytes(hr) && (e = hipMalloc(&d.traj, traj_bytes(hr))) != hipSuccess) return fail("trajectories", e);
     d.bytes = request_bytes(hr);
-    SP_HIP(hipMemcpyAsync(d.params, hr.params.data(), hr.params.size() * 8, hipMemcpyHostToDevice, s));
-    SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
+    if (hr.prov) {
+        if (int e2 = run_provider(hr, d, s)) { free_dev(d); return e2; }
+    } else {
+        SP_HIP(hipMemcpyAsync(d.params, hr.params.data(), hr.params.size() * 8, hipMemcpyHostToDevice, s));
+        SP_HIP(hipMemcpyAsync(d.state, hr.y0.data(), hr.y0.size() * 8, hipMemcpyHostToDevice, s));
+    }
     if (!hr.replay.empty()) SP_HIP(hipMemcpyAsync(d.replay, hr.replay.data(), hr.replay.size() * 8, hipMemcpyHostToDevice, s));
     SP_HIP(hipMemsetAsync(d.out, 0, hr.n_out * hr.dout * 8, s));
     SP_HIP(hipMemsetAsync(d.check, 0, check_bytes(hr), s));
@@ -321,6 +362,7 @@ inline void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
     r->used_fallback = o.used_fallback;
     r->precision_used = o.precision_used;
     r->newton_failures = o.newton_failures;
+    for (int k = 0; k < 8; ++k) r->particle_means[k] = hr.prop_sum[k] / (double)hr.N;
 }
 
 // ------------------------------------------------------------------ context
@@ -465,4 +507,11 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun);
 int run_plain(sp_ctx* ctx, HostRequest& hr, Outputs& o, bool keep_final, const sp_tuning& tun);
 // defined in sp_server.cpp
 void stop_worker(sp_ctx* ctx);
+// queues a validated request on the server (sp_submit after its checks)
+int submit_request(sp_ctx* ctx, HostRequest&& hr, bool keep_final, uint64_t* id);
 }  // namespace sp
+
+struct sp_prepared {
+    sp::Prepared pr;
+    sp_ctx* ctx = nullptr;
+};

@@ -11,8 +11,6 @@ namespace sp {
 int request_dims(int problem, uint32_t* dy, uint32_t* dw, uint32_t* dout);
 void compute_params(const sp_request& rq, const sp_particles& pt, double* out, uint64_t stride);
 void time_and_field(const sp_request& rq, double* times, double* fields);
-// rotation matrix of the uniaxial part of uniaxialcubic anisotropy from its Euler angles
-void uniaxial_rotation_matrix(const double* euler, double R[3][3]);
 // the request's field as FieldParams (angular frequencies, rectangular edge constants)
 FieldParams field_params(const sp_request& rq);
 

@@ -1,6 +1,7 @@
 // sp_kernels.h -- host-side entry points of the compiled kernel classes.
 #pragma once
 #include <hip/hip_runtime.h>
+#include "sp_provider.h"
 #include "sp_types.h"
 
 namespace sp {
@@ -43,6 +44,9 @@ SP_BNES_DECL(2)
 SP_BNES_DECL(4)
 #undef SP_BNES_DECL
 
+// the provider kernel: particles [0, d.N) drawn into d.params / d.y0, per-group sums into d.sums
+// (blocks = ceil(N / PROV_BLOCK))
+void launch_provide(hipStream_t s, uint32_t blocks, const ProvDesc& d);
 // deterministic second-stage reduction; one block per (entry, row)
 void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows);
 // zero n counters (a kernel, so that it can sit in a captured graph)

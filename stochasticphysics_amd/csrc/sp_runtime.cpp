@@ -31,9 +31,10 @@ void free_prepared(Prepared& pr) {
 int prepare_plain(sp_ctx* ctx, Prepared& pr) {
     SP_HIP(hipSetDevice(ctx->device));
     if (int e = alloc_dev(pr.hr, pr.d, ctx->stream)) return e;
-    hipError_t e = hipMalloc(&pr.state0, pr.hr.y0.size() * 8);
+    const size_t state_bytes = (size_t)pr.hr.dy * pr.hr.N * 8;
+    hipError_t e = hipMalloc(&pr.state0, state_bytes);
     if (e != hipSuccess) { free_dev(pr.d); return set_error(SP_ERR_OUT_OF_MEMORY, "initial state copy"); }
-    SP_HIP(hipMemcpyAsync(pr.state0, pr.d.state, pr.hr.y0.size() * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    SP_HIP(hipMemcpyAsync(pr.state0, pr.d.state, state_bytes, hipMemcpyDeviceToDevice, ctx->stream));
     SP_HIP(hipStreamSynchronize(ctx->stream));
     return SP_OK;
 }
@@ -45,7 +46,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     hipStream_t s = ctx->stream;
     SP_HIP(hipSetDevice(ctx->device));
     const auto t0 = Clock::now();
-    SP_HIP(hipMemcpyAsync(d.state, pr.state0, hr.y0.size() * 8, hipMemcpyDeviceToDevice, s));
+    SP_HIP(hipMemcpyAsync(d.state, pr.state0, (size_t)hr.dy * hr.N * 8, hipMemcpyDeviceToDevice, s));
     SP_HIP(hipMemsetAsync(d.out, 0, hr.n_out * hr.dout * 8, s));
     SP_HIP(hipMemsetAsync(d.check, 0, check_bytes(hr), s));
     SP_HIP(hipMemsetAsync(d.newton_fail, 0, 8, s));
@@ -112,6 +113,48 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     ctx->last_stats[5] = ki.vgprs;
     ctx->last_stats[6] = (double)grid.x;
     ctx->last_stats[7] = (double)block;
+    return SP_OK;
+}
+
+// ------------------------------------------------------------------ device-side particle provider
+// One thread per particle draws the particle, evaluates its parameter block and start state into the
+// request's buffers, and each work-group leaves the sums of its particles' properties and weights; the host
+// adds the per-group sums in group order, so the totals do not depend on scheduling.
+int run_provider(HostRequest& hr, ReqDev& d, hipStream_t s) {
+    const ProvSpec& spec = *hr.prov;
+    const uint32_t blocks = (uint32_t)((hr.N + PROV_BLOCK - 1) / PROV_BLOCK);
+    ProvType* d_types = nullptr;
+    double* d_sums = nullptr;
+    struct Guard {
+        ProvType** t; double** s;
+        ~Guard() { if (*t) hipFree(*t); if (*s) hipFree(*s); }
+    } guard{&d_types, &d_sums};
+    SP_HIP(hipMalloc(&d_types, spec.types.size() * sizeof(ProvType)));
+    SP_HIP(hipMalloc(&d_sums, (size_t)blocks * PROV_NSUM * 8));
+    SP_HIP(hipMemcpyAsync(d_types, spec.types.data(), spec.types.size() * sizeof(ProvType), hipMemcpyHostToDevice, s));
+    ProvDesc desc = spec.base;
+    desc.N = hr.N;
+    desc.n_types = (uint32_t)spec.types.size();
+    desc.n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
+    desc.dy = hr.dy;
+    desc.types = d_types;
+    desc.params = d.params;
+    desc.y0 = d.state;
+    desc.sums = d_sums;
+    launch_provide(s, blocks, desc);
+    SP_HIP(hipGetLastError());
+    std::vector<double> sums((size_t)blocks * PROV_NSUM);
+    SP_HIP(hipMemcpyAsync(sums.data(), d_sums, sums.size() * 8, hipMemcpyDeviceToHost, s));
+    SP_HIP(hipStreamSynchronize(s));
+    double acc[PROV_NSUM] = {0};
+    for (uint32_t b = 0; b < blocks; ++b)
+        for (int k = 0; k < PROV_NSUM; ++k) acc[k] += sums[(size_t)b * PROV_NSUM + k];
+    for (uint32_t c = 0; c < hr.dout; ++c) {
+        const bool unit = hr.dout == 6 && c < 3;      // BrownAndNeelRelaxation.h:152-159: weight 1 for the axis
+        hr.wsum[c] = unit ? (double)hr.N : acc[8];
+        hr.wsum_check[c] = unit ? (double)desc.n_check : acc[9];
+    }
+    for (int k = 0; k < 8; ++k) hr.prop_sum[k] = acc[k];
     return SP_OK;
 }
 
@@ -243,11 +286,6 @@ int sp_run(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_res
     return SP_OK;
 }
 
-struct sp_prepared {
-    sp::Prepared pr;
-    sp_ctx* ctx = nullptr;
-};
-
 int sp_prepare(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_prepared** out) {
     if (!ctx || !out) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
     std::unique_ptr<sp_prepared> h(new sp_prepared());
@@ -273,6 +311,21 @@ int sp_execute(sp_prepared* h, sp_result* res) {
     Outputs o;
     if (int e = execute_plain(h->ctx, h->pr, o, t)) return e;
     copy_result(h->pr.hr, o, res);
+    return SP_OK;
+}
+
+int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_state) {
+    if (!h) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    const HostRequest& hr = h->pr.hr;
+    std::lock_guard<std::mutex> lk(h->ctx->run_mu);
+    SP_HIP(hipSetDevice(h->ctx->device));
+    if (params) SP_HIP(hipMemcpy(params, h->pr.d.params, (size_t)NPARAM * hr.N * 8, hipMemcpyDeviceToHost));
+    if (initial_state) {
+        std::vector<double> soa((size_t)hr.dy * hr.N);
+        SP_HIP(hipMemcpy(soa.data(), h->pr.state0, soa.size() * 8, hipMemcpyDeviceToHost));
+        for (uint64_t p = 0; p < hr.N; ++p)
+            for (uint32_t i = 0; i < hr.dy; ++i) initial_state[p * hr.dy + i] = soa[i * hr.N + p];
+    }
     return SP_OK;
 }
 

@@ -195,12 +195,18 @@ static void make_shadow(const HostRequest& parent, HostRequest& sh) {
     sh.n_tiles = (uint32_t)((sh.N + 63) / 64);
     sh.mixed = false;
     sh.kind = kind_for(parent.rq, false);
-    sh.params.resize((size_t)NPARAM * sh.N);
-    for (uint32_t k = 0; k < NPARAM; ++k)
-        for (uint64_t p = 0; p < sh.N; ++p) sh.params[k * sh.N + p] = parent.params[k * parent.N + p];
-    sh.y0.resize((size_t)sh.dy * sh.N);
-    for (uint32_t i = 0; i < sh.dy; ++i)
-        for (uint64_t p = 0; p < sh.N; ++p) sh.y0[i * sh.N + p] = parent.y0[i * parent.N + p];
+    if (parent.prov) {      // particle p is a function of (seed, p): the provider draws the same first particles
+        auto spec = std::make_shared<ProvSpec>(*parent.prov);
+        spec->base.N = sh.N;
+        sh.prov = std::move(spec);
+    } else {
+        sh.params.resize((size_t)NPARAM * sh.N);
+        for (uint32_t k = 0; k < NPARAM; ++k)
+            for (uint64_t p = 0; p < sh.N; ++p) sh.params[k * sh.N + p] = parent.params[k * parent.N + p];
+        sh.y0.resize((size_t)sh.dy * sh.N);
+        for (uint32_t i = 0; i < sh.dy; ++i)
+            for (uint64_t p = 0; p < sh.N; ++p) sh.y0[i * sh.N + p] = parent.y0[i * parent.N + p];
+    }
     sh.n_check_tiles = sh.n_tiles;              // the same tiles, kept one by one
     for (uint32_t c = 0; c < sh.dout; ++c) { sh.wsum[c] = parent.wsum_check[c]; sh.wsum_check[c] = parent.wsum_check[c]; }
 }
@@ -666,9 +672,18 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_p
 
 int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uint64_t* id) {
     if (!ctx || !id) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    HostRequest hr;
+    if (int e = prepare_request(req, parts, hr)) return e;
+    return submit_request(ctx, std::move(hr), (req->flags & SP_FLAG_KEEP_FINAL_STATE) != 0, id);
+}
+
+}  // extern "C"
+
+namespace sp {
+int submit_request(sp_ctx* ctx, HostRequest&& hr, bool keep_final, uint64_t* id) {
     auto job = std::make_shared<Job>();
-    if (int e = prepare_request(req, parts, job->hr)) return e;
-    job->keep_final = (req->flags & SP_FLAG_KEEP_FINAL_STATE) != 0;
+    job->hr = std::move(hr);
+    job->keep_final = keep_final;
     job->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -682,6 +697,9 @@ int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uin
     *id = job->id;
     return SP_OK;
 }
+}  // namespace sp
+
+extern "C" {
 
 int sp_wait(sp_ctx* ctx, uint64_t id, int timeout_ms) {
     if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");

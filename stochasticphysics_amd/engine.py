@@ -240,6 +240,92 @@ class Prepared:
             pass
 
 
+class Settings:
+    """A reference parameter file (sp_settings): the request, and its particles on the host on demand."""
+
+    def __init__(self, path):
+        self._h = C.c_void_p()
+        check(lib().sp_settings_load(str(path).encode(), C.byref(self._h)))
+        self.request = Request()
+        check(lib().sp_settings_request(self._h, C.byref(self.request), None))
+
+    @property
+    def n(self):
+        return self.request.n_particles
+
+    @property
+    def dims(self):
+        n_out, dy, dw, dout = C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().sp_request_shape(C.byref(self.request), C.byref(n_out), C.byref(dy), C.byref(dw), C.byref(dout)))
+        return n_out.value, dy.value, dout.value
+
+    def alloc_result(self, keep_final=False):
+        n_out, dy, dout = self.dims
+        o = dict(time=np.zeros(n_out), field=np.zeros((n_out, 3)), mean=np.zeros((n_out, dout)), weight=np.zeros(dout))
+        res = Result()
+        res.time, res.field, res.mean, res.weight = dptr(o["time"]), dptr(o["field"]), dptr(o["mean"]), dptr(o["weight"])
+        if keep_final:
+            o["final_state"] = np.zeros((self.n, dy))
+            res.final_state = dptr(o["final_state"])
+        k = self.request.save_interval
+        if k:
+            o["trajectories"] = np.zeros((n_out, dout, self.n // k))
+            res.trajectories = dptr(o["trajectories"])
+        return res, o
+
+    def finish(self, res, o):
+        o = _finish_result(res, o)
+        o["particle_means"] = np.array(res.particle_means[:])
+        return o
+
+    def write_result(self, res, path):
+        check(lib().sp_settings_write_result(self._h, C.byref(res), str(path).encode()))
+
+    def close(self):
+        if self._h:
+            lib().sp_settings_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ProvidedPrepared:
+    """A parameter file's ensemble prepared on the device by the provider kernel (sp_settings_prepare)."""
+
+    def __init__(self, ctx, settings: Settings):
+        self.ctx, self.settings = ctx, settings
+        self._h = C.c_void_p()
+        check(lib().sp_settings_prepare(ctx._ctx, settings._h, C.byref(self._h)))
+
+    def execute(self, keep_final=False):
+        res, o = self.settings.alloc_result(keep_final)
+        check(lib().sp_execute(self._h, C.byref(res)))
+        self.last_result = res
+        return self.settings.finish(res, o)
+
+    def inputs(self):
+        """The resident parameter block [NPARAM, N] and initial state [N, dim_y] the provider drew."""
+        n, (_, dy, _) = self.settings.n, self.settings.dims
+        params, y0 = np.zeros(capi.NPARAM * n), np.zeros(n * dy)
+        check(lib().sp_prepared_inputs(self._h, dptr(params), dptr(y0)))
+        return params.reshape(capi.NPARAM, n), y0.reshape(n, dy)
+
+    def release(self):
+        if self._h:
+            lib().sp_release(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU (sp_ctx)."""
 
@@ -281,6 +367,20 @@ class Context:
         check(lib().sp_ctx_last_run_stats(self._ctx, dptr(s)))
         keys = ("launches", "kernel_ms", "reduce_ms", "wall_ms", "particle_steps", "vgprs", "grid", "block")
         return dict(zip(keys, s.tolist()))
+
+    def prepare_settings(self, settings: Settings):
+        """Draw a parameter file's particles on the device; the handle executes with them resident in HBM."""
+        return ProvidedPrepared(self, settings)
+
+    def submit_settings(self, settings: Settings):
+        rid = C.c_uint64()
+        check(lib().sp_settings_submit(self._ctx, settings._h, C.byref(rid)))
+        return rid.value
+
+    def settings_result(self, rid, settings: Settings):
+        res, o = settings.alloc_result(False)
+        check(lib().sp_result_get(self._ctx, rid, C.byref(res)))
+        return settings.finish(res, o)
 
     # ---- batched path
     def run_batch(self, ensembles, keep_final=False):
