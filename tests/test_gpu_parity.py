@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
-from stochasticphysics_amd import Context, Ensemble, configs, host_box_muller
+from stochasticphysics_amd import Context, Ensemble, compute_params, configs, host_box_muller
 from stochasticphysics_amd.capi import DIM_W
 
 pytestmark = pytest.mark.gpu
@@ -448,6 +448,43 @@ def test_zero_temperature_fixed_point(ctx):
     # the mean is sum(mu m_z) / sum(mu) with the two sums taken in different orders: each carries a relative
     # rounding error of at most (n - 1) u for n = 64 positive terms, so the ratio is 1 within 2 * 63 * 2^-53
     assert np.max(np.abs(g["mean"] - np.tile([0.0, 0.0, 1.0], (e.n_out, 1)))) <= 2 * 63 * 2.0**-53
+
+
+# ---------------------------------------------------------------- full-noise statistics
+def weighted_mean_and_se(per_particle, weights):
+    """per_particle [N, n_out, dout], weights [N, dout] -> the weighted ensemble mean [n_out, dout] and its
+    standard error from the particles' spread."""
+    w = weights[:, None, :]
+    mean = (w * per_particle).sum(0) / w.sum(0)
+    var = (w * w * (per_particle - mean) ** 2).sum(0) / w.sum(0) ** 2
+    return mean, np.sqrt(var)
+
+
+@pytest.mark.parametrize("make", [lambda s: configs.config1(n=8192, n_steps=6000, oversampling=100, seed=s),
+                                  lambda s: configs.config2(n=8192, n_steps=3000, oversampling=50, seed=s),
+                                  lambda s: configs.config3(n=8192, n_steps=4000, oversampling=100, seed=s)])
+def test_full_noise_ensemble_means_agree(ctx, make):
+    """Ensemble means at full noise: the GPU on one Philox stream and the oracle on an independent one agree
+    within 3 sigma / sqrt(N) (sigma from both ensembles' spread) in >= 90% of the output windows and within
+    4.5 sigma in every window (the windows are correlated in time; 4.5 sigma keeps a false alarm below 1e-3 over
+    them).  Config 1 (Neel EM, m0 on the easy axis), config 2 (BrownAndNeel Heun, sinusoidal drive) and config 3
+    (NeelSpherical cubic Heun, Lissajous drive) with their particle distributions, 8,192 particles."""
+    g_e, o_e = make(101), make(101)     # the same particles ...
+    o_e.seed = 303                      # ... and an independent noise stream for the oracle
+    g_e.save_interval = 1
+    g = ctx.run(g_e)
+    o = run_oracle(o_e, keep_final=False, per_particle=True)
+    mu = compute_params(g_e)[4]
+    w = np.repeat(mu[:, None], g_e.dim_out, axis=1)
+    if g_e.problem == "BrownAndNeel":
+        w[:, :3] = 1.0
+    gm, gse = weighted_mean_and_se(np.transpose(g["trajectories"], (2, 0, 1)), w)
+    om, ose = weighted_mean_and_se(o["per_particle"] / w[:, None, :], w)    # the oracle keeps weighted outputs
+    assert np.allclose(gm, g["mean"], rtol=0, atol=1e-12)           # the kept trajectories are the mean's inputs
+    z = np.abs(gm - om) / np.sqrt(gse ** 2 + ose ** 2 + 1e-300)
+    z = z[1:]                           # window 0 is the shared start state
+    assert np.mean(z <= 3.0) >= 0.9, np.mean(z <= 3.0)
+    assert z.max() <= 4.5, z.max()
 
 
 # ---------------------------------------------------------------- mixed precision
