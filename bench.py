@@ -77,7 +77,16 @@ def cpu_baseline(e, budget_s=12.0):
     sub = e.subset(np.arange(n))
     secs = run(sub, e.n_steps)
     rate = n * sub.total_steps / secs
+    model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return dict(value=rate, unit="particle-steps/s", cores=cores, kind="port",
+                per_core=rate / cores, ns_per_particle_step=1e9 * cores / rate, cpu_model=model,
                 sample=f"{n} of {e.n} particles x {sub.total_steps} steps ({secs:.1f} s, oracle -O3 -march=native, "
                        f"{cores} threads)")
 

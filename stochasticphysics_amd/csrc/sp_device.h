@@ -90,11 +90,31 @@ struct FieldCache {
 
 // ---------------------------------------------------------------- noise
 // 12 standard normals = 3 Philox blocks, starting at block `call`.
+// The FP64 Box-Muller tables in LDS: the simulation kernels read a table row per log and per sin/cos, and an
+// LDS read returns in a fraction of the latency of a vector-memory load that hits the cache.  Every FP64
+// simulation kernel fills its work-group's copy once at its start (bm_lds_fill); the values are those of
+// SP_LOG_TAB and SP_SINCOS_TAB, so the normals stay bit-identical to the host's.
+struct BMTables {
+    double log_tab[97][2];
+    double sc_tab[128][2];
+};
+static __shared__ BMTables sp_bm_lds;
+
+__device__ __forceinline__ void bm_lds_fill() {
+    constexpr int NL = 97 * 2, NS = 128 * 2;
+    double* dst = &sp_bm_lds.log_tab[0][0];
+    const double* lt = &SP_LOG_TAB[0][0];
+    const double* st = &SP_SINCOS_TAB[0][0];
+    for (int i = threadIdx.x; i < NL + NS; i += blockDim.x) dst[i] = i < NL ? lt[i] : st[i - NL];
+    __syncthreads();
+}
+
 template <class R> struct NormalsT;
 template <> struct NormalsT<double> {
     __device__ __forceinline__ static void philox(uint64_t seed, uint64_t p, uint64_t call, double z[12]) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) box_muller4_f64(noise_block(seed, p, call + b, 0u), z + 4 * b);
+        for (int b = 0; b < 3; ++b)
+            box_muller4_f64(noise_block(seed, p, call + b, 0u), z + 4 * b, sp_bm_lds.log_tab, sp_bm_lds.sc_tab);
     }
 };
 template <> struct NormalsT<float> {

@@ -10,6 +10,7 @@ template <class Cfg, class R, bool KAHAN, bool MID>
 __global__ __launch_bounds__(MAX_BLOCK) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
                                                double* partial, uint32_t pstride) {
     const DevRequest& rq = *rqp;
+    if (sizeof(R) == 8) bm_lds_fill();       // every thread, before any wave leaves
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (wave >= rq.n_tiles) return;
     const TileWork tw{0u, wave, step_begin, step_end, partial, pstride, wave};
@@ -37,6 +38,7 @@ __device__ __forceinline__ uint32_t find_slot(const SlotWork* __restrict__ slots
 template <class Cfg, class R, bool KAHAN, bool MID>
 __global__ __launch_bounds__(MAX_BLOCK) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
                                                uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
+    if (sizeof(R) == 8) bm_lds_fill();
     for (;;) {
         const uint32_t idx = next_work(counter);
         if (idx >= n_tiles) break;

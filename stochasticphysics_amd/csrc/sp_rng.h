@@ -84,7 +84,8 @@ SP_HD double u01_f64(uint32_t w) { return ((double)w + 0.5) * 2.3283064365386963
 // carries into the exponent exactly when the mantissa is >= 3/2, which gives the exponent e of
 // v = m 2^e with m in [3/4, 3/2); j = rint(128 (m - 1)) comes from the round-to-integer constant
 // 1.5 * 2^52, whose sum with 128 (m - 1) holds j in its low word.
-SP_HD double log_u01_f64(uint32_t w) {
+// LT: the ln table (SP_LOG_TAB, or a copy of it in LDS on the device)
+SP_HD double log_u01_f64(uint32_t w, const double (*LT)[2] = SP_LOG_TAB) {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
@@ -97,7 +98,7 @@ SP_HD double log_u01_f64(uint32_t w) {
     const double t = u + RND;
     const double jd = t - RND;                        // j = rint(128 (m - 1)), exact
     const uint32_t row = (uint32_t)__builtin_bit_cast(uint64_t, t) + 32u;   // j + 32 from the low word
-    const double* T = SP_LOG_TAB[row];                // ln c_j, 2^-7 / c_j
+    const double* T = LT[row];                        // ln c_j, 2^-7 / c_j
     const double r = (u - jd) * T[1];                 // (m - c_j) / c_j: 128 (m - c_j) is exact
     double p = 0.14285714285714285;                   // 1/7
     p = fma(p, r, -0.16666666666666666);
@@ -132,12 +133,13 @@ SP_HD double sqrt_radius(double x) {
 }
 
 // ---- FP64 sin and cos of 2 pi u, u = (w + 1/2) 2^-32 ------------------------------
-SP_HD void sincos2pi_u01_f64(uint32_t w, double* s_out, double* c_out) {
+// ST: the sin/cos table (SP_SINCOS_TAB, or a copy of it in LDS on the device)
+SP_HD void sincos2pi_u01_f64(uint32_t w, double* s_out, double* c_out, const double (*ST)[2] = SP_SINCOS_TAB) {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
     constexpr double TWO_PI_2M32 = 0x1.921fb54442d18p-30;   // (nearest double to 2 pi) 2^-32
-    const double* T = SP_SINCOS_TAB[w >> 25];              // sin t_j, cos t_j
+    const double* T = ST[w >> 25];                         // sin t_j, cos t_j
     const int d = (int)(w & 0x1FFFFFFu) - 0x1000000;       // u = (j + 1/2)/128 + (d + 1/2) 2^-32
     const double x = ((double)d + 0.5) * TWO_PI_2M32;      // |x| <= 2 pi / 256
     const double x2 = x * x;
@@ -153,13 +155,13 @@ SP_HD void sincos2pi_u01_f64(uint32_t w, double* s_out, double* c_out) {
     *c_out = T[1] + fma(T[1], cm1, -(T[0] * sx));
 }
 
-// four standard normals (FP64) from one Philox block
-SP_HD void box_muller4_f64(U4 w, double z[4]) {
-    const double r0 = sqrt_radius(-2.0 * log_u01_f64(w.x));
-    const double r1 = sqrt_radius(-2.0 * log_u01_f64(w.z));
+// four standard normals (FP64) from one Philox block; the tables may be given (LDS copies on the device)
+SP_HD void box_muller4_f64(U4 w, double z[4], const double (*LT)[2] = SP_LOG_TAB, const double (*ST)[2] = SP_SINCOS_TAB) {
+    const double r0 = sqrt_radius(-2.0 * log_u01_f64(w.x, LT));
+    const double r1 = sqrt_radius(-2.0 * log_u01_f64(w.z, LT));
     double s0, c0, s1, c1;
-    sincos2pi_u01_f64(w.y, &s0, &c0);
-    sincos2pi_u01_f64(w.w, &s1, &c1);
+    sincos2pi_u01_f64(w.y, &s0, &c0, ST);
+    sincos2pi_u01_f64(w.w, &s1, &c1, ST);
     z[0] = r0 * c0;
     z[1] = r0 * s0;
     z[2] = r1 * c1;
