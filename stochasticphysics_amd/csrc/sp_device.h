@@ -54,8 +54,22 @@ __device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
 // evaluated at t = dt (n+1) (+ dt/2 for ImplicitMidpoint), the reference's
 // time = _timestep*(++counter) (SingleParticleSimulator.h:145) and
 // Implicit_Midpoint.inl:59; the field itself is field_eval (sp_field.h).  The
-// refill copies the parameters with volatile loads so that they are not kept in
-// scalar registers across the loop.
+// refill runs once per 64 steps and is kept out of line, so that the code of every
+// field type stays out of the simulation loop; it copies the parameters with
+// volatile loads so that they are not kept in scalar registers across the loop.
+static __device__ __attribute__((noinline)) V3<double> field_refill(const DevRequest* rq, uint64_t n, int midpoint) {
+    FieldParams f;
+    const volatile double* src = reinterpret_cast<const volatile double*>(&rq->field);
+    double* dst = reinterpret_cast<double*>(&f);
+    for (int i = 0; i < (int)(sizeof(FieldParams) / 8); ++i) dst[i] = src[i];
+    const int lane = threadIdx.x & 63;
+    double t = rq->dt * (double)(n + lane + 1);
+    if (midpoint) t = t + 0.5 * rq->dt;
+    double h[3];
+    field_eval(f, t, h);
+    return {h[0], h[1], h[2]};
+}
+
 struct FieldCache {
     uint64_t base;
     double hx, hy, hz;
@@ -70,16 +84,8 @@ struct FieldCache {
         if (!active) return;
         if (base != ~0ull && n - base + count <= 64) return;
         base = n;
-        FieldParams f;
-        const volatile double* src = reinterpret_cast<const volatile double*>(&rq.field);
-        double* dst = reinterpret_cast<double*>(&f);
-        for (int i = 0; i < (int)(sizeof(FieldParams) / 8); ++i) dst[i] = src[i];
-        const int lane = threadIdx.x & 63;
-        double t = rq.dt * (double)(n + lane + 1);
-        if (MIDPOINT) t = t + 0.5 * rq.dt;
-        double h[3];
-        field_eval(f, t, h);
-        hx = h[0]; hy = h[1]; hz = h[2];
+        const V3<double> h = field_refill(&rq, n, MIDPOINT ? 1 : 0);
+        hx = h.x; hy = h.y; hz = h.z;
     }
     // branch-free: an inactive (zero) field keeps hx = hy = hz = 0 in every lane
     __device__ __forceinline__ V3<double> at(uint64_t n) const {
@@ -369,8 +375,23 @@ template <int ANISO, int SOLVER, class R> struct BNCfg {
 // pi/2 in three parts (k * part exact for |k| < 2^20), Taylor series of sin r to r^17 and cos r to r^16 on
 // |r| <= pi/4 (the next terms are below 1e-16 relative), quadrant from the low bits of k without branches.
 // Larger arguments (never reached by angles that stay near [0, 2 pi]) take the library routine.
+struct SinCos { double s, c; };
+// The library routines for the arguments the inline paths leave out, kept out of line: inlined, their long
+// reductions would be copied into every call site of the simulation loops and crowd the instruction cache.
+static __device__ __attribute__((noinline)) SinCos sincos_library(double x) {
+    SinCos r;
+    sincos(x, &r.s, &r.c);
+    return r;
+}
+static __device__ __attribute__((noinline)) double fmod_library(double x, double y) { return fmod(x, y); }
+
 __device__ __forceinline__ void sincos_fast(double x, double* s_out, double* c_out) {
-    if (!(fabs(x) < 823549.6)) { sincos(x, s_out, c_out); return; }
+    if (!(fabs(x) < 823549.6)) {
+        const SinCos r = sincos_library(x);
+        *s_out = r.s;
+        *c_out = r.c;
+        return;
+    }
     const double k = rint(x * 0x1.45f306dc9c883p-1);          // x 2/pi
     double r = fma(-k, 0x1.921fb54400000p+0, x);              // pi/2 = P1 + P2 + P3
     r = fma(-k, 0x1.0b4611a600000p-34, r);
@@ -402,8 +423,14 @@ __device__ __forceinline__ void sincos_fast(double x, double* s_out, double* c_o
     *c_out = cc;
 }
 
-// std::fmod(x, 2 pi): x itself when |x| < 2 pi (exact), the library routine otherwise
-__device__ __forceinline__ double fmod_2pi(double x) { return fabs(x) < TWO_PI ? x : fmod(x, TWO_PI); }
+// std::fmod(x, 2 pi): x itself when |x| < 2 pi; x -+ 2 pi when 2 pi <= |x| < 4 pi (exact by Sterbenz's lemma,
+// and exactly what fmod returns there); the library routine otherwise
+__device__ __forceinline__ double fmod_2pi(double x) {
+    const double a = fabs(x);
+    if (a < TWO_PI) return x;
+    if (a < 2.0 * TWO_PI) return x > 0.0 ? x - TWO_PI : x + TWO_PI;
+    return fmod_library(x, TWO_PI);
+}
 
 template <class R> struct Trig { R st, ct, sp, cp; };
 __device__ __forceinline__ void trig_of(double th, double ph, Trig<double>& T) {
@@ -584,7 +611,7 @@ template <int N> __device__ __forceinline__ Dual<N> datan2(const Dual<N>& y, con
     for (int i = 0; i < N; ++i) r.d[i] = (x.v * y.d[i] - y.v * x.d[i]) * den;
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> dfmod(const Dual<N>& a, double m) { Dual<N> r = a; r.v = fmod(a.v, m); return r; }
+template <int N> __device__ __forceinline__ Dual<N> dfmod_2pi(const Dual<N>& a) { Dual<N> r = a; r.v = fmod_2pi(a.v); return r; }
 
 template <int N> struct DV3 { Dual<N> x, y, z; };
 template <int N> __device__ __forceinline__ DV3<N> operator+(const DV3<N>& a, const DV3<N>& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -757,7 +784,7 @@ template <int ANISO> struct IMSph {
             dsincos(q0, s0, c0); dsincos(q1, s1, c1);
             const Dual<2> n0 = dacos(c1 * s0), n1 = datan2(s1 * s0, -c0);
             q0 = n0; q1 = n1;
-        } else { q0 = dfmod(q0, TWO_PI); q1 = dfmod(q1, TWO_PI); }
+        } else { q0 = dfmod_2pi(q0); q1 = dfmod_2pi(q1); }
         const Dual<2> f0 = q0 + Dual<2>(0.5 * yi[0]) - 1.5 * y0;
         const Dual<2> f1 = q1 + Dual<2>(0.5 * yi[1]) - 1.5 * y1;
         F[0] = f0.v; F[1] = f1.v;
