@@ -502,7 +502,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         Y[0] = y[0]; Y[1] = y[1];
         if (rot) {
             double st, ct, sp, cp;
-            sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
+            if constexpr (sizeof(R) == 8) { st = T.st; ct = T.ct; sp = T.sp; cp = T.cp; }   // the cached trig of y
+            else { sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp); }
             Y[0] = acos(-cp * st);
             Y[1] = atan2(sp * st, ct);
             Trig<double> Td;

@@ -31,6 +31,15 @@ template <class R> __device__ __forceinline__ void trig5_exact(const double* y, 
 #pragma unroll
     for (int i = 0; i < 5; ++i) { T.s[i] = (R)Td.s[i]; T.c[i] = (R)Td.c[i]; }
 }
+// the same for the angles [I0, I1) only
+template <int I0, int I1, class R> __device__ __forceinline__ void trig_exact(const double* y, Trig5<R>& T) {
+#pragma unroll
+    for (int i = I0; i < I1; ++i) {
+        double sd, cd;
+        sincos_fast(y[i], &sd, &cd);
+        T.s[i] = (R)sd; T.c[i] = (R)cd;
+    }
+}
 
 template <int ANISO, class R> struct BNESLane {
     R a, bpd, c, d;              // NeelFactor1, b + d, BrownPrefactor, d = c mu
@@ -232,12 +241,15 @@ __device__ __forceinline__ void bnes_noise(const BNESLane<ANISO, R>& L, const BC
 
 // EulertoEulerRotated (:859-877), EulerRotatedtoEuler (:880-898), the 90 degree chart maps of the Neel angles
 // (:823-834, :844-855); all in double
-__device__ __forceinline__ void euler_to_rotated(const double* y, double* r) {
-    double S0, C0, S1, C1, S2, C2;
-    sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
+__device__ __forceinline__ void euler_to_rotated(double S0, double C0, double S1, double C1, double S2, double C2, double* r) {
     r[0] = atan2(-C2 * C0 + C1 * S0 * S2, S0 * C2 + C1 * C0 * S2);
     r[1] = acos(-S1 * S2);
     r[2] = atan2(C1, C2 * S1);
+}
+__device__ __forceinline__ void euler_to_rotated(const double* y, double* r) {
+    double S0, C0, S1, C1, S2, C2;
+    sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
+    euler_to_rotated(S0, C0, S1, C1, S2, C2, r);
 }
 __device__ __forceinline__ void rotated_to_euler(const double* y, double* r) {
     double S0, C0, S1, C1, S2, C2;
@@ -246,11 +258,14 @@ __device__ __forceinline__ void rotated_to_euler(const double* y, double* r) {
     r[1] = acos(S1 * S2);
     r[2] = atan2(-C1, C2 * S1);
 }
+__device__ __forceinline__ void neel_to_rotated(double st, double ct, double sp, double cp, double* r) {
+    r[0] = acos(-cp * st);
+    r[1] = atan2(sp * st, ct);
+}
 __device__ __forceinline__ void neel_to_rotated(const double* y, double* r) {
     double st, ct, sp, cp;
     sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
-    r[0] = acos(-cp * st);
-    r[1] = atan2(sp * st, ct);
+    neel_to_rotated(st, ct, sp, cp, r);
 }
 __device__ __forceinline__ void rotated_to_neel(const double* y, double* r) {
     double st, ct, sp, cp;
@@ -285,9 +300,17 @@ template <int ANISO, int SOLVER, class R> struct BNESCfg {
         nrot = fabs(y[3]) < rq.tmin || fabs(y[3]) > rq.tmax;
 #pragma unroll
         for (int i = 0; i < 5; ++i) Y[i] = y[i];
-        if (brot) euler_to_rotated(y, Y);
-        if (nrot) neel_to_rotated(y + 3, Y + 3);
-        if (brot || nrot) trig5_exact(Y, T);
+        // in FP64 the cache holds exactly the sines and cosines of y the chart maps need
+        if (brot) {
+            if constexpr (sizeof(R) == 8) euler_to_rotated(T.s[0], T.c[0], T.s[1], T.c[1], T.s[2], T.c[2], Y);
+            else euler_to_rotated(y, Y);
+            trig_exact<0, 3>(Y, T);
+        }
+        if (nrot) {
+            if constexpr (sizeof(R) == 8) neel_to_rotated(T.s[3], T.c[3], T.s[4], T.c[4], Y + 3);
+            else neel_to_rotated(y + 3, Y + 3);
+            trig_exact<3, 5>(Y, T);
+        }
     }
     // finishCalculations (:635-665), then the trig of the new state
     __device__ __forceinline__ static void finish(const double r[5], bool brot, bool nrot, double* y, Trig5<R>& T) {
