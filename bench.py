@@ -208,6 +208,7 @@ def precision_study(ctx, configs, n, n_steps):
             log(f"bench: precision study {solver} {prec}: {row[prec]:.3e} particle-steps/s ({dt:.1f} s)")
         d = res["mixed"]["mean"] - res["fp64"]["mean"]
         row.update(err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
+                   err_bias_max=float(np.abs(d[1:].mean(axis=0)).max()),     # time-averaged difference per component
                    speedup=row["mixed"] / row["fp64"], monte_carlo_sigma_over_sqrt_n=float(0.58 / math.sqrt(n)))
         log(f"bench: precision study {solver}: mixed - FP64 <m> max {row['err_max']:.2e} rms {row['err_rms']:.2e}")
         out[solver] = row
