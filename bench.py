@@ -148,7 +148,8 @@ def mixed_leg(ctx, configs, n_sets, seed=44):
                 check_noise_max=float(max(r["check_noise"] for r in res)), fp64_fallbacks=fb,
                 note="all sets renormalised after every step on both sides: over up to 5e5 steps the reference's "
                      "EulerMaruyama and Heun_Strong let |m| run away in some particles; the per-output error is "
-                     "Monte-Carlo decorrelation, see DESIGN.md")
+                     "Monte-Carlo decorrelation (trajectories that differ by rounding decorrelate), so the "
+                     "time-averaged bias is the precision figure")
 
 
 def serving_leg(ctx, configs, seconds, rate, seed=55):

@@ -5,7 +5,7 @@
 //   Millstein.inl:34-44, Implicit_Midpoint.inl:38-188,
 //   Simulator/SingleParticleSimulator.h:129-159 and :236-273,
 //   Results/MeanSimulationResult.h:63-119.
-// Engine decisions where the reference has no working path (documented in DESIGN.md):
+// Engine decisions where the reference has no working path:
 //   * Heun_Strong and Millstein take xi = field(t_{n+1}), like EulerMaruyama.
 //   * NeelSpherical under Heun_Strong / Millstein: the chart (rotated or not) is
 //     chosen once from y_i, every stage of the step is evaluated in that chart,

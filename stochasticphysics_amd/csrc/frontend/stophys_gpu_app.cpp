@@ -1,7 +1,7 @@
 // stophys_gpu_app.cpp -- command line front-end with the options of the reference's
 // StoPhysApp_MultiArch (app/InputParams.cpp:46-76; app/StoPhysApp_MultiArch.cpp:27-93):
 //   --parameter_file=FILE     the SimulationManagerSettings .ini file
-//   --instruction_set=GFX950  the only value this build runs (the GPU backend)
+//   --instruction_set=GFX950  the only value this program runs (the GPU backend)
 //   --result_file=FILE        where Mean_Results are written (default: Savefile with .json)
 //   --device=N                GPU index
 #include <chrono>
