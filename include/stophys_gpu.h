@@ -288,7 +288,9 @@ void sp_settings_free(sp_settings* s);
 int sp_settings_prepare(sp_ctx* ctx, const sp_settings* s, sp_prepared** out);
 int sp_settings_submit(sp_ctx* ctx, const sp_settings* s, uint64_t* id);
 /* copies a prepared request's resident inputs to the host: params [NPARAM = 23][n_particles] (param-major,
- * the rows of sp_compute_particle_params), initial_state [n_particles][dim_y]; either may be NULL */
+ * the rows of sp_compute_particle_params), initial_state [n_particles][dim_y]; either may be NULL.  No
+ * counterpart in the reference (its particles live in its Problem objects): it lets a caller inspect what the
+ * device provider drew */
 int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_state);
 
 /* ---- autotuner: times candidate tunings on a short synthetic run ---- */
