@@ -194,7 +194,7 @@ def test_request_validation_errors():
     assert (n_out.value, dy.value, dw.value, do.value) == (10, 3, 3, 3)
     out = np.zeros(capi.NPARAM * 8)
     n_par = C.c_uint32()
-    r.n_particles = 0  # an empty ensemble is an error, as the reference needs Number_of_simulations >= 1
+    r.n_particles = 0  # an empty ensemble is rejected with INVALID_ARG
     assert lib().sp_compute_particle_params(C.byref(r), C.byref(p), capi.dptr(out), C.byref(n_par)) == 1
     assert "n_particles" in lib().sp_last_error().decode()
     r.n_particles = 8
