@@ -17,6 +17,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -54,7 +55,9 @@ struct Settings {
     std::vector<ParticleFile> particles;
     std::string savefile, single_savefile, single_prefix = "Single";
     std::shared_ptr<const ProvSpec> spec;   // the particle types as the provider draws them
-    // host arrays, materialised on first use
+    // host arrays, materialised on first use (under mu: the C API hands out const handles, which callers
+    // may share between threads)
+    std::mutex mu;
     bool materialised = false;
     std::vector<double> radius, Ms, alpha, gamma, T, K, r_h, visc, axes, y0, K2, urot;
 };
@@ -192,6 +195,7 @@ static std::shared_ptr<const ProvSpec> build_spec(const Settings& s) {
 
 // the N particles as host arrays (Provider::getProvidedObject, ParticleSimulationSettings::applySettingsToParticleProperties)
 static void materialise(Settings& s) {
+    std::lock_guard<std::mutex> lk(s.mu);
     if (s.materialised) return;
     const uint64_t N = s.rq.n_particles;
     uint32_t dy = 0, dw = 0, dout = 0;
@@ -214,9 +218,14 @@ static void materialise(Settings& s) {
 }
 
 // particle p's properties: from the host arrays when they exist, else drawn on the host for this particle
-static ParticleProps particle_at(const Settings& s, uint64_t p) {
+static ParticleProps particle_at(Settings& s, uint64_t p) {
     ParticleProps q{};
-    if (s.materialised) {
+    bool have;
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        have = s.materialised;
+    }
+    if (have) {
         q.radius = s.radius[p]; q.Ms = s.Ms[p]; q.alpha = s.alpha[p]; q.gamma = s.gamma[p]; q.T = s.T[p];
         q.K = s.K[p]; q.r_h = s.r_h[p]; q.visc = s.visc[p];
     } else {
@@ -359,7 +368,7 @@ int write_result(const Settings& s, const sp_result& res, const std::string& pat
 // one named entry per kept particle with its properties, Time, Weight, Field and its own (unweighted)
 // window means.  The entry of particle p is named <Save_single_name_prefix>_<p + 1>: the reference numbers
 // its entries by completion order, which has no fixed meaning; here the number is the particle's.
-int write_single_results(const Settings& s, const sp_result& res, const std::string& path) {
+int write_single_results(Settings& s, const sp_result& res, const std::string& path) {
     if (!res.trajectories || res.n_saved == 0) return set_error(SP_ERR_INVALID_ARG, "the result holds no single-simulation outputs");
     std::ofstream o(path);
     if (!o) return set_error(SP_ERR_INVALID_ARG, "cannot write " + path);
