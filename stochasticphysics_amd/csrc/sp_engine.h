@@ -304,7 +304,9 @@ struct Outputs {
     bool check_failed = false;
 };
 
-inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t s, bool keep_final, bool want_check, Outputs& o) {
+// axes: the request's time and field axes when the caller has them already (a prepared request executed again)
+inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t s, bool keep_final, bool want_check, Outputs& o,
+                            const Outputs* axes = nullptr) {
     const uint64_t M = hr.n_out * hr.dout;
     std::vector<double> sums(M);
     SP_HIP(hipMemcpyAsync(sums.data(), d.out, M * 8, hipMemcpyDeviceToHost, s));
@@ -335,9 +337,14 @@ inline int download_outputs(const HostRequest& hr, const ReqDev& d, hipStream_t 
     for (uint64_t k = 0; k < hr.n_out; ++k)
         for (uint32_t c = 0; c < hr.dout; ++c) o.mean[k * hr.dout + c] = zero ? sums[k * hr.dout + c] : sums[k * hr.dout + c] / hr.wsum[c];
     o.weight.assign(hr.wsum, hr.wsum + hr.dout);
-    o.time.resize(hr.n_out);
-    o.field.resize(hr.n_out * 3);
-    time_and_field(hr.rq, o.time.data(), o.field.data());
+    if (axes && axes->time.size() == hr.n_out) {
+        o.time = axes->time;
+        o.field = axes->field;
+    } else {
+        o.time.resize(hr.n_out);
+        o.field.resize(hr.n_out * 3);
+        time_and_field(hr.rq, o.time.data(), o.field.data());
+    }
     o.particle_steps = (double)hr.N * (double)hr.total_steps;
     o.precision_used = hr.mixed ? SP_PREC_MIXED : SP_PREC_FP64;
     return SP_OK;
@@ -500,6 +507,7 @@ struct Prepared {
     ReqDev d;
     double* state0 = nullptr;     // device copy of the initial state
     bool keep_final = false;
+    Outputs axes;                 // time and field axes, computed at the first execution
 };
 void free_prepared(Prepared& pr);
 int prepare_plain(sp_ctx* ctx, Prepared& pr);

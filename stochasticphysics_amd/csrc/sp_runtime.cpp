@@ -72,21 +72,31 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     std::vector<hipEvent_t> ev(2 * n_chunks + 2);
     for (auto& e : ev) SP_HIP(hipEventCreate(&e));
     struct EvGuard { std::vector<hipEvent_t>* v; ~EvGuard() { for (auto e : *v) hipEventDestroy(e); } } evg{&ev};
+    // the reduction work of every chunk goes to the device in one copy, ahead of the launches
+    std::vector<uint32_t> chunk_rows(n_chunks);
+    {
+        uint64_t sb = 0;
+        for (uint64_t k = 0; k < n_chunks; ++k) {
+            const uint64_t se = std::min(sb + C, hr.total_steps);
+            const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
+            const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
+            chunk_rows[k] = (uint32_t)(rows * hr.dout);
+            rw_host[k] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, chunk_rows[k],
+                                    (uint32_t)(first_window * hr.dout), d.out, d.check};
+            sb = se;
+        }
+    }
+    SP_HIP(hipMemcpyAsync(rw_dev, rw_host, n_chunks * sizeof(ReduceWork), hipMemcpyHostToDevice, s));
     SP_HIP(hipEventRecord(ev[2 * n_chunks], s));
     uint64_t sb = 0;
     for (uint64_t k = 0; k < n_chunks; ++k) {
         const uint64_t se = std::min(sb + C, hr.total_steps);
-        const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
-        const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
         SP_HIP(hipEventRecord(ev[2 * k], s));
         if (!launch_plain(hr.kind, grid, dim3(block), s, d.dreq, sb, se, ctx->partial, hr.n_tiles))
             return set_error(SP_ERR_UNSUPPORTED, "no kernel for this problem/solver/anisotropy/precision");
         SP_HIP(hipGetLastError());
         SP_HIP(hipEventRecord(ev[2 * k + 1], s));
-        rw_host[k] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, (uint32_t)(rows * hr.dout),
-                                (uint32_t)(first_window * hr.dout), d.out, d.check};
-        SP_HIP(hipMemcpyAsync(rw_dev + k, rw_host + k, sizeof(ReduceWork), hipMemcpyHostToDevice, s));
-        launch_reduce(s, rw_dev + k, 1, (uint32_t)(rows * hr.dout));
+        launch_reduce(s, rw_dev + k, 1, chunk_rows[k]);
         SP_HIP(hipGetLastError());
         sb = se;
     }
@@ -100,7 +110,8 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     }
     float total_ms = 0.f;
     SP_HIP(hipEventElapsedTime(&total_ms, ev[2 * n_chunks], ev[2 * n_chunks + 1]));
-    if (int e = download_outputs(hr, d, s, pr.keep_final, false, o)) return e;
+    if (int e = download_outputs(hr, d, s, pr.keep_final, false, o, &pr.axes)) return e;
+    if (pr.axes.time.size() != hr.n_out) { pr.axes.time = o.time; pr.axes.field = o.field; }
     o.device_ms = total_ms;
     o.latency_ms = ms_since(t0, Clock::now());
     KernelInfo ki{};
