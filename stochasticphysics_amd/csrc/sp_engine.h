@@ -508,6 +508,7 @@ struct Prepared {
     double* state0 = nullptr;     // device copy of the initial state
     bool keep_final = false;
     Outputs axes;                 // time and field axes, computed at the first execution
+    std::vector<hipEvent_t> events;   // per-chunk timing events, created at the first execution
 };
 void free_prepared(Prepared& pr);
 int prepare_plain(sp_ctx* ctx, Prepared& pr);

@@ -23,6 +23,8 @@ int set_error(int code, const std::string& msg) {
 // A Prepared request keeps its parameters and initial state resident in HBM, so
 // it can be executed repeatedly without host-to-device traffic.
 void free_prepared(Prepared& pr) {
+    for (auto e : pr.events) hipEventDestroy(e);
+    pr.events.clear();
     free_dev(pr.d);
     if (pr.state0) hipFree(pr.state0);
     pr.state0 = nullptr;
@@ -69,9 +71,17 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, tun.block_size ? tun.block_size / 64 * 64 : MAX_BLOCK));
     const uint32_t waves_per_block = block / 64;
     const dim3 grid((hr.n_tiles + waves_per_block - 1) / waves_per_block);
-    std::vector<hipEvent_t> ev(2 * n_chunks + 2);
-    for (auto& e : ev) SP_HIP(hipEventCreate(&e));
-    struct EvGuard { std::vector<hipEvent_t>* v; ~EvGuard() { for (auto e : *v) hipEventDestroy(e); } } evg{&ev};
+    // timing events, kept with the prepared request so that repeated executions do not create them again
+    if (pr.events.size() != 2 * n_chunks + 2) {
+        for (auto e : pr.events) hipEventDestroy(e);
+        pr.events.clear();
+        for (uint64_t k = 0; k < 2 * n_chunks + 2; ++k) {
+            hipEvent_t e;
+            SP_HIP(hipEventCreate(&e));
+            pr.events.push_back(e);
+        }
+    }
+    const std::vector<hipEvent_t>& ev = pr.events;
     // the reduction work of every chunk goes to the device in one copy, ahead of the launches
     std::vector<uint32_t> chunk_rows(n_chunks);
     {
