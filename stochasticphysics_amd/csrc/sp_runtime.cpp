@@ -63,7 +63,11 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     if (tun.chunk_steps && tun.chunk_steps < hr.total_steps) C = std::min<uint64_t>(C, std::max<uint64_t>(align, tun.chunk_steps / align * align));
     C = std::min<uint64_t>(C, std::max<uint64_t>(hr.total_steps, align));
     const uint64_t rows_max = C / OS + 1;
-    if (int e = ensure_partial(ctx, rows_max * row_bytes)) return e;
+    // when the partial sums of the whole pass fit 4 GiB, every chunk writes its own rows and one reduction
+    // runs at the end (one kernel boundary per chunk instead of two); otherwise each chunk is reduced
+    // before the next one reuses the buffer
+    const bool whole = hr.n_out * row_bytes <= (4ull << 30);
+    if (int e = ensure_partial(ctx, whole ? hr.n_out * row_bytes : rows_max * row_bytes)) return e;
     const uint64_t n_chunks = std::max<uint64_t>(1, (hr.total_steps + C - 1) / C);
     if (int e = ensure_table(ctx, n_chunks * sizeof(ReduceWork) + 256)) return e;
     ReduceWork* rw_host = (ReduceWork*)ctx->pinned;
@@ -82,8 +86,9 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
         }
     }
     const std::vector<hipEvent_t>& ev = pr.events;
-    // the reduction work of every chunk goes to the device in one copy, ahead of the launches
+    // the reduction work goes to the device in one copy, ahead of the launches
     std::vector<uint32_t> chunk_rows(n_chunks);
+    std::vector<double*> chunk_partial(n_chunks, ctx->partial);
     {
         uint64_t sb = 0;
         for (uint64_t k = 0; k < n_chunks; ++k) {
@@ -91,24 +96,33 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
             const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
             const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
             chunk_rows[k] = (uint32_t)(rows * hr.dout);
-            rw_host[k] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, chunk_rows[k],
-                                    (uint32_t)(first_window * hr.dout), d.out, d.check};
+            if (whole) chunk_partial[k] = ctx->partial + first_window * hr.dout * hr.n_tiles;
+            else rw_host[k] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, chunk_rows[k],
+                                         (uint32_t)(first_window * hr.dout), d.out, d.check};
             sb = se;
         }
+        if (whole) rw_host[0] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, (uint32_t)(hr.n_out * hr.dout), 0u,
+                                           d.out, d.check};
     }
-    SP_HIP(hipMemcpyAsync(rw_dev, rw_host, n_chunks * sizeof(ReduceWork), hipMemcpyHostToDevice, s));
+    SP_HIP(hipMemcpyAsync(rw_dev, rw_host, (whole ? 1 : n_chunks) * sizeof(ReduceWork), hipMemcpyHostToDevice, s));
     SP_HIP(hipEventRecord(ev[2 * n_chunks], s));
     uint64_t sb = 0;
     for (uint64_t k = 0; k < n_chunks; ++k) {
         const uint64_t se = std::min(sb + C, hr.total_steps);
         SP_HIP(hipEventRecord(ev[2 * k], s));
-        if (!launch_plain(hr.kind, grid, dim3(block), s, d.dreq, sb, se, ctx->partial, hr.n_tiles))
+        if (!launch_plain(hr.kind, grid, dim3(block), s, d.dreq, sb, se, chunk_partial[k], hr.n_tiles))
             return set_error(SP_ERR_UNSUPPORTED, "no kernel for this problem/solver/anisotropy/precision");
         SP_HIP(hipGetLastError());
         SP_HIP(hipEventRecord(ev[2 * k + 1], s));
-        launch_reduce(s, rw_dev + k, 1, chunk_rows[k]);
-        SP_HIP(hipGetLastError());
+        if (!whole) {
+            launch_reduce(s, rw_dev + k, 1, chunk_rows[k]);
+            SP_HIP(hipGetLastError());
+        }
         sb = se;
+    }
+    if (whole) {
+        launch_reduce(s, rw_dev, 1, (uint32_t)(hr.n_out * hr.dout));
+        SP_HIP(hipGetLastError());
     }
     SP_HIP(hipEventRecord(ev[2 * n_chunks + 1], s));
     SP_HIP(hipStreamSynchronize(s));
