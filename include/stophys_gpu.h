@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 6
+#define SP_ABI_VERSION 7
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -227,6 +227,13 @@ int sp_ctx_get_tuning(sp_ctx* ctx, int problem, int solver, int precision, sp_tu
 /* device memory budget for admitted requests (state + params + outputs) */
 int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
 
+/* plain path (sp_run, sp_execute): the largest partial-sum buffer used to reduce a whole pass at once
+ * (default 4 GiB, and at most a quarter of the free device memory); a pass whose per-window partial sums
+ * exceed it, or whose buffer cannot be allocated, is reduced chunk by chunk in a buffer of one chunk's rows.
+ * Results do not depend on it.  No reference counterpart (the reference merges per-particle trajectories
+ * under a mutex, SimulationManager.h:170-175). */
+int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes);
+
 /* statistics of the last sp_run on this context: [0] kernel launches, [1] kernel ms,
  * [2] reduction + copy ms, [3] wall ms, [4] particle-steps, [5] VGPRs of the kernel,
  * [6] grid size (work-groups), [7] work-group size */
@@ -309,6 +316,9 @@ int sp_host_normals(uint64_t seed, uint64_t particle, uint64_t first, uint64_t c
  * FP64 (bit-identical to sp_host_box_muller) or the mixed-precision (FP32 hardware) form */
 int sp_dev_box_muller(sp_ctx* ctx, const uint32_t* words, uint64_t n_blocks, int precision, double* out);
 int sp_host_box_muller(const uint32_t* words, uint64_t n_blocks, double* out);
+/* the request server's next n_iterations scheduler iterations fail on the host, before their launches, with
+ * SP_ERR_DEVICE: exercises the error path of sp_submit/sp_wait/sp_result_get without a device fault */
+int sp_ctx_inject_failures(sp_ctx* ctx, uint32_t n_iterations);
 
 #ifdef __cplusplus
 }

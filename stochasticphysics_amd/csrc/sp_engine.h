@@ -161,14 +161,12 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     for (uint64_t p = 0; p < hr.N; ++p)
         for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
     const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
-    for (uint32_t c = 0; c < hr.dout; ++c) { hr.wsum[c] = 0.0; hr.wsum_check[c] = 0.0; }
-    for (uint64_t p = 0; p < hr.N; ++p) {
-        const double mu = hr.params[P_MU * hr.N + p];
-        for (uint32_t c = 0; c < hr.dout; ++c) {
-            const double w = (hr.dout == 6 && c < 3) ? 1.0 : mu;   // BrownAndNeelRelaxation.h:152-159; all mu otherwise
-            hr.wsum[c] += w;
-            if (p < n_check) hr.wsum_check[c] += w;
-        }
+    const double* mu = &hr.params[P_MU * hr.N];
+    const double mu_sum = weight_sum_fixed(mu, hr.N), mu_sum_check = weight_sum_fixed(mu, n_check);
+    for (uint32_t c = 0; c < hr.dout; ++c) {
+        const bool unit = hr.dout == 6 && c < 3;    // BrownAndNeelRelaxation.h:152-159: weight 1; all mu otherwise
+        hr.wsum[c] = unit ? (double)hr.N : mu_sum;
+        hr.wsum_check[c] = unit ? (double)n_check : mu_sum_check;
     }
     const double* props[8] = {pt->temperature, pt->magnetic_radius, pt->saturation_magnetisation, pt->damping,
                               pt->gyromagnetic_ratio, pt->anisotropy_constant, pt->hydrodynamic_radius, pt->viscosity};
@@ -234,6 +232,7 @@ int run_provider(HostRequest& hr, ReqDev& d, hipStream_t s);   // sp_runtime.cpp
 // the provider kernel (which also fills the weight and property sums of hr)
 inline int alloc_dev(HostRequest& hr, ReqDev& d, hipStream_t s) {
     auto fail = [&](const char* what, hipError_t e) {
+        (void)hipGetLastError();      // clear the failed allocation's error
         free_dev(d);
         return set_error(e == hipErrorOutOfMemory ? SP_ERR_OUT_OF_MEMORY : SP_ERR_DEVICE,
                          std::string("allocating ") + what + ": " + hipGetErrorString(e));
@@ -440,6 +439,11 @@ struct sp_ctx {
     size_t pinned_bytes = 0;
     uint32_t* counters = nullptr;           // per-launch work counters
     size_t n_counters = 0;
+    // plain path: largest partial-sum buffer for reducing a whole pass at once (larger passes are reduced
+    // chunk by chunk)
+    uint64_t whole_pass_limit = 4ull << 30;
+    // server: the next n iterations fail before their launches (a host-side hook for the error paths)
+    uint32_t inject_failures = 0;
     // last synchronous run statistics
     double last_stats[8] = {0};
 };
@@ -470,7 +474,11 @@ inline int ensure_partial(sp_ctx* ctx, size_t bytes) {
     ctx->partial = nullptr;
     ctx->partial_bytes = 0;
     hipError_t e = hipMalloc(&ctx->partial, bytes);
-    if (e != hipSuccess) return set_error(SP_ERR_OUT_OF_MEMORY, std::string("partial buffer: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();      // a failed allocation must not be reported by a later launch check
+        ctx->partial = nullptr;
+        return set_error(SP_ERR_OUT_OF_MEMORY, std::string("partial buffer: ") + hipGetErrorString(e));
+    }
     ctx->partial_bytes = bytes;
     return SP_OK;
 }

@@ -54,32 +54,13 @@ bool kernel_info(uint32_t kind, KernelInfo* info) {
     return false;
 }
 
-// Fixed-order reduction: thread t sums slots t, t+256, ... in order, then a
-// fixed LDS tree; the result depends only on the partial values, not on how
-// the tiles were scheduled.
 __global__ __launch_bounds__(256) void k_reduce(const ReduceWork* __restrict__ work) {
-    const ReduceWork rw = work[blockIdx.y];
-    const uint32_t r = blockIdx.x;
-    if (r >= rw.n_rows) return;
-    __shared__ double sh[256];
-    const double* row = rw.partial + (uint64_t)r * rw.pstride;
-    double s = 0.0;
-    for (uint32_t i = threadIdx.x; i < rw.pstride; i += 256) s += row[i];
-    // the check tiles are kept one by one (n_check <= 64 <= pstride)
-    if (rw.check_out && threadIdx.x < rw.n_check)
-        rw.check_out[(uint64_t)(rw.row_base + r) * rw.n_check + threadIdx.x] = row[threadIdx.x];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (uint32_t w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) rw.out[rw.row_base + r] = sh[0];
+    reduce_rows(work[blockIdx.y]);
 }
 
 void launch_reduce(hipStream_t s, const ReduceWork* work, uint32_t n_work, uint32_t max_rows) {
     if (!n_work || !max_rows) return;
-    k_reduce<<<dim3(max_rows, n_work), dim3(256), 0, s>>>(work);
+    k_reduce<<<dim3(reduce_grid_x(max_rows), n_work), dim3(256), 0, s>>>(work);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {

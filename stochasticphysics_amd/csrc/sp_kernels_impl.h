@@ -49,6 +49,30 @@ __global__ __launch_bounds__(MAX_BLOCK) void k_batch(const DevRequest* __restric
     }
 }
 
+// Fixed-order reduction of one ReduceWork entry by one 256-thread block: thread t sums slots t, t+256, ... in
+// order, then a fixed LDS tree; the result depends only on the partial values, not on how the tiles were
+// scheduled.  The block reduces rows blockIdx.x, blockIdx.x + gridDim.x, ... (the row index is uniform over the
+// block, so every thread reaches every barrier).
+__device__ __forceinline__ void reduce_rows(const ReduceWork& rw) {
+    __shared__ double sh[256];
+    for (uint32_t r = blockIdx.x; r < rw.n_rows; r += gridDim.x) {
+        const double* row = rw.partial + (uint64_t)r * rw.pstride;
+        double s = 0.0;
+        for (uint32_t i = threadIdx.x; i < rw.pstride; i += 256) s += row[i];
+        // the check tiles are kept one by one (n_check <= 64 <= pstride)
+        if (rw.check_out && threadIdx.x < rw.n_check)
+            rw.check_out[(uint64_t)(rw.row_base + r) * rw.n_check + threadIdx.x] = row[threadIdx.x];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (uint32_t w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) rw.out[rw.row_base + r] = sh[0];
+        __syncthreads();     // sh is written again by the next row
+    }
+}
+
 template <class Cfg, class R, bool KAHAN, bool MID>
 bool fill_info(KernelInfo* info, const char* name) {
     hipFuncAttributes a{};

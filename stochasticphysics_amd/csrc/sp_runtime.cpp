@@ -63,11 +63,19 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     if (tun.chunk_steps && tun.chunk_steps < hr.total_steps) C = std::min<uint64_t>(C, std::max<uint64_t>(align, tun.chunk_steps / align * align));
     C = std::min<uint64_t>(C, std::max<uint64_t>(hr.total_steps, align));
     const uint64_t rows_max = C / OS + 1;
-    // when the partial sums of the whole pass fit 4 GiB, every chunk writes its own rows and one reduction
-    // runs at the end (one kernel boundary per chunk instead of two); otherwise each chunk is reduced
-    // before the next one reuses the buffer
-    const bool whole = hr.n_out * row_bytes <= (4ull << 30);
-    if (int e = ensure_partial(ctx, whole ? hr.n_out * row_bytes : rows_max * row_bytes)) return e;
+    // when the partial sums of the whole pass fit the context's reduction limit (4 GiB by default) and a
+    // quarter of the free device memory, every chunk writes its own rows and one reduction runs at the end
+    // (one kernel boundary per chunk instead of two); otherwise, or when that buffer cannot be allocated,
+    // each chunk is reduced before the next one reuses a buffer of one chunk's rows
+    bool whole = hr.n_out * row_bytes <= ctx->whole_pass_limit && hr.n_out * hr.dout < (1ull << 32);
+    if (whole && hr.n_out * row_bytes > ctx->partial_bytes) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || hr.n_out * row_bytes > free_b / 4) whole = false;
+    }
+    if (whole && ensure_partial(ctx, hr.n_out * row_bytes) != SP_OK) whole = false;
+    if (!whole) {
+        if (int e = ensure_partial(ctx, rows_max * row_bytes)) return e;
+    }
     const uint64_t n_chunks = std::max<uint64_t>(1, (hr.total_steps + C - 1) / C);
     if (int e = ensure_table(ctx, n_chunks * sizeof(ReduceWork) + 256)) return e;
     ReduceWork* rw_host = (ReduceWork*)ctx->pinned;
@@ -293,6 +301,20 @@ int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->mem_budget = bytes;
     ctx->max_active = max_active ? max_active : 1;
+    return SP_OK;
+}
+
+int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes) {
+    if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    ctx->whole_pass_limit = bytes;
+    return SP_OK;
+}
+
+int sp_ctx_inject_failures(sp_ctx* ctx, uint32_t n_iterations) {
+    if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->inject_failures = n_iterations;
     return SP_OK;
 }
 

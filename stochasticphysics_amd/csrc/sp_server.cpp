@@ -410,8 +410,13 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
 
 // ------------------------------------------------------------------ server worker
 static void finish_job(sp_ctx* ctx, Slot* sl) {
-    // called by the worker without ctx->mu; publishes under the lock
+    // called by the worker without ctx->mu; publishes under the lock.  A request that has already ended
+    // (failed, cancelled) is never published again.
     std::shared_ptr<Job> job = sl->job;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (job->status != SP_ERR_NOT_READY) return;
+    }
     Outputs o;
     int rc = download_outputs(*sl->hr, sl->dev, ctx->stream, job->keep_final, sl->shadow != nullptr, o);
     bool requeue = false;
@@ -463,6 +468,7 @@ static void worker_main(sp_ctx* ctx) {
     std::vector<std::unique_ptr<Slot>> active;
     for (;;) {
         std::vector<std::shared_ptr<Job>> admitted;
+        std::vector<size_t> admitted_need;
         {
             std::unique_lock<std::mutex> lk(ctx->mu);
             ctx->cv.wait(lk, [&] { return ctx->stop || !ctx->pending.empty() || !active.empty(); });
@@ -492,12 +498,14 @@ static void worker_main(sp_ctx* ctx) {
                 for (auto& a : active) if (!a->is_shadow && !a->done) ++n_active;
                 if (n_active > 0 && (ctx->mem_used + need > ctx->mem_budget || n_active >= ctx->max_active)) break;
                 admitted.push_back(j);
+                admitted_need.push_back(need);
                 ctx->mem_used += need;
                 ctx->pending.pop_front();
             }
         }
         std::unique_lock<std::mutex> device_lock(ctx->run_mu);   // the worker owns the stream and scratch from here
-        for (auto& j : admitted) {
+        for (size_t ai = 0; ai < admitted.size(); ++ai) {
+            auto& j = admitted[ai];
             auto sl = std::make_unique<Slot>();
             sl->job = j;
             sl->hr = &j->hr;
@@ -522,6 +530,7 @@ static void worker_main(sp_ctx* ctx) {
             if (rc != SP_OK) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 j->status = rc;
+                ctx->mem_used -= std::min<uint64_t>(ctx->mem_used, admitted_need[ai]);   // never admitted
                 ctx->done_cv.notify_all();
                 free_dev(sl->dev);
                 continue;
@@ -542,7 +551,15 @@ static void worker_main(sp_ctx* ctx) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 tun = tunings_for(ctx, P, slots);
             }
-            int rc = ensure_partial(ctx, std::max<size_t>(8, P.partial_doubles * 8));
+            int rc = SP_OK;
+            {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                if (ctx->inject_failures > 0) {
+                    --ctx->inject_failures;
+                    rc = set_error(SP_ERR_DEVICE, "iteration failed (injected)");
+                }
+            }
+            if (rc == SP_OK) rc = ensure_partial(ctx, std::max<size_t>(8, P.partial_doubles * 8));
             if (rc == SP_OK) rc = ensure_table(ctx, L.bytes);
             if (rc == SP_OK) rc = ensure_counters(ctx, std::max<size_t>(1, P.kinds.size()));
             if (rc == SP_OK) {
@@ -556,7 +573,7 @@ static void worker_main(sp_ctx* ctx) {
                 if (hipStreamSynchronize(s) != hipSuccess && rc == SP_OK) rc = SP_ERR_DEVICE;
             }
             float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
+            if (rc == SP_OK) hipEventElapsedTime(&ms, e0, e1);
             {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 ctx->stat_iterations += 1;
@@ -568,8 +585,14 @@ static void worker_main(sp_ctx* ctx) {
                 x->work_share = 0.0;
             }
             if (rc != SP_OK) {
+                // every slot of the failed iteration ends, FP64 shadows included: a shadow left running would
+                // later publish its parent's partial sums
                 std::lock_guard<std::mutex> lk(ctx->mu);
-                for (Slot* x : slots) if (x->job && !x->done) { x->job->status = rc; x->done = true; }
+                for (Slot* x : slots) {
+                    if (x->done) continue;
+                    if (x->job && x->job->status == SP_ERR_NOT_READY) x->job->status = rc;
+                    x->done = true;
+                }
                 ctx->done_cv.notify_all();
             } else {
                 for (Slot* f : P.finishing) f->done = true;
@@ -780,6 +803,14 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
     double best_r = -1.0;
     sp_tuning bt{};
     int err = SP_OK;
+    const uint64_t key = tuning_key(r.problem, r.solver, prec);
+    bool had_prev = false;
+    sp_tuning prev{};
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        auto it = ctx->tuning.find(key);
+        if (it != ctx->tuning.end()) { had_prev = true; prev = it->second; }
+    }
     auto trial = [&](const sp_tuning& t) -> double {
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
@@ -801,7 +832,8 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
                 const double rate = err == SP_OK ? trial(t) : -1.0;
                 if (err != SP_OK) {               // a failed trial ends the search with its error
                     std::lock_guard<std::mutex> lk(ctx->mu);
-                    ctx->tuning.erase(tuning_key(r.problem, r.solver, prec));
+                    if (had_prev) ctx->tuning[key] = prev;     // the caller's tuning stays as it was
+                    else ctx->tuning.erase(key);
                     return err;
                 }
                 if (rate > best_r) { best_r = rate; bt = t; }

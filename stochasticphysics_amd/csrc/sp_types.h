@@ -87,7 +87,28 @@ inline constexpr uint32_t kernel_kind(int problem, int solver, int aniso, int mi
 }
 inline constexpr bool kind_is_uniaxialcubic(uint32_t kind) { return ((kind >> 1) & 3u) == 2u; }
 
-// deterministic reduction of partial sums (one block per output row)
+// Sum of the ensemble weights (the mu of every particle, MeanSimulationResult.h:63-75) in one fixed order, the
+// same on the host and on the device: WSUM_THREADS partial sums, partial t adding particles t, t + WSUM_THREADS,
+// ... in turn, then folded by halves (partial t += partial t + h for t < h, h = WSUM_THREADS / 2, / 4, ..., 1).
+inline constexpr uint32_t WSUM_THREADS = 256;
+inline double weight_sum_fixed(const double* w, uint64_t n) {
+    double part[WSUM_THREADS];
+    for (uint32_t t = 0; t < WSUM_THREADS; ++t) {
+        double s = 0.0;
+        for (uint64_t p = t; p < n; p += WSUM_THREADS) s += w[p];
+        part[t] = s;
+    }
+    for (uint32_t h = WSUM_THREADS / 2; h > 0; h >>= 1)
+        for (uint32_t t = 0; t < h; ++t) part[t] += part[t + h];
+    return part[0];
+}
+
+// blocks along x of a reduction launch over `rows` rows: the grid of a launch may hold at most 2^32 - 1
+// work-items per dimension (16,777,215 blocks of 256), so a block reduces every gridDim.x-th row
+inline constexpr uint32_t REDUCE_MAX_GRID_X = 65535;
+inline constexpr uint32_t reduce_grid_x(uint32_t rows) { return rows < REDUCE_MAX_GRID_X ? rows : REDUCE_MAX_GRID_X; }
+
+// deterministic reduction of partial sums (one block per output row, rows strided over the grid)
 struct ReduceWork {
     const double* partial;     // rows of pstride doubles
     uint32_t pstride;

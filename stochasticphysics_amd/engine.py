@@ -423,6 +423,12 @@ class Context:
     def set_memory_budget(self, nbytes, max_active=64):
         check(lib().sp_ctx_set_memory_budget(self._ctx, int(nbytes), int(max_active)))
 
+    def set_reduction_limit(self, nbytes):
+        check(lib().sp_ctx_set_reduction_limit(self._ctx, int(nbytes)))
+
+    def inject_failures(self, n_iterations):
+        check(lib().sp_ctx_inject_failures(self._ctx, int(n_iterations)))
+
     def set_tuning(self, problem, solver, precision, **kw):
         t = Tuning(**{k: int(v) for k, v in kw.items()})
         check(lib().sp_ctx_set_tuning(self._ctx, PROBLEM[problem], SOLVER[solver], PREC[precision], C.byref(t)))

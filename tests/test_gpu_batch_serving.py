@@ -147,3 +147,67 @@ def test_server_cancel(ctx):
     rid2 = ctx.submit(small)
     assert ctx.wait(rid2, 60_000) == SP_OK
     same(ctx.result(rid2, small), ctx.run(small))
+
+
+def test_whole_pass_and_chunked_reduction_agree(ctx):
+    """The plain path reduces a whole pass at once when its partial sums fit the reduction limit, chunk by
+    chunk otherwise: the two give the same means and mixed-precision check tiles bit for bit."""
+    e = bn_ensemble(n=300, n_steps=6000, solver="Heun_Strong", seed=71)
+    m = sph_ensemble(n=700, n_steps=5000, solver="Heun_Strong", seed=72)
+    m.precision = "mixed"
+    for x in (e, m):
+        ctx.set_tuning(x.problem, x.solver, x.precision, block_size=256, particles_per_lane=1, chunk_steps=1000,
+                       use_graph=1, persistent_waves_per_simd=2)
+    try:
+        whole = [ctx.run(x) for x in (e, m)]
+        ctx.set_reduction_limit(0)
+        chunked = [ctx.run(x) for x in (e, m)]
+    finally:
+        ctx.set_reduction_limit(4 << 30)
+    for a, b in zip(whole, chunked):
+        same(a, b)
+
+
+def test_reduction_over_more_rows_than_one_launch_dimension_holds(ctx):
+    """64 particles with an output every step: 5.7M windows x 3 components = 1.7e7 reduction rows, more than
+    the 16,777,215 blocks of 256 work-items one launch dimension can hold.  Deterministic relaxation (T = 0),
+    compared with the oracle; the chunked reduction gives the same sums."""
+    e = neel_ensemble(n=64, n_steps=5_700_000, os_=1, T=0.0, seed=73)
+    assert e.n_out * 3 > (1 << 24)
+    g = ctx.run(e)
+    o = run_oracle(e, keep_final=False, threads=16)
+    assert np.max(np.abs(g["mean"] - o["mean"])) <= 1e-10
+    ctx.set_reduction_limit(0)
+    try:
+        c = ctx.run(e)
+    finally:
+        ctx.set_reduction_limit(4 << 30)
+    same(g, c)
+
+
+def test_failed_server_iteration_ends_request_and_shadow(ctx):
+    """An iteration that fails ends every request in it, the FP64 shadow of a mixed request included: the
+    failed request stays failed (its shadow never publishes partial sums later) and the server goes on."""
+    a = sph_ensemble(n=640, n_steps=20_000, solver="Heun_Strong", seed=74)
+    a.precision = "mixed"
+    ctx.inject_failures(1)
+    rid = ctx.submit(a)
+    assert ctx.wait(rid, 120_000) != SP_OK
+    b = neel_ensemble(n=256, n_steps=3000, solver="Heun_Strong", seed=75)
+    rb = ctx.submit(b)
+    assert ctx.wait(rb, 120_000) == SP_OK
+    same(ctx.result(rb, b), ctx.run(b))
+    assert ctx.wait(rid, 0) != SP_OK
+    with pytest.raises(SpError):
+        ctx.result(rid, a)
+    # a failure in the middle of a long mixed request
+    c = sph_ensemble(n=640, n_steps=400_000, solver="Heun_Strong", seed=76)
+    c.precision = "mixed"
+    rc_ = ctx.submit(c)
+    time.sleep(0.05)
+    ctx.inject_failures(1)
+    assert ctx.wait(rc_, 120_000) != SP_OK
+    time.sleep(0.2)
+    assert ctx.wait(rc_, 0) != SP_OK
+    st = ctx.server_stats()
+    assert st["active"] == 0
