@@ -251,6 +251,22 @@ int sp_prepare(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp
 int sp_execute(sp_prepared* h, sp_result* res);
 void sp_release(sp_prepared* h);
 
+/* ---- allocation-free execution.  No counterpart in the reference: for callers that cannot allocate
+ *      device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Runs one request like sp_execute, asynchronously on `stream` (a hipStream_t; NULL = the null stream), on
+ * caller-owned device buffers: d_params [NPARAM = 23][n_particles] (param-major, the rows of
+ * sp_compute_particle_params), d_state [dim_y][n_particles] (particle index fastest; advanced in place to the
+ * final state), d_mean [n_out][dim_out] (the weighted ensemble means, equal to sp_run's bit for bit).  It
+ * allocates nothing: the request descriptor, the partial sums of the output windows and the weight sums live
+ * in a 32 MiB device array that each GPU holds from the first call on it until the process ends.  A request
+ * whose partial sums for one chunk, (lcm(Oversampling, 4) / Oversampling + 1) * dim_out * ceil(n_particles / 64)
+ * * 8 bytes, exceed 32 MiB - 4 KiB is rejected with SP_ERR_INVALID_ARG (about 22 million BrownAndNeel particles
+ * at Oversampling 50).  Calls that overlap on one GPU share that array and race.  Philox noise only, no
+ * single-simulation outputs; mixed precision runs without its FP64 check; ImplicitMidpoint's Newton failures
+ * are not reported. */
+int sp_execute_noalloc(int device, void* stream, const sp_request* req, const double* d_params,
+                       double* d_state, double* d_mean);
+
 /* ---- batched path: many uneven requests in one persistent launch per chunk ---- */
 int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs,
                  const sp_particles* parts, sp_result* res);

@@ -33,7 +33,7 @@ SOURCES = [
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=1"], "sp_kernels_bnes_a1.o"),
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=2"], "sp_kernels_bnes_a2.o"),
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=4"], "sp_kernels_bnes_a4.o"),
-    "sp_kernels_common.hip", "sp_kernels_provider.hip",
+    "sp_kernels_common.hip", "sp_kernels_provider.hip", "sp_noalloc.hip",
     "sp_params.cpp", "sp_enums.cpp", "sp_runtime.cpp", "sp_server.cpp",
     "frontend/ini.cpp", "frontend/settings.cpp",
 ]
