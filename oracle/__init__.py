@@ -33,29 +33,51 @@ def lib(native=False):
         path = _DIR / name
         if not path.exists():
             build()
-        L = C.CDLL(str(path))
-        d, u64, i64 = C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_int64)
-        L.or_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
-        L.or_normals.argtypes = [u64, u64, u64, u64, C.c_int, d]
-        L.or_bm_log.argtypes = [C.c_uint32]
-        L.or_bm_log.restype = C.c_double
-        L.or_bm_sincos2pi.argtypes = [C.c_uint32, d, d]
-        L.or_box_muller.argtypes = [C.POINTER(C.c_uint32), u64, d]
-        L.or_neel_params.argtypes = [C.c_double] * 5 + [d]
-        L.or_brown_params.argtypes = [C.c_double] * 3 + [d]
-        L.or_aniso_field.argtypes = [C.c_int, d, d, d, d, d]
-        L.or_euler_axes.argtypes = [C.c_double] * 3 + [d]
-        L.or_field.argtypes = [d, C.c_double, d]
-        L.or_eval.argtypes = [C.c_int, C.c_int, d, d, C.c_int, C.c_double, d, d, C.c_double, C.c_int, d]
-        L.or_eval.restype = C.c_int
-        L.or_newton_testfn.argtypes = [C.c_int, d, C.c_double, C.c_int, d]
-        L.or_newton_testfn.restype = C.c_int
-        L.or_simulate.argtypes = [i64, C.POINTER(C.c_uint64), d, d, d, d, d, d, d, d, d, d, d, d]
-        L.or_provide.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, u64, u64, d, d, d, d, i64]
-        L.or_provide.restype = None
-        L.or_simulate.restype = C.c_int
-        _LIBS[name] = L
+        _LIBS[name] = _bind(C.CDLL(str(path)))
     return _LIBS[name]
+
+
+def _bind(L):
+    """Argument and result types of the oracle's C entry points on a loaded library."""
+    d, u64, i64 = C.POINTER(C.c_double), C.c_uint64, C.POINTER(C.c_int64)
+    L.or_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+    L.or_normals.argtypes = [u64, u64, u64, u64, C.c_int, d]
+    L.or_bm_log.argtypes = [C.c_uint32]
+    L.or_bm_log.restype = C.c_double
+    L.or_bm_sincos2pi.argtypes = [C.c_uint32, d, d]
+    L.or_box_muller.argtypes = [C.POINTER(C.c_uint32), u64, d]
+    L.or_neel_params.argtypes = [C.c_double] * 5 + [d]
+    L.or_brown_params.argtypes = [C.c_double] * 3 + [d]
+    L.or_aniso_field.argtypes = [C.c_int, d, d, d, d, d]
+    L.or_euler_axes.argtypes = [C.c_double] * 3 + [d]
+    L.or_field.argtypes = [d, C.c_double, d]
+    L.or_eval.argtypes = [C.c_int, C.c_int, d, d, C.c_int, C.c_double, d, d, C.c_double, C.c_int, d]
+    L.or_eval.restype = C.c_int
+    L.or_newton_testfn.argtypes = [C.c_int, d, C.c_double, C.c_int, d]
+    L.or_newton_testfn.restype = C.c_int
+    L.or_simulate.argtypes = [i64, C.POINTER(C.c_uint64), d, d, d, d, d, d, d, d, d, d, d, d]
+    L.or_provide.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, u64, u64, d, d, d, d, i64]
+    L.or_provide.restype = None
+    L.or_simulate.restype = C.c_int
+    return L
+
+
+class use_library:
+    """Context manager: the oracle functions of this module call the shared library at `path` (a copy of the
+    oracle built elsewhere, e.g. with a mistake planted on purpose by a test) instead of liboracle.so."""
+
+    def __init__(self, path):
+        self.path = str(path)
+
+    def __enter__(self):
+        self.saved = dict(_LIBS)
+        _LIBS.clear()
+        _LIBS["liboracle.so"] = _bind(C.CDLL(self.path))
+        return self
+
+    def __exit__(self, *a):
+        _LIBS.clear()
+        _LIBS.update(self.saved)
 
 
 def _p(a, ct=C.c_double):
