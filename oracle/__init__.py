@@ -22,13 +22,14 @@ DIM_OUT = {1: 3, 2: 3, 3: 6, 4: 9}
 
 def build(force=False):
     """Compile the oracle with its Makefile (g++)."""
-    if force or not (_DIR / "liboracle.so").exists() or not (_DIR / "liboracle_native.so").exists():
+    if force or not all((_DIR / n).exists() for n in ("liboracle.so", "liboracle_native.so", "liboracle_fma.so")):
         subprocess.run(["make", "-C", str(_DIR), "-B" if force else "all"], check=True,
                        stdout=subprocess.DEVNULL)
 
 
-def lib(native=False):
-    name = "liboracle_native.so" if native else "liboracle.so"
+def lib(native=False, contracted=False):
+    """native: the -O3 -march=native build (CPU baseline); contracted: the FMA-contracted build (rounding floor)."""
+    name = "liboracle_native.so" if native else ("liboracle_fma.so" if contracted else "liboracle.so")
     if name not in _LIBS:
         path = _DIR / name
         if not path.exists():
@@ -194,12 +195,13 @@ def newton_testfn(fn, x0, tol, max_iter):
 def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0, axes=None, field=None,
              bn_simple=False, coord_transform=False, min_angle=0.0, newton_tol=1e-10, newton_max_iter=20, renormalize=False,
              replay=None, threads=1, keep_final=False, per_particle=False, native=False, euler_transform=False,
-             brown_min_angle=0.0):
+             brown_min_angle=0.0, contracted=False):
     """Run the oracle ensemble.  props: [N,8] (radius, Ms, alpha, gamma, T, K, r_h, viscosity; [N,12] with K_cubic and
     the uniaxial rotation for uniaxialcubic); BrownAndNeelEulerSpherical: coord_transform / min_angle for its Neel part,
     euler_transform / brown_min_angle for its Brown part;
     y0: [N, dim_y]; axes: [N, 9] or None; field: dict(type, offset, amp, freq, phase) or None;
-    replay: [K, N] standard normals or None."""
+    replay: [K, N] standard normals or None; contracted: run the FMA-contracted build (the same arithmetic rounded
+    differently)."""
     props = np.ascontiguousarray(props, dtype=np.float64)
     N = props.shape[0]
     y0 = np.ascontiguousarray(y0, dtype=np.float64)
@@ -223,7 +225,7 @@ def simulate(problem, solver, aniso, n_steps, oversampling, dt, seed, props, y0,
     pp = np.zeros(N * n_out * dout) if per_particle else None
     stats = np.zeros(2)
     ax = None if axes is None else np.ascontiguousarray(axes, dtype=np.float64)
-    rc = lib(native).or_simulate(_p(ispec, C.c_int64), _p(uspec, C.c_uint64), _p(dspec), _p(props), _p(ax), _p(y0),
+    rc = lib(native, contracted).or_simulate(_p(ispec, C.c_int64), _p(uspec, C.c_uint64), _p(dspec), _p(props), _p(ax), _p(y0),
                                  _p(rep), _p(mean), _p(weight), _p(times), _p(fields), _p(final), _p(pp), _p(stats))
     if rc:
         raise ValueError(f"or_simulate -> {rc}")

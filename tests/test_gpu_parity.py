@@ -27,7 +27,7 @@ def ctx():
     c.close()
 
 
-def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False):
+def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False, contracted=False):
     return oracle.simulate(e.pid, {"EulerMaruyama": 1, "EulerMaruyama_Normalized": 2, "Implicit_Midpoint": 3, "Millstein": 6,
                                    "Heun_Strong": 7,
                                    "Heun_NotConsistent": 8}[e.solver],
@@ -37,7 +37,7 @@ def run_oracle(e: Ensemble, keep_final=True, threads=8, per_particle=False):
                            coord_transform=e.coord_transform, min_angle=e.min_angle, euler_transform=e.euler_transform,
                            brown_min_angle=e.brown_min_angle, newton_tol=e.newton_tol,
                            newton_max_iter=e.newton_max_iter, renormalize=e.renormalize, replay=e.replay, threads=threads,
-                           keep_final=keep_final, per_particle=per_particle)
+                           keep_final=keep_final, per_particle=per_particle, contracted=contracted)
 
 
 # ---------------------------------------------------------------- noise
