@@ -14,6 +14,7 @@
 // ensemble accumulation, chart rotation and Newton stay in double.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "sp_rng.h"
 #include "sp_types.h"
 
@@ -100,34 +101,50 @@ struct FieldCache {
 // LDS read returns in a fraction of the latency of a vector-memory load that hits the cache.  Every FP64
 // simulation kernel fills its work-group's copy once at its start (bm_lds_fill); the values are those of
 // SP_LOG_TAB and SP_SINCOS_TAB, so the normals stay bit-identical to the host's.
+// The same LDS block holds the table of sin and cos at the multiples of pi/64 that sincos_fast reads; every
+// simulation kernel fills it.
 struct BMTables {
     double log_tab[97][2];
     double sc_tab[128][2];
+    double trig_tab[128][4];     // (sin, cos)(k pi / 64) as hi and lo parts, SP_TRIG64_TAB
 };
 static __shared__ BMTables sp_bm_lds;
 
 __device__ __forceinline__ void bm_lds_fill() {
-    constexpr int NL = 97 * 2, NS = 128 * 2;
-    double* dst = &sp_bm_lds.log_tab[0][0];
+    constexpr int NL = 97 * 2, NS = 128 * 2, NT = 128 * 4;
+    double* dst = &sp_bm_lds.log_tab[0][0];          // the three tables are contiguous
     const double* lt = &SP_LOG_TAB[0][0];
     const double* st = &SP_SINCOS_TAB[0][0];
-    for (int i = threadIdx.x; i < NL + NS; i += blockDim.x) dst[i] = i < NL ? lt[i] : st[i - NL];
+    const double* tt = &SP_TRIG64_TAB[0][0];
+    for (int i = threadIdx.x; i < NL + NS + NT; i += blockDim.x)
+        dst[i] = i < NL ? lt[i] : (i < NL + NS ? st[i - NL] : tt[i - NL - NS]);
     __syncthreads();
 }
 
 template <class R> struct NormalsT;
 template <> struct NormalsT<double> {
+    __device__ __forceinline__ static void block(uint64_t seed, uint64_t p, uint64_t call, double z[4]) {
+        box_muller4_f64(noise_block(seed, p, call, 0u), z, sp_bm_lds.log_tab, sp_bm_lds.sc_tab);
+    }
     __device__ __forceinline__ static void philox(uint64_t seed, uint64_t p, uint64_t call, double z[12]) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
-            box_muller4_f64(noise_block(seed, p, call + b, 0u), z + 4 * b, sp_bm_lds.log_tab, sp_bm_lds.sc_tab);
+        for (int b = 0; b < 3; ++b) block(seed, p, call + b, z + 4 * b);
     }
 };
 template <> struct NormalsT<float> {
+    __device__ __forceinline__ static void block(uint64_t seed, uint64_t p, uint64_t call, float z[4]) {
+        box_muller4_f32(noise_block(seed, p, call, 0u), z);
+    }
     __device__ __forceinline__ static void philox(uint64_t seed, uint64_t p, uint64_t call, float z[12]) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) box_muller4_f32(noise_block(seed, p, call + b, 0u), z + 4 * b);
+        for (int b = 0; b < 3; ++b) block(seed, p, call + b, z + 4 * b);
     }
+};
+// Whether a kernel class computes the next step group's normals beside the current group's physics (PIPE_NOISE,
+// the default) or each Philox block just before the step that first uses it, which keeps 12 fewer normals live.
+template <class Cfg, class = void> struct pipe_noise { static constexpr bool value = true; };
+template <class Cfg> struct pipe_noise<Cfg, std::void_t<decltype(Cfg::PIPE_NOISE)>> {
+    static constexpr bool value = Cfg::PIPE_NOISE;
 };
 template <class R>
 __device__ __forceinline__ void normals12(const DevRequest& rq, uint64_t p, uint64_t k0, R z[12]) {
@@ -371,10 +388,12 @@ template <int ANISO, int SOLVER, class R> struct BNCfg {
 // vectors and projection P (:162-236), det = P H (:239-265), S = sigma P
 // (:267-293), drift (-D/2 cos(theta)/sin(theta), 0) (:295-309), finish (:440-471),
 // output (sin t cos p, sin t sin p, cos t) (:499-516).
-// sin and cos of a double, |x| < 2^19 pi/2, to about one ulp: Cody-Waite reduction x = k pi/2 + r with
-// pi/2 in three parts (k * part exact for |k| < 2^20), Taylor series of sin r to r^17 and cos r to r^16 on
-// |r| <= pi/4 (the next terms are below 1e-16 relative), quadrant from the low bits of k without branches.
-// Larger arguments (never reached by angles that stay near [0, 2 pi]) take the library routine.
+// sin and cos of a double, |x| < 16384, within about half an ulp: x = k pi/64 + r with pi/64 in three parts
+// (k * part exact for |k| < 2^20) and |r| <= pi/128; sin r by its series to r^7 and cos r - 1 to r^6 (the next
+// terms are below 4e-19 relative and 4e-18 absolute), combined with (sin, cos)(k pi/64) from the LDS table --
+// each value as a double and the double nearest to its remainder -- by the angle-sum formulas.  The table covers
+// the whole circle, so k mod 128 picks the row and no quadrant logic is needed.  Larger arguments (never reached
+// by angles that stay near [0, 2 pi]) take the library routine.
 struct SinCos { double s, c; };
 // The library routines for the arguments the inline paths leave out, kept out of line: inlined, their long
 // reductions would be copied into every call site of the simulation loops and crowd the instruction cache.
@@ -386,41 +405,41 @@ static __device__ __attribute__((noinline)) SinCos sincos_library(double x) {
 static __device__ __attribute__((noinline)) double fmod_library(double x, double y) { return fmod(x, y); }
 
 __device__ __forceinline__ void sincos_fast(double x, double* s_out, double* c_out) {
-    if (!(fabs(x) < 823549.6)) {
+    if (!(fabs(x) < 16384.0)) {
         const SinCos r = sincos_library(x);
         *s_out = r.s;
         *c_out = r.c;
         return;
     }
-    const double k = rint(x * 0x1.45f306dc9c883p-1);          // x 2/pi
-    double r = fma(-k, 0x1.921fb54400000p+0, x);              // pi/2 = P1 + P2 + P3
-    r = fma(-k, 0x1.0b4611a600000p-34, r);
-    r = fma(-k, 0x1.3198a2e037073p-69, r);
+    const double kd = rint(x * 0x1.45f306dc9c883p+4);         // x 64/pi
+    double r = fma(-kd, 0x1.921fb54400000p-5, x);             // pi/64 = P1 + P2 + P3
+    r = fma(-kd, 0x1.0b4611a600000p-39, r);
+    r = fma(-kd, 0x1.3198a2e037073p-74, r);
+    const double* T = sp_bm_lds.trig_tab[(int)kd & 127];      // sin, cos of k pi/64: hi, hi, lo, lo
     const double r2 = r * r;
-    double sp = 2.8114572543455206e-15;                       // (-1)^n / (2n+1)!, n = 8 .. 1
-    sp = fma(sp, r2, -7.647163731819816e-13);
-    sp = fma(sp, r2, 1.6059043836821613e-10);
-    sp = fma(sp, r2, -2.505210838544172e-08);
-    sp = fma(sp, r2, 2.7557319223985893e-06);
-    sp = fma(sp, r2, -0.0001984126984126984);
-    sp = fma(sp, r2, 0.008333333333333333);
-    sp = fma(sp, r2, -0.16666666666666666);
-    const double sn = fma(r * r2, sp, r);
-    double cp = 4.779477332387385e-14;                        // (-1)^n / (2n)!, n = 8 .. 1
-    cp = fma(cp, r2, -1.1470745597729725e-11);
-    cp = fma(cp, r2, 2.08767569878681e-09);
-    cp = fma(cp, r2, -2.755731922398589e-07);
-    cp = fma(cp, r2, 2.48015873015873e-05);
-    cp = fma(cp, r2, -0.001388888888888889);
-    cp = fma(cp, r2, 0.041666666666666664);
-    cp = fma(cp, r2, -0.5);
-    const double cs = fma(r2, cp, 1.0);
-    const int q = (int)k & 3;
-    double ss = (q & 1) ? cs : sn, cc = (q & 1) ? sn : cs;
-    ss = (q & 2) ? -ss : ss;
-    cc = ((q + 1) & 2) ? -cc : cc;
-    *s_out = ss;
-    *c_out = cc;
+    const double sr = fma(r * r2, fma(fma(-1.984126984126984e-4, r2, 8.333333333333333e-3), r2, -0.16666666666666666), r);
+    const double cm1 = r2 * fma(fma(-1.388888888888889e-3, r2, 4.1666666666666664e-2), r2, -0.5);
+    *s_out = T[0] + fma(T[1], sr, fma(T[0], cm1, T[2]));
+    *c_out = T[1] + fma(-T[0], sr, fma(T[1], cm1, T[3]));
+}
+
+// 1 / x for |x| >= 2^-1020: the hardware reciprocal and two Newton corrections (within an ulp)
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+static __device__ __attribute__((noinline)) double inv_tiny(double x) {
+    const double r = 1.0 / x;
+    return isinf(r) ? 0.0 : r;
+}
+// 1 / sin(theta) with an infinite value replaced by 0 (NeelRelaxationSpherical.h:156-160)
+__device__ __forceinline__ double inv_or_zero(double x) { return fabs(x) >= 0x1p-1020 ? rcp_nr(x) : inv_tiny(x); }
+__device__ __forceinline__ float inv_or_zero(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return isinf(r) ? 0.0f : r;
 }
 
 // std::fmod(x, 2 pi): x itself when |x| < 2 pi; x -+ 2 pi when 2 pi <= |x| < 4 pi (exact by Sterbenz's lemma,
@@ -433,6 +452,22 @@ __device__ __forceinline__ double fmod_2pi(double x) {
 }
 
 template <class R> struct Trig { R st, ct, sp, cp; };
+// sin and cos of the angles (acos(c), atan2(ys, xs)) of a chart map, from c, ys and xs on the unit sphere
+// (c^2 + xs^2 + ys^2 = 1): sin acos(c) = hypot(xs, ys) >= 0 and (xs, ys) / hypot(xs, ys) are the cosine and sine
+// of atan2(ys, xs); where hypot(xs, ys) = 0, atan2(+-0, x) is +-0 for x = +0 and +-pi for x = -0
+__device__ __forceinline__ void chart_trig(double c, double ys, double xs, Trig<double>& T) {
+    const double h = sqrt(fma(xs, xs, ys * ys));
+    T.st = h;
+    T.ct = c;
+    if (h > 0.0) {
+        const double ih = rcp_nr(h);
+        T.cp = xs * ih;
+        T.sp = ys * ih;
+    } else {
+        T.cp = signbit(xs) ? -1.0 : 1.0;
+        T.sp = ys;
+    }
+}
 __device__ __forceinline__ void trig_of(double th, double ph, Trig<double>& T) {
     sincos_fast(th, &T.st, &T.ct);
     sincos_fast(ph, &T.sp, &T.cp);
@@ -456,8 +491,7 @@ template <class R> struct Chart { V3<R> P0, P1; R drift0; };
 // chart quantities from the (possibly rotated) coordinates' trig values
 template <int ANISO, class R>
 __device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Trig<R>& T, bool rot, V3<R> h, Chart<R>& c, V3<R>& det) {
-    R ods = (R)1 / T.st;
-    if (isinf(ods)) ods = (R)0;
+    const R ods = inv_or_zero(T.st);
     V3<R> M, et, ep;
     if (!rot) {
         M = {T.st * T.cp, T.st * T.sp, T.ct};
@@ -480,6 +514,8 @@ __device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Tri
 
 template <int ANISO, int SOLVER, class R> struct SphCfg {
     static constexpr int DY = 2, DW = 3, DOUT = 3, NORM = 0;
+    static constexpr bool PIPE_NOISE = false;
+    static constexpr int WAVES_PER_SIMD = 2;
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = SphLane<ANISO, R>;
     using Cache = Trig<R>;          // trig of the current (unrotated) state
@@ -504,10 +540,11 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
             double st, ct, sp, cp;
             if constexpr (sizeof(R) == 8) { st = T.st; ct = T.ct; sp = T.sp; cp = T.cp; }   // the cached trig of y
             else { sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp); }
-            Y[0] = acos(-cp * st);
-            Y[1] = atan2(sp * st, ct);
+            const double c = -cp * st, ys = sp * st;
+            Y[0] = acos(c);
+            Y[1] = atan2(ys, ct);
             Trig<double> Td;
-            trig_of(Y[0], Y[1], Td);
+            chart_trig(c, ys, ct, Td);          // sin and cos of Y without evaluating them
             T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
         }
         return rot;
@@ -517,13 +554,17 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         if (rot) {
             double st, ct, sp, cp;
             sincos_fast(r[0], &st, &ct); sincos_fast(r[1], &sp, &cp);
-            y[0] = acos(cp * st);
-            y[1] = atan2(sp * st, -ct);
+            const double c = cp * st, ys = sp * st, xs = -ct;
+            y[0] = acos(c);
+            y[1] = atan2(ys, xs);
+            Trig<double> Td;
+            chart_trig(c, ys, xs, Td);
+            T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
         } else {
             y[0] = fmod_2pi(r[0]);
             y[1] = fmod_2pi(r[1]);
+            trig_of(y[0], y[1], T);
         }
-        trig_of(y[0], y[1], T);
     }
     __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& T, const R* z, V3<double> hd) {
         const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
@@ -927,7 +968,37 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
     // Fast path (Philox noise, windows ending on group boundaries): each group is straight-line code, the
     // next group's Philox + Box-Muller is issued alongside this group's physics, and the per-step
     // normalisation is decided outside the loop.
+    auto fast_unpiped = [&](auto renorm_tag) {
+        constexpr bool RN = decltype(renorm_tag)::value;
+        const uint64_t seed = rq.seed;
+        for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
+            fc.ensure<MIDPOINT>(rq, n0, G);
+            const uint64_t call0 = (n0 * NZ) >> 2;
+            R z[12];
+            double gsum[DOUT];
+#pragma unroll
+            for (int s = 0; s < G; ++s) {
+                // block b is first used by step (4 b) / NZ of the group
+#pragma unroll
+                for (int b = 0; b < 3; ++b)
+                    if (s == (4 * b) / NZ) NormalsT<R>::block(seed, p, call0 + b, z + 4 * b);
+                Cfg::step(rq, L, y, cache, z + s * NZ, fc.at(n0 + s));
+                if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
+                double o[DOUT];
+                Cfg::output(L, y, cache, o);
+#pragma unroll
+                for (int c = 0; c < DOUT; ++c) gsum[c] = s == 0 ? o[c] : gsum[c] + o[c];
+            }
+            accumulate(gsum);
+            l += G;
+            if (l == OS) { close_window(); l = 0; }
+        }
+    };
     auto fast = [&](auto renorm_tag) {
+        if constexpr (!pipe_noise<Cfg>::value) {
+            fast_unpiped(renorm_tag);
+            return;
+        }
         constexpr bool RN = decltype(renorm_tag)::value;
         const uint64_t seed = rq.seed;
         R zc[12], zn[12];

@@ -5,12 +5,19 @@
 
 namespace sp {
 
+// Waves per SIMD a kernel class is compiled for (the register budget per wave is the SIMD's 512 registers
+// divided by it): WAVES_PER_SIMD of the class, 1 by default.
+template <class Cfg, class = void> struct waves_per_simd { static constexpr int value = 1; };
+template <class Cfg> struct waves_per_simd<Cfg, std::void_t<decltype(Cfg::WAVES_PER_SIMD)>> {
+    static constexpr int value = Cfg::WAVES_PER_SIMD;
+};
+
 // One wave per 64-particle tile of one request.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(MAX_BLOCK) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
+__global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(waves_per_simd<Cfg>::value))) void k_plain(const DevRequest* __restrict__ rqp, uint64_t step_begin, uint64_t step_end,
                                                double* partial, uint32_t pstride) {
     const DevRequest& rq = *rqp;
-    if (sizeof(R) == 8) bm_lds_fill();       // every thread, before any wave leaves
+    bm_lds_fill();       // every thread, before any wave leaves
     const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
     if (wave >= rq.n_tiles) return;
     const TileWork tw{0u, wave, step_begin, step_end, partial, pstride, wave};
@@ -36,9 +43,9 @@ __device__ __forceinline__ uint32_t find_slot(const SlotWork* __restrict__ slots
 // Persistent: every wave pulls global tile indices from the counter until all
 // n_tiles tiles of the launch are taken; every wave reaches that exit.
 template <class Cfg, class R, bool KAHAN, bool MID>
-__global__ __launch_bounds__(MAX_BLOCK) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
+__global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(waves_per_simd<Cfg>::value))) void k_batch(const DevRequest* __restrict__ reqs, const SlotWork* __restrict__ slots,
                                                uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
-    if (sizeof(R) == 8) bm_lds_fill();
+    bm_lds_fill();
     for (;;) {
         const uint32_t idx = next_work(counter);
         if (idx >= n_tiles) break;

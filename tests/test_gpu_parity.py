@@ -392,9 +392,11 @@ def compare_bnes(g, o, tol):
 def test_bnes_parity(ctx, solver, aniso):
     """BrownAndNeelEulerSpherical (BrownAndNeelRelaxationEulerSpherical.h): both charts rotate near their poles
     (theta_min = pi/36, particles start there); an FMA-contracted oracle build differs from the plain one by at most
-    7e-12 on these cases."""
+    7e-12 on these cases.  Ten sines and cosines per stage make them sensitive to the last bit of sin and cos as
+    well: the kernels' table-driven sin/cos (which equals the C library's on 98% of arguments) put into the oracle
+    in place of the C library's moves the cubic Heun_Strong case by 7e-11 on the CPU, hence 3e-10 here."""
     e = bnes_ensemble(solver=solver, aniso=aniso)
-    compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
+    compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 3e-10)
 
 
 def test_bnes_replay_parity(ctx):
