@@ -10,15 +10,21 @@ in HBM.
 
 One JSON line on stdout: particle-steps/s of the plain FP64 path (value), the
 vector-ALU roofline of its kernel, a CPU baseline on host cores, and
-  * "mixed": uneven ensembles of the 512-set batching distribution, plain FP64
-    one request at a time vs the batched persistent path in FP64 and in mixed
-    precision (with its FP64 check), and the <m> error of mixed against FP64;
+  * "mixed": the 512 uneven sets of the batching distribution, plain FP64 one
+    request at a time vs the batched persistent path in FP64 and in mixed
+    precision (with its FP64 check), autotuned, and the <m> error of mixed
+    against FP64 beside the rounding floor of FP64 itself;
+  * "precision": the NeelSpherical cubic precision study at full size
+    (1,048,576 particles x 5e5 steps), Heun_Strong and Millstein, FP64 vs mixed;
   * "serving": Poisson arrivals of spherical-cubic requests (Implicit_Midpoint
-    for damping >= 0.3) through the continuous-batching server, p50/p99
-    end-to-end latency and throughput;
-  * "problems": FP64 and mixed throughput of every problem class;
+    for damping >= 0.3) at 200 req/s for 60 s through the continuous-batching
+    server, then at 50% and 80% of the measured capacity: p50/p99 end-to-end
+    latency and throughput;
+  * "neel": the 4,096-particle Neel relaxation, FP64 and mixed, beside the CPU
+    oracle on the whole ensemble;
   * "provider": the particles of a parameter file drawn on the device vs
-    materialised on the host and uploaded.
+    materialised on the host and uploaded;
+  * "problems" (--problems): FP64 and mixed throughput of every problem class.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--serve-seconds S]
 """
@@ -58,10 +64,11 @@ def cpu_baseline(e, budget_s=12.0):
     subprocess.run(["make", "-s", "-B", "-C", str(ROOT / "oracle"), "liboracle_native.so"], check=True,
                    stdout=subprocess.DEVNULL)
     try:
-        cores = len(os.sched_getaffinity(0))
+        visible = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        visible = os.cpu_count() or 1
+    # a one-GPU lease owns 16 host cores whatever nproc shows (the rest belong to other GPUs' leases)
+    cores = max(1, min(visible, 16))
     solver = {"EulerMaruyama": 1, "Implicit_Midpoint": 3, "Millstein": 6, "Heun_Strong": 7, "Heun_NotConsistent": 8}[e.solver]
 
     def run(sub, steps):
@@ -79,13 +86,18 @@ def cpu_baseline(e, budget_s=12.0):
     rate = n * sub.total_steps / secs
     model = ""
     try:
-        for line in Path("/proc/cpuinfo").read_text().splitlines():
-            if line.startswith("model name"):
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True).stdout.splitlines():
+            if line.startswith("Model name"):
                 model = line.split(":", 1)[1].strip()
                 break
     except OSError:
         pass
-    return dict(value=rate, unit="particle-steps/s", cores=cores, kind="port",
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count() or 0
+    return dict(value=rate, unit="particle-steps/s", cores=cores, kind="port", nproc=nproc, affinity_cores=visible,
+                core_note="threads = the 16 host cores of a one-GPU lease (nproc and the affinity mask show the host's)",
                 per_core=rate / cores, ns_per_particle_step=1e9 * cores / rate, cpu_model=model,
                 sample=f"{n} of {e.n} particles x {sub.total_steps} steps ({secs:.1f} s, oracle -O3 -march=native, "
                        f"{cores} threads)")
@@ -101,18 +113,46 @@ def traffic_per_launch():
         return None
 
 
-def mixed_leg(ctx, configs, n_sets, seed=44):
-    """Uneven batched ensembles (a draw of n_sets sets of the 512-set distribution: N log-uniform in
-    [1e3, 1.28e5], steps U[5e4, 5e5], half Neel EM/Heun, half BrownAndNeel Heun, all with per-step
-    renormalisation): plain FP64 one request at a time (inputs resident) vs one batched persistent schedule
-    in FP64 and in mixed precision with its FP64 check; <m> error of the mixed results against plain FP64."""
-    def sets(precision):
+def perturbed(e, ulps=1):
+    """The same ensemble with every start-state component moved by `ulps` units in the last place: the distance of its
+    FP64 result from the unperturbed one is the rounding floor of the comparison (what any other rounding of the
+    same FP64 arithmetic would give on the same noise stream)."""
+    import copy
+    p = copy.copy(e)
+    y = np.array(e.y0, dtype=np.float64)
+    for _ in range(ulps):
+        y = np.nextafter(y, np.inf)
+    p.y0 = y
+    return p
+
+
+def m_columns(e):
+    """Columns of the ensemble mean that hold <m> (BrownAndNeel: 3..5; the particle axis n is 0..2)."""
+    return slice(3, 6) if e.problem == "BrownAndNeel" else slice(0, 3)
+
+
+def diff_stats(a, b, e):
+    d = (a["mean"] - b["mean"])[1:, m_columns(e)]           # window 0 is the shared start state
+    return float(np.abs(d).max()), float(np.sqrt(np.mean(d ** 2))), float(np.abs(d.mean(axis=0)).max())
+
+
+def mixed_leg(ctx, configs, n_sets, seed=44, floor_every=8, tune=True, literal=False):
+    """The 512-set batching distribution (N log-uniform in [1e3, 1.28e5], steps U[5e4, 5e5], half Neel
+    EulerMaruyama / Heun_Strong, half BrownAndNeel Heun_Strong).  As the reference integrates them (no
+    renormalisation), the moments of many sets run away over up to 5e5 steps and their means overflow: one batched FP64
+    run of that scheme counts them (literal).  The timed legs renormalise after every step (the reference's own
+    normalize(), which its EulerMaruyama_Normalized applies): plain FP64 one request at a time (inputs resident) vs one
+    batched persistent schedule in FP64 and in mixed precision (with its FP64 check), with tunings chosen by the
+    autotuner per kernel class.  <m> error of mixed against FP64, beside the rounding floor (FP64 against FP64 from a
+    start state moved by one ulp) on every floor_every-th set."""
+    def sets(precision, renorm=True):
         out = configs.config4_sets(n_sets=n_sets, seed=seed, precision=precision)
         for e in out:
-            e.renormalize = True
+            e.renormalize = renorm
         return out
-    s64 = sets("fp64")
+    s64, smx = sets("fp64"), sets("mixed")
     ps = sum(e.particle_steps() for e in s64)
+    # the plain baseline runs first, with the engine's default tuning (the autotuner tunes the batched path)
     preps = [ctx.prepare(e) for e in s64]
     preps[0].execute()
     t0 = time.perf_counter()
@@ -120,36 +160,80 @@ def mixed_leg(ctx, configs, n_sets, seed=44):
     t_plain = time.perf_counter() - t0
     for p in preps:
         p.release()
-    log(f"bench: {n_sets} uneven sets ({ps:.3e} particle-steps), plain FP64 one by one {ps / t_plain:.3e} /s")
+    log(f"bench: {n_sets} uneven sets ({ps:.3e} particle-steps): plain FP64 one by one {ps / t_plain:.3e} /s "
+        f"({t_plain:.1f} s)")
+    lit = None
+    if literal:
+        t0 = time.perf_counter()
+        raw = ctx.run_batch(sets("fp64", renorm=False))
+        t_lit = time.perf_counter() - t0
+        fin_lit = [bool(np.isfinite(r["mean"]).all()) for r in raw]
+        del raw
+        lit = dict(sets_with_finite_mean=int(sum(fin_lit)), neel_sets_finite=int(sum(fin_lit[0::2])),
+                   brown_and_neel_sets_finite=int(sum(fin_lit[1::2])), batched_fp64_value=ps / t_lit,
+                   note="the reference's scheme (no renormalisation): sets whose FP64 ensemble mean stays finite")
+        log(f"bench: reference scheme (no renormalisation): {lit['sets_with_finite_mean']} of {n_sets} sets finite "
+            f"({lit['neel_sets_finite']} Neel, {lit['brown_and_neel_sets_finite']} BrownAndNeel); batched FP64 "
+            f"{ps / t_lit:.3e} /s")
+    tunings = {}
     t0 = time.perf_counter()
-    ctx.run_batch(s64)
+    if tune:
+        for sets in (s64, smx):
+            largest = {}
+            for e in sets:
+                key = (e.problem, e.solver, e.precision)
+                if key not in largest or e.particle_steps() > largest[key].particle_steps():
+                    largest[key] = e
+            for key, e in largest.items():
+                tun, rate = ctx.autotune(e)
+                tunings["/".join(key)] = dict(tun, probe_rate=rate)
+    log(f"bench: autotuned {len(tunings)} kernel classes in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    b64 = ctx.run_batch(s64)
     t_b64 = time.perf_counter() - t0
-    log(f"bench: batched FP64 {ps / t_b64:.3e} /s")
-    smx = sets("mixed")
+    same = all(np.array_equal(x["mean"], y["mean"], equal_nan=True) for x, y in zip(b64, ref))
+    log(f"bench: batched FP64 {ps / t_b64:.3e} /s ({t_b64:.1f} s); equal to plain bit for bit: {same}")
     t0 = time.perf_counter()
     res = ctx.run_batch(smx)
     t_bmx = time.perf_counter() - t0
-    err = np.array([np.abs(r["mean"] - f["mean"]).max() for r, f in zip(res, ref)])
-    rms = np.array([np.sqrt(np.mean((r["mean"] - f["mean"]) ** 2)) for r, f in zip(res, ref)])
-    # systematic part: the difference averaged over the run's windows, per component (Monte-Carlo decorrelation of
-    # single trajectories averages out, a precision bias does not)
-    bias = np.array([np.abs((r["mean"] - f["mean"]).mean(axis=0)).max() for r, f in zip(res, ref)])
+    log(f"bench: batched mixed {ps / t_bmx:.3e} /s ({t_bmx:.1f} s)")
+    fin = [i for i, r in enumerate(ref) if np.isfinite(r["mean"]).all()]      # all of them once renormalised
+    st = np.array([diff_stats(res[i], ref[i], s64[i]) for i in fin])
+    sub = fin[::floor_every]
+    fl_res = ctx.run_batch([perturbed(s64[i]) for i in sub])
+    fl = np.array([diff_stats(f, ref[i], s64[i]) for f, i in zip(fl_res, sub)])
+    mx_sub = np.array([diff_stats(res[i], ref[i], s64[i]) for i in sub])
     fb = sum(int(r["used_fallback"]) for r in res)
-    log(f"bench: batched mixed {ps / t_bmx:.3e} /s; <m> error vs FP64: max {err.max():.2e}, median of per-set max "
-        f"{np.median(err):.2e}, rms {np.sqrt(np.mean(rms ** 2)):.2e}, time-averaged bias max {bias.max():.2e}; "
-        f"fallbacks {fb}/{n_sets}")
-    return dict(value=ps / t_bmx, unit="particle-steps/s", workload=f"{n_sets} uneven sets, {ps:.4g} particle-steps",
-                path="batched persistent launch, hipGraph", dtype="fp32 bulk, fp64 state and sums",
-                fp64_plain_value=ps / t_plain, fp64_batched_value=ps / t_b64,
+    inv_sqrt_n = np.array([1.0 / math.sqrt(s64[i].n) for i in fin])
+    log(f"bench: mixed vs FP64 <m> over {len(fin)} finite sets: max {st[:, 0].max():.2e}, rms {np.sqrt(np.mean(st[:, 1] ** 2)):.2e}, "
+        f"time-averaged bias max {st[:, 2].max():.2e}; rounding floor on {len(sub)} sets: max {fl[:, 0].max():.2e} rms "
+        f"{np.sqrt(np.mean(fl[:, 1] ** 2)):.2e} (mixed on the same sets: max {mx_sub[:, 0].max():.2e} rms "
+        f"{np.sqrt(np.mean(mx_sub[:, 1] ** 2)):.2e}); fallbacks {fb}/{n_sets}")
+    return dict(value=ps / t_bmx, unit="particle-steps/s",
+                workload=f"{n_sets} uneven sets of the batching distribution, {ps:.4g} particle-steps, renormalised "
+                         "after every step", reference_scheme=lit,
+                path="batched persistent launch, hipGraph, autotuned tiles", dtype="fp32 bulk, fp64 state and sums",
+                fp64_plain_value=ps / t_plain, fp64_batched_value=ps / t_b64, batched_equals_plain=bool(same),
                 speedup_vs_fp64_plain=t_plain / t_bmx, speedup_vs_fp64_batched=t_b64 / t_bmx,
-                err_max_vs_fp64=float(err.max()), err_median_of_set_max=float(np.median(err)),
-                err_rms_vs_fp64=float(np.sqrt(np.mean(rms ** 2))), err_bias_max_vs_fp64=float(bias.max()),
+                seconds=dict(plain_fp64=t_plain, batched_fp64=t_b64, batched_mixed=t_bmx),
+                tunings=tunings,
+                sets_with_finite_fp64_mean=len(fin),
+                sets_with_finite_mixed_mean=int(sum(np.isfinite(r["mean"]).all() for r in res)),
+                err_max_vs_fp64=float(st[:, 0].max()), err_median_of_set_max=float(np.median(st[:, 0])),
+                err_rms_vs_fp64=float(np.sqrt(np.mean(st[:, 1] ** 2))), err_bias_max_vs_fp64=float(st[:, 2].max()),
+                sets_within_1e3_per_window=int((st[:, 0] <= 1e-3).sum()),
+                sets_within_1e3_bias=int((st[:, 2] <= 1e-3).sum()),
+                rounding_floor=dict(sets=len(sub), err_max=float(fl[:, 0].max()), err_rms=float(np.sqrt(np.mean(fl[:, 1] ** 2))),
+                                    bias_max=float(fl[:, 2].max()), mixed_err_max_same_sets=float(mx_sub[:, 0].max()),
+                                    mixed_err_rms_same_sets=float(np.sqrt(np.mean(mx_sub[:, 1] ** 2))),
+                                    how="FP64 from a start state moved by one ulp vs FP64, same noise stream"),
+                err_max_over_inv_sqrt_n_max=float((st[:, 0] / inv_sqrt_n).max()),
                 check_error_max=float(max(r["check_error"] for r in res)),
+                check_bias_max=float(max(r["check_bias"] for r in res)),
                 check_noise_max=float(max(r["check_noise"] for r in res)), fp64_fallbacks=fb,
-                note="all sets renormalised after every step on both sides: over up to 5e5 steps the reference's "
-                     "EulerMaruyama and Heun_Strong let |m| run away in some particles; the per-output error is "
-                     "Monte-Carlo decorrelation (trajectories that differ by rounding decorrelate), so the "
-                     "time-averaged bias is the precision figure")
+                note="<m> = the magnetisation columns of the mean; trajectories that differ only by rounding decorrelate, "
+                     "so the per-window error is Monte-Carlo noise of order sigma/sqrt(N) whose size the rounding floor "
+                     "shows; the time-averaged bias is the precision figure (the one the FP64 check accepts on)")
 
 
 def serving_leg(ctx, configs, seconds, rate, seed=55):
@@ -160,6 +244,7 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
     reqs = [configs.serving_request(rng, i) for i in range(n_req)]
     log(f"bench: serving {n_req} requests over {arrivals[-1]:.1f} s ({rate:.0f} req/s offered)")
     ids = [None] * n_req
+    st0 = ctx.server_stats()
     t0 = time.perf_counter()
     for i, e in enumerate(reqs):
         delay = arrivals[i] - (time.perf_counter() - t0)
@@ -176,7 +261,8 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
         fallbacks += int(r["used_fallback"])
         im += int(e.solver == "Implicit_Midpoint")
     wall = time.perf_counter() - t0
-    st = ctx.server_stats()
+    st1 = ctx.server_stats()
+    st = {k: st1[k] - st0[k] for k in ("busy_ms", "iterations", "completed")}
     lat = np.array(lat)
     offered = sum(e.particle_steps() for e in reqs) / arrivals[-1]
     log(f"bench: serving p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, "
@@ -185,9 +271,27 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
                 wall_s=wall, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), throughput=ps / wall, unit="particle-steps/s",
                 offered_particle_steps_per_s=offered, device_busy_frac=st["busy_ms"] / (wall * 1e3),
-                implicit_midpoint_requests=im, fp64_fallbacks=fallbacks,
+                implicit_midpoint_requests=im, fp64_fallbacks=fallbacks, iterations=int(st["iterations"]),
                 request_sizes="N log-uniform [256, 4096], steps U[2e3, 1e4], OS 100; NeelSpherical cubic Lissajous, "
                               "mixed (Heun_Strong) or FP64 Implicit_Midpoint for damping >= 0.3")
+
+
+def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55):
+    """The serving configuration (Poisson arrivals at `rate` for `seconds`), then the same request distribution at
+    50% and 80% of the capacity that run measured (served particle-steps/s over the device's busy fraction), for
+    `low_seconds` each: p50/p99 latency against offered load."""
+    main = serving_leg(ctx, configs, seconds, rate, seed)
+    capacity = main["throughput"] / max(main["device_busy_frac"], 1e-9)
+    per_request = main["offered_particle_steps_per_s"] / rate
+    main["offered_fraction_of_capacity"] = main["offered_particle_steps_per_s"] / capacity
+    loads = []
+    for k, frac in enumerate((0.5, 0.8)):
+        r = serving_leg(ctx, configs, low_seconds, frac * capacity / per_request, seed + 1 + k)
+        loads.append({key: r[key] for key in ("requests", "offered_req_per_s", "p50_ms", "p99_ms", "mean_ms", "throughput",
+                                              "offered_particle_steps_per_s", "device_busy_frac", "fp64_fallbacks")})
+        loads[-1]["offered_fraction_of_capacity"] = r["offered_particle_steps_per_s"] / capacity
+    main.update(capacity_particle_steps_per_s=capacity, lower_loads=loads)
+    return main
 
 
 def precision_study(ctx, configs, n, n_steps):
@@ -197,6 +301,7 @@ def precision_study(ctx, configs, n, n_steps):
     for solver in ("Heun_Strong", "Millstein"):
         row = {}
         res = {}
+        with_floor = solver == "Heun_Strong"
         for prec in ("fp64", "mixed"):
             e = configs.config3(n=n, n_steps=n_steps, solver=solver, precision=prec)
             e.skip_check = True
@@ -211,11 +316,20 @@ def precision_study(ctx, configs, n, n_steps):
         row.update(err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
                    err_bias_max=float(np.abs(d[1:].mean(axis=0)).max()),     # time-averaged difference per component
                    speedup=row["mixed"] / row["fp64"], monte_carlo_sigma_over_sqrt_n=float(0.58 / math.sqrt(n)))
-        log(f"bench: precision study {solver}: mixed - FP64 <m> max {row['err_max']:.2e} rms {row['err_rms']:.2e}")
+        if with_floor:      # rounding floor: FP64 from a start state moved by one ulp, on the same noise stream
+            e = perturbed(configs.config3(n=n, n_steps=n_steps, solver=solver, precision="fp64"))
+            p = ctx.prepare(e)
+            f = p.execute()["mean"] - res["fp64"]["mean"]
+            p.release()
+            row.update(floor_max=float(np.abs(f).max()), floor_rms=float(np.sqrt(np.mean(f ** 2))),
+                       floor_bias_max=float(np.abs(f[1:].mean(axis=0)).max()))
+        log(f"bench: precision study {solver}: mixed - FP64 <m> max {row['err_max']:.2e} rms {row['err_rms']:.2e} bias "
+            f"{row['err_bias_max']:.2e}" + (f"; floor (FP64 one ulp apart) max {row['floor_max']:.2e} rms "
+                                           f"{row['floor_rms']:.2e}" if with_floor else ""))
         out[solver] = row
     return dict(workload=f"NeelSpherical cubic, {n} particles x {(n_steps // 100 - 1) * 100} steps, dt 2e-12 s, "
                          "Lissajous 12 mT at 25.25/24.5 kHz, output every 100 steps", unit="particle-steps/s",
-                results=out)
+                floor="FP64 from a start state moved by one ulp vs FP64, same noise stream", results=out)
 
 
 def neel_relaxation(ctx, configs):
@@ -239,6 +353,8 @@ def neel_relaxation(ctx, configs):
     cpu = cpu_baseline(e, budget_s=60.0)
     ps = e.particle_steps()
     d = rm["mean"] - r64["mean"]
+    log(f"bench: Neel relaxation (4,096 particles): FP64 {ps / t64:.3e}, mixed {ps / tm:.3e} particle-steps/s; CPU "
+        f"{cpu['value']:.3e} on {cpu['cores']} threads ({cpu['sample']})")
     return dict(workload="Neel uniaxial EulerMaruyama, 4096 particles x 199,900 steps, dt 1e-12 s, zero field",
                 fp64=ps / t64, mixed=ps / tm, unit="particle-steps/s", cpu_baseline=cpu, fp64_vs_cpu=ps / t64 / cpu["value"],
                 err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
@@ -386,15 +502,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-precision", action="store_true", help="headline run without the precision study")
+    ap.add_argument("--no-neel", action="store_true", help="headline run without the Neel relaxation leg")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--batch-sets", type=int, default=48, help="uneven sets in the mixed/batched leg")
+    ap.add_argument("--serve-low-seconds", type=float, default=8.0,
+                    help="duration of each of the two lower-load serving runs (50%% and 80%% of capacity)")
+    ap.add_argument("--batch-sets", type=int, default=512, help="uneven sets in the mixed/batched leg")
+    ap.add_argument("--no-tune", action="store_true", help="batched legs with the default tuning (no autotuner)")
+    ap.add_argument("--reference-scheme", action="store_true",
+                    help="also run the 512 sets as the reference integrates them (no renormalisation) and count the "
+                         "sets whose FP64 mean stays finite")
+    ap.add_argument("--problems", action="store_true", help="headline run with the per-problem leg")
     ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems", "provider"), default="headline",
                     help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation; "
                          "problems: every problem class on a 65,536-particle ensemble; provider: particles of a "
                          "parameter file drawn on the device vs on the host (each prints its own JSON line instead "
                          "of the headline one)")
-    ap.add_argument("--no-problems", action="store_true", help="headline run without the per-problem leg")
     ap.add_argument("--no-provider", action="store_true", help="headline run without the particle-provider leg")
     ap.add_argument("--particles", type=int, default=1_048_576, help="precision study: particles")
     ap.add_argument("--study-steps", type=int, default=500_000, help="precision study: steps")
@@ -474,10 +598,17 @@ def main():
                                      note="Heun_Strong as the reference writes it does not keep |m| = 1 for this "
                                           "problem; particles whose |m| runs away overflow, as in the reference"))
     if not args.no_mixed:
-        out["mixed"] = mixed_leg(ctx, configs, args.batch_sets if not args.quick else 4)
+        out["mixed"] = mixed_leg(ctx, configs, args.batch_sets if not args.quick else 4, tune=not args.no_tune,
+                                 literal=args.reference_scheme)
+    if not args.no_precision:
+        out["precision"] = precision_study(ctx, configs, *((1 << 14, 20_000) if args.quick else (args.particles,
+                                                                                                 args.study_steps)))
     if not args.no_serving:
-        out["serving"] = serving_leg(ctx, configs, args.serve_seconds, args.serve_rate)
-    if not args.no_problems and not args.quick:
+        out["serving"] = serving_study(ctx, configs, 5.0 if args.quick else args.serve_seconds, args.serve_rate,
+                                       2.0 if args.quick else args.serve_low_seconds)
+    if not args.no_neel and not args.quick:
+        out["neel"] = neel_relaxation(ctx, configs)
+    if args.problems:
         out["problems"] = problem_rates(ctx, configs)
     if not args.no_provider:
         out["provider"] = provider_leg(ctx, *((1 << 16, 1 << 12) if args.quick else ()))

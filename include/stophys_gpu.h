@@ -186,6 +186,10 @@ typedef struct sp_result {
                                   Saturation_magnetisation, Damping_constant, Gyromagnetic_ratio,
                                   Anisotropy_constant, Hydrodynamic_Radius, Viscosity: the Mean_Properties of
                                   Mean_Results (MeanSimulationResult.h:131-141) */
+    double check_bias;         /* mixed: the figure the check accepts on, max over components of the time-averaged
+                                  (windows >= 1) difference |<m>_mixed - <m>_FP64| on the check subset (-1 if not checked) */
+    double check_bias_se;      /* mixed: the largest standard error of that average, from the spread of the subset's
+                                  64-particle tiles; the check fails where |bias| - t_0.999 * se > check_tol */
 } sp_result;
 
 /* execution knobs chosen by the autotuner (or by hand) */
@@ -277,6 +281,10 @@ int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uin
 int sp_wait(sp_ctx* ctx, uint64_t id, int timeout_ms);
 int sp_result_get(sp_ctx* ctx, uint64_t id, sp_result* res);
 int sp_cancel(sp_ctx* ctx, uint64_t id);   /* replaces SimulationManager::abort (SimulationManager.h:595-599) */
+/* fraction of a submitted request's steps done, updated after every chunk (1 once its result is ready); replaces
+ * the per-finished-simulation count of SimulationManager::finishSimulationTask (SimulationManager.h:177-214), which a
+ * GPU ensemble that advances all its particles together cannot give */
+int sp_progress(sp_ctx* ctx, uint64_t id, double* fraction);
 int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending, iterations,
                                                  busy_ms, wall_ms, particle_steps, fallbacks */);
 

@@ -3,15 +3,19 @@
 // Header-only front-end with the public surface of the reference's
 // SimulationApplication::SimulationManager<prec, InstructionSet>
 // (Particle_Simulation/Simulator/SimulationManager.h:545-615): constructor from
-// the parameters, StartSimulationManager, waitUntilFinsihed (sic), isFinished,
-// abort and the static progress accessors.  The whole ensemble runs on one GPU
+// a parameter file, StartSimulationManager, waitUntilFinsihed (sic), isFinished,
+// abort and the static progress accessors (updated per chunk while waiting).
+// stophys_gpu_simulation_manager.hpp builds the reference's own
+// SimulationManager<prec, InstructionSet::GFX950> on it.  The whole ensemble runs on one GPU
 // through libstophys_gpu.so, its particles drawn on the device by the provider
 // kernel (no per-particle array on the host); results are written like the
 // reference's Mean_Results when the manager finishes.  Errors are rethrown as
 // std::runtime_error, as the reference does.
 #pragma once
 #include <atomic>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -24,8 +28,9 @@ class SimulationManagerGFX950 {
 public:
     using Parameters = std::string;   // path of a SimulationManagerSettings parameter file
 
+    // the parameter file is read here; the device context is opened by StartSimulationManager
     explicit SimulationManagerGFX950(const Parameters& parameter_file, int device = 0, std::string result_file = {})
-        : result_file_(std::move(result_file)) {
+        : device_(device), result_file_(std::move(result_file)) {
         check(sp_settings_load(parameter_file.c_str(), &settings_));
         check(sp_settings_request(settings_, &req_, nullptr));
         if (result_file_.empty()) {
@@ -35,7 +40,6 @@ public:
             if (dot != std::string::npos) result_file_ = result_file_.substr(0, dot);
             result_file_ += ".json";
         }
-        check(sp_ctx_create(device, &ctx_));
     }
     SimulationManagerGFX950(const SimulationManagerGFX950&) = delete;
     SimulationManagerGFX950& operator=(const SimulationManagerGFX950&) = delete;
@@ -47,21 +51,30 @@ public:
     // SimulationManager.h:549-574: starts the ensemble (asynchronously here)
     void StartSimulationManager() {
         if (started_) return;
+        if (!ctx_) check(sp_ctx_create(device_, &ctx_));
         check(sp_settings_submit(ctx_, settings_, &id_));
         started_ = true;
     }
-    // SimulationManager.h:577-584
+    // SimulationManager.h:577-584; reports progress while it waits
     void waitUntilFinsihed() {
         if (!started_) throw std::runtime_error("SimulationManager: not started");
         if (collected_) return;
-        check(sp_wait(ctx_, id_, -1));
+        for (;;) {
+            const int rc = sp_wait(ctx_, id_, 200);
+            if (rc == SP_OK) break;
+            if (rc != SP_ERR_TIMEOUT) check(rc);
+            update_progress();
+        }
         collect();
     }
     // SimulationManager.h:587-592
     bool isFinished() {
         if (!started_) return false;
         if (collected_) return true;
-        if (sp_wait(ctx_, id_, 0) != SP_OK) return false;
+        if (sp_wait(ctx_, id_, 0) != SP_OK) {
+            update_progress();
+            return false;
+        }
         collect();
         return true;
     }
@@ -74,6 +87,7 @@ public:
     static double& ProgressFactor() { static double v{1.0}; return v; }
     static std::atomic<std::size_t>& ProgressCache() { static std::atomic<std::size_t> v{0}; return v; }
 
+    const sp_request& request() const { return req_; }    // the run the parameter file describes
     const sp_result& result() const { return res_; }
     const std::vector<double>& time() const { return time_; }
     const std::vector<double>& mean() const { return mean_; }
@@ -84,6 +98,19 @@ public:
 private:
     static void check(int rc) {
         if (rc != SP_OK) throw std::runtime_error(std::string("stophys_gpu: ") + sp_last_error());
+    }
+    // SimulationManager.h:177-214 with the fraction of steps done in place of the fraction of finished
+    // simulations: the percentage (scaled by ProgressFactor, shifted by ProgressModifier) goes to ProgressCache
+    // and, when it changes, a 70-character bar to stderr
+    void update_progress() {
+        double f = 0.0;
+        if (sp_progress(ctx_, id_, &f) != SP_OK) return;
+        const auto progress = (std::size_t)std::llround(f * 100.0 * ProgressFactor() + ProgressModifier() * 100.0);
+        if (ProgressCache().exchange(progress) == progress) return;
+        constexpr int barlength = 70;
+        std::string bar(1, '[');
+        for (std::size_t i = 0; i < barlength; ++i) bar += i * 100 < progress * barlength ? '=' : ' ';
+        std::fprintf(stderr, "%s] %zu%%\n", bar.c_str(), progress);
     }
     void collect() {
         uint64_t n_out = 0;
@@ -115,10 +142,11 @@ private:
             check(sp_settings_write_single_results(settings_, &res_, path.c_str()));
             single_file_ = path;
         }
-        ProgressCache().store(100);
+        ProgressCache().store((std::size_t)std::llround(100.0 * ProgressFactor() + ProgressModifier() * 100.0));
         collected_ = true;
     }
 
+    int device_ = 0;
     sp_settings* settings_ = nullptr;
     sp_ctx* ctx_ = nullptr;
     sp_request req_{};

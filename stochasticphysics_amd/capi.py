@@ -64,7 +64,8 @@ class Result(C.Structure):
                 ("device_ms", C.c_double), ("latency_ms", C.c_double), ("particle_steps", C.c_double),
                 ("check_error", C.c_double), ("check_noise", C.c_double), ("used_fallback", C.c_uint32),
                 ("precision_used", C.c_uint32),
-                ("newton_failures", C.c_uint64), ("particle_means", C.c_double * 8)]
+                ("newton_failures", C.c_uint64), ("particle_means", C.c_double * 8),
+                ("check_bias", C.c_double), ("check_bias_se", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -80,7 +81,7 @@ EXPORTS = [
     "sp_host_normals", "sp_prepare", "sp_execute", "sp_release", "sp_settings_load", "sp_settings_request",
     "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free", "sp_settings_single_savefile",
     "sp_settings_write_single_results", "sp_settings_prepare", "sp_settings_submit", "sp_prepared_inputs",
-    "sp_ctx_set_reduction_limit", "sp_ctx_inject_failures", "sp_execute_noalloc",
+    "sp_ctx_set_reduction_limit", "sp_ctx_inject_failures", "sp_execute_noalloc", "sp_progress",
 ]
 
 _LIB = None
@@ -115,6 +116,7 @@ def lib():
         L.sp_wait.argtypes = [C.c_void_p, C.c_uint64, C.c_int]
         L.sp_result_get.argtypes = [C.c_void_p, C.c_uint64, P(Result)]
         L.sp_cancel.argtypes = [C.c_void_p, C.c_uint64]
+        L.sp_progress.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]
         L.sp_server_stats.argtypes = [C.c_void_p, d]
         L.sp_autotune.argtypes = [C.c_void_p, P(Request), P(Particles), P(Tuning), d]
         L.sp_dev_philox.argtypes = [C.c_void_p, P(C.c_uint32), P(C.c_uint32), C.c_uint64, P(C.c_uint32)]

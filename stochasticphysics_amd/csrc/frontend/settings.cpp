@@ -86,10 +86,12 @@ static int parse_enum_or_throw(const char* kind, const std::string& name) {
     return v;
 }
 
-static ParticleFile load_particle(const std::string& path) {
-    const ini::File f = ini::parse_file(path);
+// One particle type's parameters (Parameters::ParticleSimulationParameters) from the sections under `root` of f
+// ("" for a particle file of its own).
+static ParticleFile load_particle_from(const ini::File& f, const std::string& root) {
     ParticleFile p;
-    const std::string PP = "Particle_Properties";
+    const std::string R = root.empty() ? std::string() : root + ".";
+    const std::string PP = R + "Particle_Properties";
     p.T = num(f, PP, {"Temperature"}, 295.0, true);
     const std::string MP = PP + ".Magnetic_Properties";
     p.radius = num(f, MP, {"Magnetic_radius"}, 0, true);
@@ -110,13 +112,13 @@ static ParticleFile load_particle(const std::string& path) {
     const std::string HP = PP + ".Hydrodynamic_Properties";
     p.r_h = num(f, HP, {"Hydrodynamic_Radius"}, 0.0);
     p.visc = num(f, HP, {"Viscosity"}, 0.0);
-    const std::string IN = "Particle_Simulation_Initialization";   // ParticleSimulationInitSettings.h:70-91
+    const std::string IN = R + "Particle_Simulation_Initialization";   // ParticleSimulationInitSettings.h:70-91
     p.rand_orient = flag(f, IN, {"Use_random_initial_particle_orientation"}, false);
     p.rand_mag = flag(f, IN, {"Use_random_initial_particle_magnetisation"}, false);
     auto e = vec(f, IN, {"Initial_particle_orientation"});
     auto m = vec(f, IN, {"Initial_magnetisation_direction"});
     for (int i = 0; i < 3; ++i) { if (i < (int)e.size()) p.euler[i] = e[i]; if (i < (int)m.size()) p.mag[i] = m[i]; }
-    const std::string SS = "Particle_Simulation_Settings";         // ParticleSimulationSettings.h:156-200
+    const std::string SS = R + "Particle_Simulation_Settings";         // ParticleSimulationSettings.h:156-200
     if (const std::string* v = opt(f, SS, {"Magnetic_radius_distribution_type"})) p.rad.type = *v;
     p.rad.width = num(f, SS, {"Magnetic_radius_distribution_width"}, 0.0);
     p.rad.relative = flag(f, SS, {"Use_relative_distribution_width_for_magnetic_radius"}, true);
@@ -142,6 +144,30 @@ static ParticleFile load_particle(const std::string& path) {
     p.aniso_dist.width = w.empty() ? 0.0 : w[0];
     p.aniso_dist.relative = flag(f, SS, {"Use_relative_distribution_width", "Use_relative_distribution_width_for_anisotropy_constants"}, true);
     return p;
+}
+
+static bool file_exists(const std::string& path) {
+    std::ifstream in(path);
+    return in.good();
+}
+
+// The particle type `name` of a parameter file (ParticleProvider.h:267-282): from the particle file its
+// Particle_List entry names -- a relative path is taken relative to the parameter file's directory when the file
+// is there, else relative to the working directory, as the reference opens it -- or, for an empty entry, from the
+// sections under [name] of the parameter file itself (or under [Particle_Provider.name], where the reference's
+// writer puts the particles it saves in the same file, ParticleProvider.h:126-138).
+static ParticleFile load_particle(const ini::File& f, const std::string& name, const std::string& file) {
+    if (file.empty()) {
+        for (const std::string root : {name, std::string("Particle_Provider.") + name})
+            if (f.has_section(root + ".Particle_Properties")) return load_particle_from(f, root);
+        throw std::runtime_error("particle type '" + name + "' has no file and no [" + name + ".Particle_Properties]");
+    }
+    std::string path = file;
+    if (file[0] != '/') {
+        const std::string beside = f.path + "/" + file;
+        path = file_exists(beside) || !file_exists(file) ? beside : file;
+    }
+    return load_particle_from(ini::parse_file(path), "");
 }
 
 // ---- particle types for the provider
@@ -316,8 +342,7 @@ Settings* load(const std::string& path) {
     auto nums = f.sections.find(PP + ".Particle_Numbers");
     if (list == f.sections.end()) throw std::runtime_error("missing [Particle_Provider.Particle_List]");
     for (auto& [name, file] : list->second) {
-        const std::string full = (!file.empty() && file[0] == '/') ? file : f.path + "/" + file;
-        ParticleFile pf = load_particle(full);
+        ParticleFile pf = load_particle(f, name, file);
         if (nums != f.sections.end() && nums->second.count(name)) pf.count = (uint64_t)ini::to_double(nums->second.at(name));
         s->particles.push_back(pf);
     }
@@ -358,8 +383,9 @@ int write_result(const Settings& s, const sp_result& res, const std::string& pat
     o << buf;
     std::snprintf(buf, sizeof buf,
                   ",\"Engine\":{\"device_ms\":%.6f,\"particle_steps\":%.17g,\"precision_used\":%u,\"check_error\":%.6g,"
-                  "\"check_noise\":%.6g,\"used_fallback\":%u}}",
-                  res.device_ms, res.particle_steps, res.precision_used, res.check_error, res.check_noise, res.used_fallback);
+                  "\"check_noise\":%.6g,\"check_bias\":%.6g,\"check_bias_se\":%.6g,\"used_fallback\":%u}}",
+                  res.device_ms, res.particle_steps, res.precision_used, res.check_error, res.check_noise, res.check_bias,
+                  res.check_bias_se, res.used_fallback);
     o << buf << '\n';
     return o ? SP_OK : set_error(SP_ERR_INVALID_ARG, "write failed: " + path);
 }

@@ -13,6 +13,7 @@ and, for the simulation kernels' FP64 sine and cosine (not part of the noise str
 exact value (Taylor series in 60-digit decimal arithmetic, pi to 63 digits):
   * SP_TRIG64_TAB[k] = (sin(k pi / 64), cos(k pi / 64), and the doubles nearest to the remainders sin - hi and
     cos - hi), k = 0..127
+  * SP_ATAN32_TAB[k] = (atan(k / 32), the double nearest to the remainder atan - hi), k = 0..32
 """
 import math
 from decimal import Decimal, getcontext
@@ -38,6 +39,20 @@ def dec_sincos(x):
         n += 1
         t = t * x / n
     return s, c
+
+
+def dec_atan(x):
+    """atan x to about 58 digits for 0 <= x <= 1: three argument halvings atan x = 2 atan(x / (1 + sqrt(1 + x^2))),
+    then the Taylor series."""
+    halvings = 3
+    for _ in range(halvings):
+        x = x / (1 + (1 + x * x).sqrt())
+    s, t, n = Decimal(0), x, 0
+    while n < 400:
+        s += t / (2 * n + 1) if n % 2 == 0 else -t / (2 * n + 1)
+        t = t * x * x
+        n += 1
+    return s * (2 ** halvings)
 
 
 def nearest(d):
@@ -76,6 +91,11 @@ def main():
         sh, ch = nearest(sk), nearest(ck)
         sl, cl = nearest(sk - Decimal(sh)), nearest(ck - Decimal(ch))
         lines.append(f"    {{{hexd(sh)}, {hexd(ch)}, {hexd(sl)}, {hexd(cl)}}},")
+    lines += ["};", "", "// (atan_hi, atan_lo)(k / 32), k = 0..32", "static constexpr double SP_ATAN32_TAB[33][2] = {"]
+    for k in range(33):
+        a = dec_atan(Decimal(k) / 32)
+        hi = nearest(a)
+        lines.append(f"    {{{hexd(hi)}, {hexd(nearest(a - Decimal(hi)))}}},")
     lines += ["};", "", "}  // namespace sp", ""]
     out = Path(__file__).resolve().parent / "sp_rng_tables.h"
     out.write_text("\n".join(lines))

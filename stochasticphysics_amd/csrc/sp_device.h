@@ -107,17 +107,19 @@ struct BMTables {
     double log_tab[97][2];
     double sc_tab[128][2];
     double trig_tab[128][4];     // (sin, cos)(k pi / 64) as hi and lo parts, SP_TRIG64_TAB
+    double atan_tab[33][2];      // atan(k / 32) as hi and lo parts, SP_ATAN32_TAB
 };
 static __shared__ BMTables sp_bm_lds;
 
 __device__ __forceinline__ void bm_lds_fill() {
-    constexpr int NL = 97 * 2, NS = 128 * 2, NT = 128 * 4;
-    double* dst = &sp_bm_lds.log_tab[0][0];          // the three tables are contiguous
+    constexpr int NL = 97 * 2, NS = 128 * 2, NT = 128 * 4, NA = 33 * 2;
+    double* dst = &sp_bm_lds.log_tab[0][0];          // the four tables are contiguous
     const double* lt = &SP_LOG_TAB[0][0];
     const double* st = &SP_SINCOS_TAB[0][0];
     const double* tt = &SP_TRIG64_TAB[0][0];
-    for (int i = threadIdx.x; i < NL + NS + NT; i += blockDim.x)
-        dst[i] = i < NL ? lt[i] : (i < NL + NS ? st[i - NL] : tt[i - NL - NS]);
+    const double* at = &SP_ATAN32_TAB[0][0];
+    for (int i = threadIdx.x; i < NL + NS + NT + NA; i += blockDim.x)
+        dst[i] = i < NL ? lt[i] : (i < NL + NS ? st[i - NL] : (i < NL + NS + NT ? tt[i - NL - NS] : at[i - NL - NS - NT]));
     __syncthreads();
 }
 
@@ -435,6 +437,56 @@ static __device__ __attribute__((noinline)) double inv_tiny(double x) {
     const double r = 1.0 / x;
     return isinf(r) ? 0.0 : r;
 }
+static __device__ __attribute__((noinline)) double atan2_library(double y, double x) { return atan2(y, x); }
+
+// a + b = s + e exactly (Knuth's two-sum)
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// atan2(y, x) within about 1.8 ulp (the C library's on 96% of arguments): t = min(|x|, |y|) / max(|x|, |y|) with its
+// rounding error tl, t = c + (t - c) for the nearest c = k/32, atan t = atan c + atan u with u = (t - c)/(1 + t c),
+// |u| <= 1/64, by its series to u^9, plus tl/(1 + t^2); atan c from the LDS table as hi and lo parts; pi/2 - a and
+// pi - a for the other octants by exact two-sums.  Zeros, infinities, NaNs and arguments beyond 2^+-500 take the
+// library routine.
+__device__ __forceinline__ double atan2_fast(double y, double x) {
+    const double ax = fabs(x), ay = fabs(y);
+    const bool swap = ay > ax;
+    const double num = swap ? ax : ay, den = swap ? ay : ax;
+    if (!(den >= 0x1p-500 && den <= 0x1p500)) return atan2_library(y, x);
+    const double r = rcp_nr(den);
+    double t = num * r;
+    t = fma(fma(-den, t, num), r, t);                          // num / den, within half an ulp
+    const double tl = fma(-den, t, num) * r;                   // num / den - t
+    const double kd = rint(t * 32.0);
+    const double c = kd * 0.03125;
+    const double q = rcp_nr(fma(t, c, 1.0));
+    const double u = (t - c) * q;                              // t - c is exact
+    const double u2 = u * u;
+    const double p = fma(fma(fma(0.1111111111111111, u2, -0.14285714285714285), u2, 0.2), u2, -0.3333333333333333);
+    const double au = fma(u * u2, p, u);
+    const double* T = sp_bm_lds.atan_tab[(int)kd];
+    double hi = T[0], lo = T[1] + fma(tl, q * q * fma(c, c, 1.0), au);
+    if (swap) {
+        double s, e;
+        two_sum(0x1.921fb54442d18p+0, -hi, s, e);              // pi/2 (hi and lo parts) - a
+        hi = s;
+        lo = (0x1.1a62633145c07p-54 - lo) + e;
+    }
+    if (signbit(x)) {
+        double s, e;
+        two_sum(0x1.921fb54442d18p+1, -hi, s, e);              // pi - a
+        hi = s;
+        lo = (0x1.1a62633145c07p-53 - lo) + e;
+    }
+    return copysign(hi + lo, y);
+}
+
+// acos(c) for |c| <= 1 as atan2(sqrt((1 - c)(1 + c)), c)
+__device__ __forceinline__ double acos_fast(double c) { return atan2_fast(sqrt((1.0 - c) * (1.0 + c)), c); }
+
 // 1 / sin(theta) with an infinite value replaced by 0 (NeelRelaxationSpherical.h:156-160)
 __device__ __forceinline__ double inv_or_zero(double x) { return fabs(x) >= 0x1p-1020 ? rcp_nr(x) : inv_tiny(x); }
 __device__ __forceinline__ float inv_or_zero(float x) {
@@ -452,13 +504,26 @@ __device__ __forceinline__ double fmod_2pi(double x) {
 }
 
 template <class R> struct Trig { R st, ct, sp, cp; };
-// sin and cos of the angles (acos(c), atan2(ys, xs)) of a chart map, from c, ys and xs on the unit sphere
-// (c^2 + xs^2 + ys^2 = 1): sin acos(c) = hypot(xs, ys) >= 0 and (xs, ys) / hypot(xs, ys) are the cosine and sine
-// of atan2(ys, xs); where hypot(xs, ys) = 0, atan2(+-0, x) is +-0 for x = +0 and +-pi for x = -0
-__device__ __forceinline__ void chart_trig(double c, double ys, double xs, Trig<double>& T) {
-    const double h = sqrt(fma(xs, xs, ys * ys));
-    T.st = h;
+// A chart map of the spherical coordinates (NeelRelaxationSpherical.h:568-602): the angles
+// (acos(c), atan2(ys, xs)) of a point on the unit sphere (c^2 + xs^2 + ys^2 = 1), and their sines and cosines
+// without evaluating them: acos(c) = atan2(sqrt((1 - c)(1 + c)), c), whose sine is that root and whose cosine is c,
+// and (xs, ys) / hypot(xs, ys) are the cosine and sine of atan2(ys, xs); where hypot(xs, ys) = 0, atan2(+-0, x) is
+// +-0 for x = +0 and +-pi for x = -0.  FAST takes the table-driven atan2 (fewer registers: the FP32-physics kernels
+// gain occupancy from it), otherwise the library's acos and atan2 (as many instructions, and faster in the FP64
+// kernels, whose occupancy is fixed by their own register count)
+template <bool FAST>
+__device__ __forceinline__ void chart_map(double c, double ys, double xs, double Y[2], Trig<double>& T) {
+    const double s = sqrt((1.0 - c) * (1.0 + c));
+    if constexpr (FAST) {
+        Y[0] = atan2_fast(s, c);
+        Y[1] = atan2_fast(ys, xs);
+    } else {
+        Y[0] = acos(c);
+        Y[1] = atan2(ys, xs);
+    }
+    T.st = s;
     T.ct = c;
+    const double h = sqrt(fma(xs, xs, ys * ys));
     if (h > 0.0) {
         const double ih = rcp_nr(h);
         T.cp = xs * ih;
@@ -540,11 +605,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
             double st, ct, sp, cp;
             if constexpr (sizeof(R) == 8) { st = T.st; ct = T.ct; sp = T.sp; cp = T.cp; }   // the cached trig of y
             else { sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp); }
-            const double c = -cp * st, ys = sp * st;
-            Y[0] = acos(c);
-            Y[1] = atan2(ys, ct);
             Trig<double> Td;
-            chart_trig(c, ys, ct, Td);          // sin and cos of Y without evaluating them
+            chart_map<sizeof(R) == 4>(-cp * st, sp * st, ct, Y, Td);       // Y = (acos(-cp st), atan2(sp st, ct)) and its trig
             T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
         }
         return rot;
@@ -554,11 +616,8 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         if (rot) {
             double st, ct, sp, cp;
             sincos_fast(r[0], &st, &ct); sincos_fast(r[1], &sp, &cp);
-            const double c = cp * st, ys = sp * st, xs = -ct;
-            y[0] = acos(c);
-            y[1] = atan2(ys, xs);
             Trig<double> Td;
-            chart_trig(c, ys, xs, Td);
+            chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);       // y = (acos(cp st), atan2(sp st, -ct)) and its trig
             T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
         } else {
             y[0] = fmod_2pi(r[0]);
@@ -867,14 +926,23 @@ template <class Prob> struct IMCfg {
 // Eigen's normalize() on each 3-block of the state, in FP64: v / |v| when |v| > 0
 // (NeelRelaxation.h:204-208, BrownAndNeelRelaxation.h:168-172; a no-op for
 // NeelSpherical, NeelRelaxationSpherical.h:534-535).
+// 1 / sqrt(q) for a normal q > 0: the hardware reciprocal square root and two Newton corrections (within an ulp)
+__device__ __forceinline__ double rsqrt_nr(double q) {
+    double r = __builtin_amdgcn_rsq(q);
+    const double h = 0.5 * q;
+    r = r * fma(-h * r, r, 1.5);
+    return r * fma(-h * r, r, 1.5);
+}
+static __device__ __attribute__((noinline)) double rsqrt_library(double q) { return 1.0 / sqrt(q); }
+
 template <int NORM>
 __device__ __forceinline__ void normalize_blocks(double* y) {
 #pragma unroll
     for (int b = 0; b < NORM; ++b) {
         double* v = y + 3 * b;
         const double q = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
-        if (q > 0.0) {                       // one division: v * (1 / |v|), within an ulp of v / |v|
-            const double inv = 1.0 / sqrt(q);
+        if (q > 0.0) {                       // v * (1 / |v|), within an ulp of v / |v|
+            const double inv = (q >= 0x1p-1000 && q <= 0x1p1000) ? rsqrt_nr(q) : rsqrt_library(q);
             v[0] = v[0] * inv; v[1] = v[1] * inv; v[2] = v[2] * inv;
         }
     }

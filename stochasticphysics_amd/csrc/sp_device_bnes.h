@@ -240,11 +240,11 @@ __device__ __forceinline__ void bnes_noise(const BNESLane<ANISO, R>& L, const BC
 }
 
 // EulertoEulerRotated (:859-877), EulerRotatedtoEuler (:880-898), the 90 degree chart maps of the Neel angles
-// (:823-834, :844-855); all in double
+// (:823-834, :844-855); all in double, with the table-driven atan2 and acos
 __device__ __forceinline__ void euler_to_rotated(double S0, double C0, double S1, double C1, double S2, double C2, double* r) {
-    r[0] = atan2(-C2 * C0 + C1 * S0 * S2, S0 * C2 + C1 * C0 * S2);
-    r[1] = acos(-S1 * S2);
-    r[2] = atan2(C1, C2 * S1);
+    r[0] = atan2_fast(-C2 * C0 + C1 * S0 * S2, S0 * C2 + C1 * C0 * S2);
+    r[1] = acos_fast(-S1 * S2);
+    r[2] = atan2_fast(C1, C2 * S1);
 }
 __device__ __forceinline__ void euler_to_rotated(const double* y, double* r) {
     double S0, C0, S1, C1, S2, C2;
@@ -254,13 +254,13 @@ __device__ __forceinline__ void euler_to_rotated(const double* y, double* r) {
 __device__ __forceinline__ void rotated_to_euler(const double* y, double* r) {
     double S0, C0, S1, C1, S2, C2;
     sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
-    r[0] = atan2(C0 * C2 - C1 * S0 * S2, -C2 * S0 - C1 * C0 * S2);
-    r[1] = acos(S1 * S2);
-    r[2] = atan2(-C1, C2 * S1);
+    r[0] = atan2_fast(C0 * C2 - C1 * S0 * S2, -C2 * S0 - C1 * C0 * S2);
+    r[1] = acos_fast(S1 * S2);
+    r[2] = atan2_fast(-C1, C2 * S1);
 }
 __device__ __forceinline__ void neel_to_rotated(double st, double ct, double sp, double cp, double* r) {
-    r[0] = acos(-cp * st);
-    r[1] = atan2(sp * st, ct);
+    r[0] = acos_fast(-cp * st);
+    r[1] = atan2_fast(sp * st, ct);
 }
 __device__ __forceinline__ void neel_to_rotated(const double* y, double* r) {
     double st, ct, sp, cp;
@@ -270,8 +270,8 @@ __device__ __forceinline__ void neel_to_rotated(const double* y, double* r) {
 __device__ __forceinline__ void rotated_to_neel(const double* y, double* r) {
     double st, ct, sp, cp;
     sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
-    r[0] = acos(cp * st);
-    r[1] = atan2(sp * st, -ct);
+    r[0] = acos_fast(cp * st);
+    r[1] = atan2_fast(sp * st, -ct);
 }
 
 template <int ANISO, int SOLVER, class R> struct BNESCfg {

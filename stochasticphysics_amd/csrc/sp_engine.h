@@ -298,6 +298,7 @@ struct Outputs {
     std::vector<double> time, field, mean, weight, final_state, traj;
     std::vector<double> check_tiles;   // [n_out*dout][n_check_tiles] raw weighted tile sums
     double device_ms = 0.0, latency_ms = 0.0, particle_steps = 0.0, check_error = -1.0, check_noise = 0.0;
+    double check_bias = -1.0, check_bias_se = 0.0;
     uint32_t used_fallback = 0, precision_used = SP_PREC_FP64;
     uint64_t newton_failures = 0;
     bool check_failed = false;
@@ -365,6 +366,8 @@ inline void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
     r->particle_steps = o.particle_steps;
     r->check_error = o.check_error;
     r->check_noise = o.check_noise;
+    r->check_bias = o.check_bias;
+    r->check_bias_se = o.check_bias_se;
     r->used_fallback = o.used_fallback;
     r->precision_used = o.precision_used;
     r->newton_failures = o.newton_failures;
@@ -381,6 +384,7 @@ struct Job {
     bool keep_final = false;
     Clock::time_point t_submit;
     double device_ms_acc = 0.0;
+    uint64_t steps_done = 0;              // steps of the request's particles completed (chunk granularity)
 };
 
 struct Slot {                 // an admitted request (or the FP64 shadow of a mixed one)

@@ -231,7 +231,7 @@ static double t_quantile_999(uint32_t dof) {
 // rows k >= 1 and its standard error from the per-tile time averages; the result fails where
 // |B| - t_999(T-1) SE(B) > tol (|B| > tol when T < 4), or where the mixed result is not finite (FP32
 // overflow of a run-away particle).  A negative tol fails every check.
-struct CheckResult { double error = 0.0, noise = 0.0; bool fail = false; };
+struct CheckResult { double error = 0.0, noise = 0.0, bias = 0.0, bias_se = 0.0; bool fail = false; };
 static CheckResult check_of(const HostRequest& hr, const Outputs& mixed, const Outputs& shadow) {
     CheckResult r;
     const double tol = hr.rq.check_tol != 0.0 ? hr.rq.check_tol : 1e-3;
@@ -264,6 +264,8 @@ static CheckResult check_of(const HostRequest& hr, const Outputs& mixed, const O
         double ss = 0.0;
         for (uint32_t t = 0; t < T; ++t) { const double x = avg[t] - B / T; ss += x * x; }
         const double seB = T > 1 ? std::sqrt(ss * T / (T - 1)) : 0.0;
+        r.bias = std::max(r.bias, std::fabs(B));
+        r.bias_se = std::max(r.bias_se, seB);
         const double margin = T >= 4 ? t_quantile_999(T - 1) * seB : 0.0;
         if (std::fabs(B) - margin > tol || tol < 0.0) r.fail = true;
     }
@@ -402,6 +404,8 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
             const CheckResult cr = check_of(reqs[i], outs[i], so);
             outs[i].check_error = cr.error;
             outs[i].check_noise = cr.noise;
+            outs[i].check_bias = cr.bias;
+            outs[i].check_bias_se = cr.bias_se;
             outs[i].check_failed = cr.fail;
         }
     }
@@ -427,6 +431,8 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
             const CheckResult cr = check_of(*sl->hr, o, so);
             o.check_error = cr.error;
             o.check_noise = cr.noise;
+            o.check_bias = cr.bias;
+            o.check_bias_se = cr.bias_se;
             if (cr.fail && !(sl->hr->rq.flags & SP_FLAG_NO_FALLBACK)) requeue = true;
         }
     }
@@ -435,6 +441,8 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
         // per-request FP64 fallback: run the request again in FP64 ahead of the queue
         job->out.check_error = o.check_error;
         job->out.check_noise = o.check_noise;
+        job->out.check_bias = o.check_bias;
+        job->out.check_bias_se = o.check_bias_se;
         job->out.used_fallback = 1;
         job->hr.rq.precision = SP_PREC_FP64;
         job->hr.mixed = false;
@@ -444,10 +452,15 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
         return;
     }
     if (rc == SP_OK) {
-        const double fallback_err = job->out.check_error, fallback_noise = job->out.check_noise;
-        const uint32_t fb = job->out.used_fallback;
+        const Outputs prev = job->out;
         job->out = std::move(o);
-        if (fb) { job->out.used_fallback = 1; job->out.check_error = fallback_err; job->out.check_noise = fallback_noise; }
+        if (prev.used_fallback) {
+            job->out.used_fallback = 1;
+            job->out.check_error = prev.check_error;
+            job->out.check_noise = prev.check_noise;
+            job->out.check_bias = prev.check_bias;
+            job->out.check_bias_se = prev.check_bias_se;
+        }
         job->out.device_ms = job->device_ms_acc;
         job->out.latency_ms = ms_since(job->t_submit, Clock::now());
         job->status = SP_OK;
@@ -584,6 +597,11 @@ static void worker_main(sp_ctx* ctx) {
                 x->job->device_ms_acc += ms * x->work_share / P.particle_steps;
                 x->work_share = 0.0;
             }
+            if (rc == SP_OK) {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                for (Slot* x : slots)
+                    if (!x->is_shadow && x->job) x->job->steps_done = x->next_step;
+            }
             if (rc != SP_OK) {
                 // every slot of the failed iteration ends, FP64 shadows included: a shadow left running would
                 // later publish its parent's partial sums
@@ -685,6 +703,8 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs, const sp_p
             if (int e = run_plain(ctx, h2, o2, keep_final, t)) return e;
             o2.check_error = outs[i].check_error;
             o2.check_noise = outs[i].check_noise;
+            o2.check_bias = outs[i].check_bias;
+            o2.check_bias_se = outs[i].check_bias_se;
             o2.used_fallback = 1;
             outs[i] = std::move(o2);
         }
@@ -764,6 +784,18 @@ int sp_cancel(sp_ctx* ctx, uint64_t id) {
         it->second->cancel = true;
     }
     ctx->cv.notify_all();
+    return SP_OK;
+}
+
+int sp_progress(sp_ctx* ctx, uint64_t id, double* fraction) {
+    if (!ctx || !fraction) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->jobs.find(id);
+    if (it == ctx->jobs.end()) return set_error(SP_ERR_NOT_FOUND, "unknown request id");
+    const Job& j = *it->second;
+    if (j.status == SP_OK) *fraction = 1.0;
+    else if (j.hr.total_steps == 0) *fraction = 0.0;
+    else *fraction = (double)std::min(j.steps_done, j.hr.total_steps) / (double)j.hr.total_steps;
     return SP_OK;
 }
 

@@ -186,7 +186,8 @@ def _alloc_result(e: Ensemble, keep_final):
 
 def _finish_result(res: Result, o):
     o.update(device_ms=res.device_ms, latency_ms=res.latency_ms, particle_steps=res.particle_steps,
-             check_error=res.check_error, check_noise=res.check_noise, used_fallback=res.used_fallback,
+             check_error=res.check_error, check_noise=res.check_noise, check_bias=res.check_bias,
+             check_bias_se=res.check_bias_se, used_fallback=res.used_fallback,
              precision_used=res.precision_used,
              newton_failures=res.newton_failures, n_saved=res.n_saved)
     return o
@@ -413,6 +414,11 @@ class Context:
 
     def cancel(self, rid):
         return lib().sp_cancel(self._ctx, rid)
+
+    def progress(self, rid):
+        f = C.c_double()
+        check(lib().sp_progress(self._ctx, rid, C.byref(f)))
+        return f.value
 
     def server_stats(self):
         s = np.zeros(8)
