@@ -580,7 +580,9 @@ __device__ __forceinline__ void make_chart(const SphLane<ANISO, R>& L, const Tri
 template <int ANISO, int SOLVER, class R> struct SphCfg {
     static constexpr int DY = 2, DW = 3, DOUT = 3, NORM = 0;
     static constexpr bool PIPE_NOISE = false;
-    static constexpr int WAVES_PER_SIMD = 2;
+    // uniaxial and cubic: FP64 at three waves per SIMD (their translation unit is built so that it fits 168
+    // registers without spilling), FP32 physics at four; uniaxialcubic at two
+    static constexpr int WAVES_PER_SIMD = ANISO == 4 ? 2 : (sizeof(R) == 8 ? 3 : 4);
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = SphLane<ANISO, R>;
     using Cache = Trig<R>;          // trig of the current (unrotated) state

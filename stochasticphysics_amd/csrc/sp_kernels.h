@@ -30,6 +30,10 @@ bool launch_batch_sph(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, cons
 bool launch_batch_bn(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool info_neel(uint32_t, KernelInfo*);
 bool info_sph(uint32_t, KernelInfo*);
+// the FP64 explicit NeelSpherical classes, a translation unit of their own (launch_plain_sph hands them on)
+bool launch_plain_sph64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
+bool launch_batch_sph64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
+bool info_sph64(uint32_t, KernelInfo*);
 bool info_bn(uint32_t, KernelInfo*);
 bool launch_plain_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
