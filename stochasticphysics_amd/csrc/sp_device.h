@@ -332,6 +332,7 @@ __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
     static constexpr int DY = 6, DW = 6, DOUT = 6, NORM = 2;
+    static constexpr int WAVES_PER_SIMD = sizeof(R) == 8 ? 2 : 4;   // FP64: two; FP32 physics: four
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNLane<ANISO, R>;
     struct Cache {};

@@ -158,8 +158,10 @@ inline int prepare_request(const sp_request* rq, const sp_particles* pt, HostReq
     hr.params.assign((size_t)NPARAM * hr.N, 0.0);
     compute_params(*rq, *pt, hr.params.data(), hr.N);
     hr.y0.resize((size_t)hr.dy * hr.N);
-    for (uint64_t p = 0; p < hr.N; ++p)
-        for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
+    parallel_ranges(hr.N, [&](uint64_t p0, uint64_t p1) {
+        for (uint64_t p = p0; p < p1; ++p)
+            for (uint32_t i = 0; i < hr.dy; ++i) hr.y0[i * hr.N + p] = pt->initial_state[p * hr.dy + i];
+    });
     const uint64_t n_check = std::min<uint64_t>(hr.N, 64ull * hr.n_check_tiles);
     const double* mu = &hr.params[P_MU * hr.N];
     const double mu_sum = weight_sum_fixed(mu, hr.N), mu_sum_check = weight_sum_fixed(mu, n_check);

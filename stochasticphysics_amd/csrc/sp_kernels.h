@@ -20,6 +20,8 @@ bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
 bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
                   const SlotWork* slots, uint32_t n_slots, uint32_t n_tiles, uint32_t* counter);
 bool kernel_info(uint32_t kind, KernelInfo* info);
+// SIMDs of the current device (four per compute unit; queried once per device)
+uint32_t device_simds();
 
 // per-problem registries (one translation unit each)
 bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
@@ -34,6 +36,10 @@ bool info_sph(uint32_t, KernelInfo*);
 bool launch_plain_sph64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_sph64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 bool info_sph64(uint32_t, KernelInfo*);
+// the FP64 and ImplicitMidpoint BrownAndNeel classes, a translation unit of their own (launch_plain_bn hands them on)
+bool launch_plain_bn64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
+bool launch_batch_bn64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
+bool info_bn64(uint32_t, KernelInfo*);
 bool info_bn(uint32_t, KernelInfo*);
 bool launch_plain_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_uc(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);

@@ -1,7 +1,29 @@
 // sp_kernels_common.hip -- reduction, dispatch over the per-problem registries, test hooks.
 #include "sp_kernels_impl.h"
 
+#include <atomic>
+
 namespace sp {
+
+uint32_t device_simds() {
+    static std::atomic<uint32_t> simds[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return 1024;                                   // an MI355X: 256 compute units
+    }
+    uint32_t v = simds[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+        v = 4u * (uint32_t)cus;
+        simds[dev].store(v, std::memory_order_relaxed);
+    }
+    return v;
+}
 
 bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                   uint64_t se, double* partial, uint32_t pstride) {

@@ -11,6 +11,10 @@ template <class Cfg, class = void> struct waves_per_simd { static constexpr int 
 template <class Cfg> struct waves_per_simd<Cfg, std::void_t<decltype(Cfg::WAVES_PER_SIMD)>> {
     static constexpr int value = Cfg::WAVES_PER_SIMD;
 };
+// A kernel class compiled for one wave per SIMD (all registers of the SIMD to one wave): the plain launch takes it
+// when the launch has no more waves than the device has SIMDs, where a second wave per SIMD could not be resident
+// anyway and the smaller register budget of the class's own build would only cost spills.
+template <class Cfg> struct OneWave : Cfg { static constexpr int WAVES_PER_SIMD = 1; };
 
 // One wave per 64-particle tile of one request.
 template <class Cfg, class R, bool KAHAN, bool MID>
@@ -92,6 +96,14 @@ bool fill_info(KernelInfo* info, const char* name) {
 }  // namespace sp
 
 // expands the per-class switch cases for one (problem, solver, aniso, mixed) class
+// the plain launch of a class that also has its one-wave build (OneWave)
+#define SP_CASE_PLAIN_FIT(KIND, CFG, R, KAHAN, MID)                                                    \
+    case KIND:                                                                                           \
+        if ((uint64_t)grid.x * block.x <= 64ull * device_simds())                                        \
+            k_plain<OneWave<CFG>, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);  \
+        else                                                                                             \
+            k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);            \
+        return true;
 #define SP_CASE_PLAIN(KIND, CFG, R, KAHAN, MID)                                                        \
     case KIND:                                                                                           \
         k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);              \

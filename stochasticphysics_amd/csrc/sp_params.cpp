@@ -24,10 +24,14 @@ int request_dims(int problem, uint32_t* dy, uint32_t* dw, uint32_t* dout) {
 }
 
 void compute_params(const sp_request& rq, const sp_particles& pt, double* out, uint64_t stride) {
-    const uint64_t N = rq.n_particles;
+    parallel_ranges(rq.n_particles, [&](uint64_t p0, uint64_t p1) { compute_params_range(rq, pt, out, stride, p0, p1); });
+}
+
+void compute_params_range(const sp_request& rq, const sp_particles& pt, double* out, uint64_t stride, uint64_t p0,
+                          uint64_t p1) {
     ParticleProps q{};
     double v[NPARAM];
-    for (uint64_t p = 0; p < N; ++p) {
+    for (uint64_t p = p0; p < p1; ++p) {
         q.radius = pt.magnetic_radius[p]; q.Ms = pt.saturation_magnetisation[p]; q.alpha = pt.damping[p];
         q.gamma = pt.gyromagnetic_ratio[p]; q.T = pt.temperature[p]; q.K = pt.anisotropy_constant[p];
         q.r_h = pt.hydrodynamic_radius ? pt.hydrodynamic_radius[p] : 0.0;
