@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define SP_ABI_VERSION 7
+#define SP_ABI_VERSION 8
 
 /* ---- status codes (the reference throws std::runtime_error instead,
  *      e.g. Settings/SolverSettings.h:139, Fields/LissajousField.h:56) ---- */
@@ -195,10 +195,9 @@ typedef struct sp_result {
 /* execution knobs chosen by the autotuner (or by hand) */
 typedef struct sp_tuning {
     uint32_t block_size;       /* work-group size: 64, 128, 192 or 256 */
-    uint32_t particles_per_lane; /* 1 or 2 */
     uint32_t chunk_steps;      /* steps advanced per request per scheduler iteration / launch */
     uint32_t use_graph;        /* capture chunk launches in a hipGraph (static batches) */
-    uint32_t persistent_waves_per_simd; /* persistent grid = CUs * 4 * this */
+    uint32_t persistent_waves_per_simd; /* persistent grid = CUs * 4 * this (1..8) */
 } sp_tuning;
 
 typedef struct sp_ctx sp_ctx;

@@ -69,7 +69,7 @@ class Result(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block_size", C.c_uint32), ("particles_per_lane", C.c_uint32), ("chunk_steps", C.c_uint32),
+    _fields_ = [("block_size", C.c_uint32), ("chunk_steps", C.c_uint32),
                 ("use_graph", C.c_uint32), ("persistent_waves_per_simd", C.c_uint32)]
 
 

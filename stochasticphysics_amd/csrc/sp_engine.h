@@ -461,7 +461,6 @@ inline uint64_t tuning_key(int problem, int solver, int precision) { return ((ui
 inline sp_tuning default_tuning(const sp_ctx* ctx, const HostRequest& hr) {
     sp_tuning t{};
     t.block_size = 256;
-    t.particles_per_lane = 1;
     t.chunk_steps = 4000;
     t.use_graph = 1;
     t.persistent_waves_per_simd = 2;

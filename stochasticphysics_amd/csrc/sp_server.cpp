@@ -830,7 +830,7 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
     if (int e = prepare_request(&r, parts, probe)) return e;
     const int prec = probe.mixed ? SP_PREC_MIXED : SP_PREC_FP64;
     const uint32_t blocks[] = {64, 128, 256};          // the kernels' __launch_bounds__ is MAX_BLOCK = 256
-    const uint32_t waves[] = {1, 2, 4};
+    const uint32_t waves[] = {1, 2, 3, 4};   // the kernel classes are built for 1 to 4 waves per SIMD
     const uint32_t chunks[] = {2000, 8000, 32000};
     double best_r = -1.0;
     sp_tuning bt{};
@@ -859,7 +859,7 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
     for (uint32_t c : chunks)
         for (uint32_t b : blocks)
             for (uint32_t w : waves) {
-                sp_tuning t{b, 1, c, 1, w};
+                sp_tuning t{b, c, 1, w};
                 trial(t);                        // warm (allocations, code load)
                 const double rate = err == SP_OK ? trial(t) : -1.0;
                 if (err != SP_OK) {               // a failed trial ends the search with its error

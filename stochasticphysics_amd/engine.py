@@ -443,7 +443,7 @@ class Context:
         r, p, keep = e.to_c()
         t, rate = Tuning(), C.c_double()
         check(lib().sp_autotune(self._ctx, C.byref(r), C.byref(p), C.byref(t), C.byref(rate)))
-        return dict(block_size=t.block_size, particles_per_lane=t.particles_per_lane, chunk_steps=t.chunk_steps,
+        return dict(block_size=t.block_size, chunk_steps=t.chunk_steps,
                     use_graph=t.use_graph, persistent_waves_per_simd=t.persistent_waves_per_simd), rate.value
 
     # ---- noise test hooks

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert lib().sp_abi_version() == 7
+    assert lib().sp_abi_version() == 8
 
 
 @pytest.mark.parametrize("kind,name,value", [("solver", "EulerMaruyama", 1), ("solver", "Implicit_Midpoint", 3),

@@ -88,7 +88,7 @@ def test_tuning_outside_the_kernels_launch_bounds_is_rejected():
     ctx = Context(0)
     try:
         for bad in (dict(block_size=512), dict(block_size=96), dict(chunk_steps=0)):
-            kw = dict(block_size=128, particles_per_lane=1, chunk_steps=1000, use_graph=1, persistent_waves_per_simd=1)
+            kw = dict(block_size=128, chunk_steps=1000, use_graph=1, persistent_waves_per_simd=1)
             kw.update(bad)
             with pytest.raises(SpError):
                 ctx.set_tuning("Neel", "Heun_Strong", "fp64", **kw)

@@ -52,7 +52,7 @@ def test_batch_graph_and_direct_launch_agree(ctx):
     bag = mixed_bag()[:3]
     g = ctx.run_batch(bag)
     for e in bag:
-        ctx.set_tuning(e.problem, e.solver, e.precision, block_size=128, particles_per_lane=1, chunk_steps=1000,
+        ctx.set_tuning(e.problem, e.solver, e.precision, block_size=128, chunk_steps=1000,
                        use_graph=0, persistent_waves_per_simd=1)
     d = ctx.run_batch(bag)
     for x, y in zip(g, d):
@@ -156,7 +156,7 @@ def test_whole_pass_and_chunked_reduction_agree(ctx):
     m = sph_ensemble(n=700, n_steps=5000, solver="Heun_Strong", seed=72)
     m.precision = "mixed"
     for x in (e, m):
-        ctx.set_tuning(x.problem, x.solver, x.precision, block_size=256, particles_per_lane=1, chunk_steps=1000,
+        ctx.set_tuning(x.problem, x.solver, x.precision, block_size=256, chunk_steps=1000,
                        use_graph=1, persistent_waves_per_simd=2)
     try:
         whole = [ctx.run(x) for x in (e, m)]

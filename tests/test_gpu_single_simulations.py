@@ -34,7 +34,7 @@ def test_trajectories_match_oracle(make):
     e = make()
     e.save_interval = 7
     with Context(0) as c:
-        c.set_tuning(e.problem, e.solver, "fp64", block_size=256, particles_per_lane=1, chunk_steps=1000, use_graph=1,
+        c.set_tuning(e.problem, e.solver, "fp64", block_size=256, chunk_steps=1000, use_graph=1,
                      persistent_waves_per_simd=2)
         g = c.run(e, keep_final=True)
     o = run_oracle(e, per_particle=True)
