@@ -265,13 +265,21 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
     st = {k: st1[k] - st0[k] for k in ("busy_ms", "iterations", "completed")}
     lat = np.array(lat)
     offered = sum(e.particle_steps() for e in reqs) / arrivals[-1]
+    by_solver = {}
+    for solver in ("Heun_Strong", "Implicit_Midpoint"):
+        x = np.array([l for l, e in zip(lat, reqs) if e.solver == solver])
+        if len(x):
+            by_solver[solver] = dict(requests=int(len(x)), p50_ms=float(np.percentile(x, 50)),
+                                     p99_ms=float(np.percentile(x, 99)))
     log(f"bench: serving p50 {np.percentile(lat, 50):.1f} ms p99 {np.percentile(lat, 99):.1f} ms, "
-        f"{ps / wall:.3e} particle-steps/s, busy {st['busy_ms'] / 1e3:.1f} s of {wall:.1f} s")
+        f"{ps / wall:.3e} particle-steps/s, busy {st['busy_ms'] / 1e3:.1f} s of {wall:.1f} s; "
+        + ", ".join(f"{k} p50 {v['p50_ms']:.1f} p99 {v['p99_ms']:.1f} ms" for k, v in by_solver.items()))
     return dict(requests=n_req, offered_req_per_s=rate, arrival_window_s=float(arrivals[-1]), submit_s=t_sub,
                 wall_s=wall, p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                 mean_ms=float(lat.mean()), throughput=ps / wall, unit="particle-steps/s",
                 offered_particle_steps_per_s=offered, device_busy_frac=st["busy_ms"] / (wall * 1e3),
                 implicit_midpoint_requests=im, fp64_fallbacks=fallbacks, iterations=int(st["iterations"]),
+                latency_by_solver=by_solver,
                 request_sizes="N log-uniform [256, 4096], steps U[2e3, 1e4], OS 100; NeelSpherical cubic Lissajous, "
                               "mixed (Heun_Strong) or FP64 Implicit_Midpoint for damping >= 0.3")
 
@@ -288,7 +296,8 @@ def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55):
     for k, frac in enumerate((0.5, 0.8)):
         r = serving_leg(ctx, configs, low_seconds, frac * capacity / per_request, seed + 1 + k)
         loads.append({key: r[key] for key in ("requests", "offered_req_per_s", "p50_ms", "p99_ms", "mean_ms", "throughput",
-                                              "offered_particle_steps_per_s", "device_busy_frac", "fp64_fallbacks")})
+                                              "offered_particle_steps_per_s", "device_busy_frac", "fp64_fallbacks",
+                                              "latency_by_solver")})
         loads[-1]["offered_fraction_of_capacity"] = r["offered_particle_steps_per_s"] / capacity
     main.update(capacity_particle_steps_per_s=capacity, lower_loads=loads)
     return main
@@ -390,6 +399,31 @@ def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
         log(f"bench: {name}: fp64 {row['fp64']:.3e}, mixed {row['mixed']:.3e} particle-steps/s")
         out[name] = row
     return dict(workload=f"{n} particles x {n_steps} steps per ensemble (inputs resident, mixed without its FP64 check)",
+                unit="particle-steps/s", results=out)
+
+
+def implicit_rates(ctx, configs, n=1_048_576, n_steps=4_000):
+    """The serving physics (NeelSpherical, cubic, Lissajous, chart rotation) with damping 0.4, where the serving
+    distribution takes Implicit_Midpoint: Implicit_Midpoint (FP64 Newton with dual-number Jacobians, tolerance 1e-10,
+    at most 20 iterations) beside Heun_Strong FP64 on the same particles, inputs resident; Newton failures are the
+    particle-steps that end at the iteration limit."""
+    rng = np.random.default_rng(1)
+    e = configs.serving_request(rng, 0, n_range=(n, n), step_range=(n_steps, n_steps + 1), precision="fp64")
+    e.alpha[:] = 0.4
+    out = {}
+    for solver in ("Implicit_Midpoint", "Heun_Strong"):
+        e.solver = solver
+        p = ctx.prepare(e)
+        p.execute()
+        t0 = time.perf_counter()
+        r = p.execute()
+        dt = time.perf_counter() - t0
+        p.release()
+        out[solver] = dict(value=e.particle_steps() / dt, newton_failures=int(r["newton_failures"]),
+                           failure_fraction=r["newton_failures"] / e.particle_steps())
+        log(f"bench: {solver} fp64: {out[solver]['value']:.3e} particle-steps/s, Newton failures "
+            f"{out[solver]['newton_failures']}")
+    return dict(workload=f"NeelSpherical cubic serving physics, damping 0.4, {n} particles x {e.n_steps} steps",
                 unit="particle-steps/s", results=out)
 
 
@@ -514,8 +548,10 @@ def main():
                     help="also run the 512 sets as the reference integrates them (no renormalisation) and count the "
                          "sets whose FP64 mean stays finite")
     ap.add_argument("--problems", action="store_true", help="headline run with the per-problem leg")
-    ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems", "provider"), default="headline",
+    ap.add_argument("--workload", choices=("headline", "precision", "neel", "problems", "provider", "implicit"),
+                    default="headline",
                     help="precision: the NeelSpherical precision study; neel: the 4,096-particle Neel relaxation; "
+                         "implicit: Implicit_Midpoint beside Heun_Strong on the serving physics (1M particles); "
                          "problems: every problem class on a 65,536-particle ensemble; provider: particles of a "
                          "parameter file drawn on the device vs on the host (each prints its own JSON line instead "
                          "of the headline one)")
@@ -551,6 +587,8 @@ def main():
             res = problem_rates(ctx, configs)
         elif args.workload == "provider":
             res = provider_leg(ctx)
+        elif args.workload == "implicit":
+            res = implicit_rates(ctx, configs, *((args.particles,) if args.particles != 1_048_576 else ()))
         else:
             res = neel_relaxation(ctx, configs)
         ctx.close()

@@ -236,6 +236,10 @@ int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
  * Results do not depend on it.  No reference counterpart (the reference merges per-particle trajectories
  * under a mutex, SimulationManager.h:170-175). */
 int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes);
+/* server (sp_submit): each kernel class's steps per scheduler iteration are cut, from its tuned chunk_steps, so
+ * that its launch takes about `ms` by the class's measured cost per particle-step: requests of cheap classes then
+ * finish without waiting on long launches of expensive ones.  0 keeps the tuned chunk; default 10. */
+int sp_ctx_set_iteration_budget(sp_ctx* ctx, double ms);
 
 /* statistics of the last sp_run on this context: [0] kernel launches, [1] kernel ms,
  * [2] reduction + copy ms, [3] wall ms, [4] particle-steps, [5] VGPRs of the kernel,

@@ -82,6 +82,7 @@ EXPORTS = [
     "sp_settings_savefile", "sp_settings_write_result", "sp_settings_free", "sp_settings_single_savefile",
     "sp_settings_write_single_results", "sp_settings_prepare", "sp_settings_submit", "sp_prepared_inputs",
     "sp_ctx_set_reduction_limit", "sp_ctx_inject_failures", "sp_execute_noalloc", "sp_progress",
+    "sp_ctx_set_iteration_budget",
 ]
 
 _LIB = None
@@ -107,6 +108,7 @@ def lib():
         L.sp_ctx_get_tuning.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(Tuning)]
         L.sp_ctx_set_memory_budget.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32]
         L.sp_ctx_set_reduction_limit.argtypes = [C.c_void_p, C.c_uint64]
+        L.sp_ctx_set_iteration_budget.argtypes = [C.c_void_p, C.c_double]
         L.sp_ctx_inject_failures.argtypes = [C.c_void_p, C.c_uint32]
         L.sp_execute_noalloc.argtypes = [C.c_int, C.c_void_p, P(Request), C.c_void_p, C.c_void_p, C.c_void_p]
         L.sp_ctx_last_run_stats.argtypes = [C.c_void_p, d]

@@ -614,14 +614,16 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         }
         return rot;
     }
-    // finishCalculations (:440-471), then the trig of the new state
+    // finishCalculations (:440-471), then the trig of the new state.  After a rotated step the trig is evaluated
+    // from y itself, as prepareCalculations evaluates it from the state (:145-146) and as a launch starting from
+    // y does (init_cache): a run then gives the same bits however its steps are split into launches.
     __device__ __forceinline__ static void finish(const double r[2], bool rot, double* y, Trig<R>& T) {
         if (rot) {
             double st, ct, sp, cp;
             sincos_fast(r[0], &st, &ct); sincos_fast(r[1], &sp, &cp);
             Trig<double> Td;
-            chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);       // y = (acos(cp st), atan2(sp st, -ct)) and its trig
-            T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
+            chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);       // y = (acos(cp st), atan2(sp st, -ct))
+            trig_of(y[0], y[1], T);
         } else {
             y[0] = fmod_2pi(r[0]);
             y[1] = fmod_2pi(r[1]);

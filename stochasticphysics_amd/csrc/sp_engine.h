@@ -397,6 +397,7 @@ struct Slot {                 // an admitted request (or the FP64 shadow of a mi
     DevRequest dr{};
     uint64_t next_step = 0;
     uint64_t chunk = 0;       // steps per iteration
+    uint64_t tuned_chunk = 0; // the class's tuned chunk (the upper bound of `chunk`)
     bool first = true;
     bool done = false;
     bool is_shadow = false;
@@ -450,6 +451,12 @@ struct sp_ctx {
     uint64_t whole_pass_limit = 4ull << 30;
     // server: the next n iterations fail before their launches (a host-side hook for the error paths)
     uint32_t inject_failures = 0;
+    // server: each kernel class's chunk is sized so that its launch takes about this long (ms), at most its tuned
+    // chunk; 0 keeps the tuned chunk.  The worker thread owns the measured costs and the timing events.
+    double iteration_budget_ms = 10.0;
+    std::map<uint32_t, double> class_ns_per_ps;   // kernel class -> smoothed device ns per particle-step
+    static constexpr int N_CLASS_EV = 32;         // classes timed per iteration (further classes are not timed)
+    hipEvent_t class_ev[2 * N_CLASS_EV] = {};
     // last synchronous run statistics
     double last_stats[8] = {0};
 };

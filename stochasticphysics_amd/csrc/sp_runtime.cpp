@@ -311,6 +311,14 @@ int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes) {
     return SP_OK;
 }
 
+int sp_ctx_set_iteration_budget(sp_ctx* ctx, double ms) {
+    if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
+    if (!(ms >= 0.0) || !std::isfinite(ms)) return set_error(SP_ERR_INVALID_ARG, "iteration budget must be finite and >= 0");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->iteration_budget_ms = ms;
+    return SP_OK;
+}
+
 int sp_ctx_inject_failures(sp_ctx* ctx, uint32_t n_iterations) {
     if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
     std::lock_guard<std::mutex> lk(ctx->mu);

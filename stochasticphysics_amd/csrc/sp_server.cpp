@@ -41,6 +41,7 @@ struct IterPlan {
     std::vector<uint32_t> kinds;                      // one persistent launch per kind
     std::vector<uint32_t> kind_begin, kind_count;     // ranges into work (slot entries)
     std::vector<uint32_t> kind_tiles;                 // tiles per kind launch
+    std::vector<double> kind_ps;                      // particle-steps per kind launch
     std::vector<SlotWork> work;
     std::vector<ReduceWork> reduce;
     uint32_t max_rows = 0;
@@ -70,6 +71,7 @@ static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
             P.kind_begin.push_back((uint32_t)P.work.size());
             P.kind_count.push_back(0);
             P.kind_tiles.push_back(0);
+            P.kind_ps.push_back(0.0);
         }
         const uint64_t OS = hr.rq.oversampling;
         const uint64_t sb = s->next_step;
@@ -98,6 +100,7 @@ static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
         P.max_rows = std::max(P.max_rows, rw.n_rows);
         off += (size_t)rows * hr.dout * hr.n_tiles;
         P.particle_steps += (double)hr.N * (double)(se - sb);
+        P.kind_ps.back() += (double)hr.N * (double)(se - sb);
         s->work_share = (double)hr.N * (double)(se - sb);
         s->next_step = se;
         if (se >= hr.total_steps) P.finishing.push_back(s);
@@ -115,8 +118,11 @@ static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning&
 }
 
 // Issue the launches of one iteration whose tables are already on the device.
+// timing: record a timing-event pair around each class's launch (ctx->class_ev; the server's iterations, not captured
+// in a graph)
 static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const SlotWork* d_work,
-                            const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun) {
+                            const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun,
+                            bool timing = false) {
     if (!P.kinds.empty()) launch_zero_u32(s, d_counters, (uint32_t)P.kinds.size());
     // fork: the classes of this iteration run side by side (the persistent launches of classes with few
     // waves, such as FP64 shadows, would otherwise serialise behind the others)
@@ -130,9 +136,12 @@ static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const
         const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, t.block_size / 64 * 64));
         const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
         hipStream_t sk = k == 0 ? s : ctx->side[(k - 1) % sp_ctx::N_SIDE];
+        const bool timed = timing && k < (size_t)sp_ctx::N_CLASS_EV;
+        if (timed) SP_HIP(hipEventRecord(ctx->class_ev[2 * k], sk));
         if (!launch_batch(P.kinds[k], grid, dim3(block), sk, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
                           d_counters + k))
             return set_error(SP_ERR_UNSUPPORTED, "no batch kernel for a requested class");
+        if (timed) SP_HIP(hipEventRecord(ctx->class_ev[2 * k + 1], sk));
     }
     for (int i = 0; i < n_side; ++i) {           // join
         SP_HIP(hipEventRecord(ctx->join_ev[i], ctx->side[i]));
@@ -279,6 +288,45 @@ static uint64_t chunk_for(const HostRequest& hr, const sp_tuning& t) {
 }
 
 static bool wants_shadow(const HostRequest& hr) { return hr.mixed && !(hr.rq.flags & SP_FLAG_SKIP_CHECK); }
+
+// Chunks of the next server iteration.  Every slot starts from its class's tuned chunk; with a budget, a class whose
+// launch would take longer than the budget at that chunk (by its measured cost per particle-step times the
+// particles of its slots) gets the largest aligned chunk that fits, at least one aligned block of steps.  A class
+// not measured yet keeps the tuned chunk.  Each launch then ends within about the budget, so requests of cheap
+// classes are not held in long iterations by an expensive one (such as Implicit_Midpoint beside Heun_Strong).
+static void assign_chunks(const sp_ctx* ctx, double budget_ms, const std::vector<Slot*>& slots) {
+    std::map<uint32_t, double> n_of_kind;
+    for (Slot* s : slots)
+        if (!s->done) n_of_kind[s->hr->kind] += (double)s->hr->N;
+    for (Slot* s : slots) {
+        if (s->done) continue;
+        s->chunk = s->tuned_chunk;
+        if (budget_ms <= 0.0) continue;
+        const auto it = ctx->class_ns_per_ps.find(s->hr->kind);
+        if (it == ctx->class_ns_per_ps.end() || !(it->second > 0.0)) continue;
+        const double fit = budget_ms * 1e6 / (it->second * n_of_kind[s->hr->kind]);
+        if (fit >= (double)s->tuned_chunk) continue;
+        const uint64_t align = lcm64(s->hr->rq.oversampling, 4);
+        s->chunk = std::max<uint64_t>(align, (uint64_t)fit / align * align);
+    }
+}
+
+// smoothed device cost per particle-step of each class timed in the iteration just finished
+static void update_class_costs(sp_ctx* ctx, const IterPlan& P) {
+    const size_t n = std::min<size_t>(P.kinds.size(), (size_t)sp_ctx::N_CLASS_EV);
+    for (size_t k = 0; k < n; ++k) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ctx->class_ev[2 * k], ctx->class_ev[2 * k + 1]) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        if (!(P.kind_ps[k] > 0.0) || !(ms > 0.f)) continue;
+        const double ns = (double)ms * 1e6 / P.kind_ps[k];
+        auto it = ctx->class_ns_per_ps.find(P.kinds[k]);
+        if (it == ctx->class_ns_per_ps.end()) ctx->class_ns_per_ps[P.kinds[k]] = ns;
+        else it->second = 0.5 * it->second + 0.5 * ns;
+    }
+}
 
 // ------------------------------------------------------------------ static batch (graph)
 static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::vector<Outputs>& outs, bool keep_final, bool use_graph) {
@@ -478,6 +526,12 @@ static void worker_main(sp_ctx* ctx) {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
+    bool class_timing = true;                       // per-class launch timing for the iteration budget
+    for (int i = 0; i < 2 * sp_ctx::N_CLASS_EV && class_timing; ++i)
+        if (hipEventCreate(&ctx->class_ev[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            class_timing = false;
+        }
     std::vector<std::unique_ptr<Slot>> active;
     for (;;) {
         std::vector<std::shared_ptr<Job>> admitted;
@@ -528,14 +582,14 @@ static void worker_main(sp_ctx* ctx) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 t = get_tuning(ctx, j->hr);
             }
-            sl->chunk = chunk_for(j->hr, t);
+            sl->tuned_chunk = sl->chunk = chunk_for(j->hr, t);
             if (rc == SP_OK && wants_shadow(j->hr)) {
                 auto sh = std::make_unique<Slot>();
                 make_shadow(j->hr, sh->shadow_req);
                 sh->hr = &sh->shadow_req;
                 sh->is_shadow = true;
                 sh->parent = sl.get();
-                sh->chunk = sl->chunk;
+                sh->tuned_chunk = sh->chunk = sl->chunk;
                 rc = alloc_dev(sh->shadow_req, sh->dev, s);
                 sl->shadow = sh.get();
                 if (rc == SP_OK) active.push_back(std::move(sh));
@@ -553,6 +607,12 @@ static void worker_main(sp_ctx* ctx) {
         // one iteration over all active slots
         std::vector<Slot*> slots;
         for (auto& a : active) slots.push_back(a.get());
+        double budget_ms = 0.0;
+        {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            budget_ms = ctx->iteration_budget_ms;
+        }
+        assign_chunks(ctx, class_timing ? budget_ms : 0.0, slots);
         IterPlan P;
         plan_iteration(slots, P);
         if (!P.work.empty()) {
@@ -581,9 +641,10 @@ static void worker_main(sp_ctx* ctx) {
                 hipEventRecord(e0, s);
                 char* base = (char*)ctx->table;
                 rc = record_iteration(ctx, s, P, (const DevRequest*)(base + L.req_off), (const SlotWork*)(base + L.work_off),
-                                      (const ReduceWork*)(base + L.red_off), ctx->counters, tun);
+                                      (const ReduceWork*)(base + L.red_off), ctx->counters, tun, class_timing);
                 hipEventRecord(e1, s);
                 if (hipStreamSynchronize(s) != hipSuccess && rc == SP_OK) rc = SP_ERR_DEVICE;
+                if (rc == SP_OK && class_timing) update_class_costs(ctx, P);
             }
             float ms = 0.f;
             if (rc == SP_OK) hipEventElapsedTime(&ms, e0, e1);
@@ -650,6 +711,8 @@ static void worker_main(sp_ctx* ctx) {
     active.clear();
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    for (int i = 0; i < 2 * sp_ctx::N_CLASS_EV; ++i)
+        if (ctx->class_ev[i]) { hipEventDestroy(ctx->class_ev[i]); ctx->class_ev[i] = nullptr; }
 }
 
 static void ensure_worker(sp_ctx* ctx) {

@@ -432,6 +432,10 @@ class Context:
     def set_reduction_limit(self, nbytes):
         check(lib().sp_ctx_set_reduction_limit(self._ctx, int(nbytes)))
 
+    def set_iteration_budget(self, ms):
+        """Server: cap each kernel class's launch per scheduler iteration at about `ms` (0: the tuned chunk)."""
+        check(lib().sp_ctx_set_iteration_budget(self._ctx, float(ms)))
+
     def inject_failures(self, n_iterations):
         check(lib().sp_ctx_inject_failures(self._ctx, int(n_iterations)))
 

@@ -4,6 +4,7 @@ Results of a request must not depend on what it runs with: Philox noise is keyed
 by (seed, particle, step) and partial sums are reduced per 64-particle tile in a
 fixed order, so alone / batched / mixed-with-others results are bit-identical.
 """
+import math
 import time
 
 import numpy as np
@@ -115,6 +116,47 @@ def test_server_alone_vs_mix(ctx):
         same(p, o)
     st = ctx.server_stats()
     assert st["completed"] >= 8 and st["iterations"] >= 1
+
+
+@pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chunk_boundaries_do_not_change_results(ctx, solver, precision):
+    """A launch restarts a tile from its stored state: NeelSpherical with the chart rotation at pi/4 (a third of
+    the steps rotated) run in launches of 100 steps and of 4,000 steps gives the same bits."""
+    e = sph_ensemble(n=256, n_steps=4000, solver=solver, precision=precision, seed=81)
+    e.min_angle = math.pi / 4
+    e.skip_check = True
+    out = []
+    for chunk in (100, 4000):
+        ctx.set_tuning(e.problem, e.solver, e.precision, block_size=256, chunk_steps=chunk, use_graph=1,
+                       persistent_waves_per_simd=2)
+        out.append(ctx.run(e))
+    ctx.set_tuning(e.problem, e.solver, e.precision, block_size=256, chunk_steps=4000, use_graph=1,
+                   persistent_waves_per_simd=2)
+    same(out[0], out[1])
+
+
+def test_server_iteration_budget_keeps_results(ctx):
+    """With an iteration budget the server cuts each class's chunk by its measured cost (down to one aligned block
+    of steps for a budget this small), and the chunk changes from one iteration to the next: served results still
+    equal the plain runs bit for bit.  The requests are submitted twice, so that the second round runs with
+    measured costs.  Budget 0 keeps the tuned chunk."""
+    bag = mixed_bag()
+    plain = [ctx.run(e) for e in bag]
+    try:
+        for budget in (1e-6, 0.0):
+            ctx.set_iteration_budget(budget)
+            for rnd in range(2):
+                ids = [ctx.submit(e) for e in bag]
+                for i, (rid, e, p) in enumerate(zip(ids, bag, plain)):
+                    assert ctx.wait(rid, 120_000) == SP_OK
+                    g = ctx.result(rid, e)
+                    assert np.array_equal(g["mean"], p["mean"]), (budget, rnd, i, e.problem, e.solver,
+                                                                  float(np.abs(g["mean"] - p["mean"]).max()))
+    finally:
+        ctx.set_iteration_budget(10.0)
+    with pytest.raises(SpError):
+        ctx.set_iteration_budget(-1.0)
 
 
 def test_server_admission_by_memory(ctx):
