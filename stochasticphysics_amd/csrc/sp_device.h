@@ -197,6 +197,20 @@ __device__ __forceinline__ V3<R> aniso_field(const AnisoLane<R>& A, V3<R> m) {
     if (ANISO == 2) return hc;
     return (A.ku * dot(m, A.eu)) * A.eu + hc;
 }
+// the derivative of aniso_field at m along v, H'(m) v: d_i = v.axis_i enters as
+// d/dm [c_i (c_j^2 + c_k^2)] . v = d_i (c_j^2 + c_k^2) + 2 c_i (c_j d_j + c_k d_k)
+template <int ANISO>
+__device__ __forceinline__ V3<double> aniso_dir(const AnisoLane<double>& A, V3<double> m, V3<double> v) {
+    if (ANISO == 1) return (A.k1 * dot(v, A.ez)) * A.ez;
+    const double c1 = dot(m, A.ex), c2 = dot(m, A.ey), c3 = dot(m, A.ez);
+    const double d1 = dot(v, A.ex), d2 = dot(v, A.ey), d3 = dot(v, A.ez);
+    const double g1 = d1 * (c2 * c2 + c3 * c3) + 2.0 * c1 * (c2 * d2 + c3 * d3);
+    const double g2 = d2 * (c1 * c1 + c3 * c3) + 2.0 * c2 * (c1 * d1 + c3 * d3);
+    const double g3 = d3 * (c1 * c1 + c2 * c2) + 2.0 * c3 * (c1 * d1 + c2 * d2);
+    const V3<double> hc = A.k1 * (g1 * A.ex + g2 * A.ey + g3 * A.ez);
+    if (ANISO == 2) return hc;
+    return (A.ku * dot(v, A.eu)) * A.eu + hc;
+}
 
 enum { S_EM = 1, S_IM = 3, S_MIL = 6, S_HEUN = 7, S_HNC = 8 };
 // normals per step: dim_W, twice that for Heun_NotConsistent (dW and dW2, Heun_NotConsistent.inl:37-38)
@@ -849,6 +863,7 @@ template <int ANISO> struct IMBN {
 };
 template <int ANISO> struct IMSph {
     static constexpr int DY = 2, DW = 3, DOUT = 3, G = 4, NORM = 0;
+    static constexpr int WAVES_PER_SIMD = 2;     // the chain-rule Jacobian fits 256 registers
     double a, b, sg, dt, tmin, tmax; AnisoLane<double> A; double w;
     __device__ void load(const DevRequest& rq, uint64_t p) {
         a = ld(rq, P_A, p); b = ld(rq, P_B, p); sg = ld(rq, P_SIG, p) * rq.sqdt; dt = rq.dt;
@@ -860,46 +875,91 @@ template <int ANISO> struct IMSph {
         double st, ct, sp, cp; sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
         o[0] = st * cp; o[1] = st * sp; o[2] = ct;
     }
+    // The residual and its exact Jacobian by the chain rule written out: in either chart, with M(theta, phi) the
+    // moment, e_t = dM/dtheta, e_p = (dM/dphi) / sin(theta) and e_r = -de_p/dphi,
+    //   de_t/dtheta = -M, de_t/dphi = cos(theta) e_p, de_p/dtheta = 0,
+    // so P0 = b e_t - a e_p and P1 = (a e_t + b e_p) / sin(theta) have
+    //   dP0/dtheta = -b M, dP0/dphi = b cos(theta) e_p + a e_r,
+    //   dP1/dtheta = -cos(theta)/sin(theta) P1 - a M / sin(theta), dP1/dphi = (a cos(theta) e_p - b e_r) / sin(theta)
+    // (zero where 1/sin(theta) is replaced by 0), and q_r = u_r + dt P_r.H(M) + sigma P_r.dW has
+    //   dq_r/du_k = delta_rk + dP_r/du_k . (dt H + sigma dW) + dt P_r . (H'(M) dM/du_k);
+    // the chart maps (acos, atan2) contribute their 2x2 Jacobians before and after.
     __device__ void residual(const double* yv, const double* yi, V3<double> h, const double* z, double F[2], double J[2][2]) const {
-        Dual<2> y0(yv[0]), y1(yv[1]);
-        y0.d[0] = 1.0; y1.d[1] = 1.0;
-        Dual<2> r0 = 0.5 * (y0 + Dual<2>(yi[0])), r1 = 0.5 * (y1 + Dual<2>(yi[1]));
-        const bool rot = (r0.v < tmin) || (r0.v > tmax);
-        Dual<2> st, ct, sp, cp;
-        if (rot) {
-            dsincos(r0, st, ct); dsincos(r1, sp, cp);
-            const Dual<2> n0 = dacos(-(cp * st)), n1 = datan2(sp * st, ct);
-            r0 = n0; r1 = n1;
+        const double r0 = 0.5 * (yv[0] + yi[0]), r1 = 0.5 * (yv[1] + yi[1]);
+        const bool rot = (r0 < tmin) || (r0 > tmax);
+        double u0 = r0, u1 = r1;
+        double JA[2][2] = {{1.0, 0.0}, {0.0, 1.0}};          // d(chart coordinates) / d(midpoint)
+        if (rot) {                                           // prepareCalculations (:568-580)
+            double st, ct, sp, cp;
+            sincos_fast(r0, &st, &ct); sincos_fast(r1, &sp, &cp);
+            const double x = -(cp * st), Y = sp * st;
+            u0 = acos(x); u1 = atan2(Y, ct);
+            const double k = -1.0 / sqrt(1.0 - x * x), den = 1.0 / (ct * ct + Y * Y);
+            JA[0][0] = k * -(cp * ct); JA[0][1] = k * (sp * st);
+            JA[1][0] = (ct * (sp * ct) + Y * st) * den; JA[1][1] = (ct * (cp * st)) * den;
         }
-        dsincos(r0, st, ct); dsincos(r1, sp, cp);
-        Dual<2> ods = Dual<2>(1.0) / st;
-        if (isinf(ods.v)) ods = Dual<2>(0.0);
-        DV3<2> M, et, ep;
-        const Dual<2> zero(0.0);
-        if (!rot) { M = {st * cp, st * sp, ct}; et = {ct * cp, ct * sp, -st}; ep = {-sp, cp, zero}; }
-        else { M = {-ct, st * sp, st * cp}; et = {st, ct * sp, ct * cp}; ep = {zero, cp, -sp}; }
-        const DV3<2> P0 = b * et - a * ep;
-        DV3<2> P1 = ods * (a * et + b * ep);
-        if (ods.v == 0.0) { if (!rot) P1.z = Dual<2>(a); else P1.x = Dual<2>(-a); }
-        const DV3<2> H = daniso<2, ANISO>(A, M) + dconst<2>(h);
-        const DV3<2> zz = dconst<2>({z[0], z[1], z[2]});
-        Dual<2> q0 = r0 + dt * ddot(P0, H) + sg * ddot(P0, zz);
-        Dual<2> q1 = r1 + dt * ddot(P1, H) + sg * ddot(P1, zz);
-        if (rot) {
-            Dual<2> s0, c0, s1, c1;
-            dsincos(q0, s0, c0); dsincos(q1, s1, c1);
-            const Dual<2> n0 = dacos(c1 * s0), n1 = datan2(s1 * s0, -c0);
-            q0 = n0; q1 = n1;
-        } else { q0 = dfmod_2pi(q0); q1 = dfmod_2pi(q1); }
-        const Dual<2> f0 = q0 + Dual<2>(0.5 * yi[0]) - 1.5 * y0;
-        const Dual<2> f1 = q1 + Dual<2>(0.5 * yi[1]) - 1.5 * y1;
-        F[0] = f0.v; F[1] = f1.v;
-        J[0][0] = f0.d[0]; J[0][1] = f0.d[1]; J[1][0] = f1.d[0]; J[1][1] = f1.d[1];
+        double st, ct, sp, cp;
+        sincos_fast(u0, &st, &ct); sincos_fast(u1, &sp, &cp);
+        double ods = 1.0 / st;
+        const bool odz = isinf(ods);
+        if (odz) ods = 0.0;
+        V3<double> M, et, ep, er;
+        if (!rot) { M = {st * cp, st * sp, ct}; et = {ct * cp, ct * sp, -st}; ep = {-sp, cp, 0.0}; er = {cp, sp, 0.0}; }
+        else { M = {-ct, st * sp, st * cp}; et = {st, ct * sp, ct * cp}; ep = {0.0, cp, -sp}; er = {0.0, sp, cp}; }
+        const V3<double> P0 = b * et - a * ep;
+        V3<double> P1 = ods * (a * et + b * ep);
+        if (odz) { if (!rot) P1.z = a; else P1.x = -a; }
+        const V3<double> H = aniso_field<ANISO, double>(A, M) + h;
+        const V3<double> zz = {z[0], z[1], z[2]};
+        double q0 = u0 + dt * dot(P0, H) + sg * dot(P0, zz);
+        double q1 = u1 + dt * dot(P1, H) + sg * dot(P1, zz);
+        // d q / d u
+        const V3<double> W = dt * H + sg * zz;
+        const V3<double> Gt = aniso_dir<ANISO>(A, M, et), Gp = st * aniso_dir<ANISO>(A, M, ep);
+        const V3<double> dP0t = -b * M, dP0p = (b * ct) * ep + a * er;
+        V3<double> dP1t = {0.0, 0.0, 0.0}, dP1p = {0.0, 0.0, 0.0};
+        if (!odz) {
+            dP1t = (-ct * ods) * P1 - (a * ods) * M;
+            dP1p = ods * ((a * ct) * ep - b * er);
+        }
+        double JQ[2][2] = {{1.0 + dot(dP0t, W) + dt * dot(P0, Gt), dot(dP0p, W) + dt * dot(P0, Gp)},
+                           {dot(dP1t, W) + dt * dot(P1, Gt), 1.0 + dot(dP1p, W) + dt * dot(P1, Gp)}};
+        if (rot) {                                           // finishCalculations (:440-471)
+            double s0, c0, s1, c1;
+            sincos_fast(q0, &s0, &c0); sincos_fast(q1, &s1, &c1);
+            const double x = c1 * s0, Y = s1 * s0;
+            q0 = acos(x); q1 = atan2(Y, -c0);
+            const double k = -1.0 / sqrt(1.0 - x * x), den = 1.0 / (c0 * c0 + Y * Y);
+            const double JB[2][2] = {{k * (c1 * c0), k * -(s1 * s0)}, {(-c0 * (s1 * c0) - Y * s0) * den, (-c0 * (c1 * s0)) * den}};
+            double T[2][2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) T[i][j] = JB[i][0] * JQ[0][j] + JB[i][1] * JQ[1][j];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) JQ[i][j] = T[i][0] * JA[0][j] + T[i][1] * JA[1][j];
+        } else {
+            q0 = fmod_2pi(q0); q1 = fmod_2pi(q1);
+        }
+        F[0] = (q0 + 0.5 * yi[0]) - 1.5 * yv[0];
+        F[1] = (q1 + 0.5 * yi[1]) - 1.5 * yv[1];
+        J[0][0] = 0.5 * JQ[0][0] - 1.5; J[0][1] = 0.5 * JQ[0][1];
+        J[1][0] = 0.5 * JQ[1][0]; J[1][1] = 0.5 * JQ[1][1] - 1.5;
     }
 };
 
+template <class Prob, class = void> struct im_waves { static constexpr int value = 1; };
+template <class Prob> struct im_waves<Prob, std::void_t<decltype(Prob::WAVES_PER_SIMD)>> {
+    static constexpr int value = Prob::WAVES_PER_SIMD;
+};
 template <class Prob> struct IMCfg {
     static constexpr int DY = Prob::DY, DW = Prob::DW, DOUT = Prob::DOUT, G = Prob::G, NORM = Prob::NORM, NZ = DW;
+    static constexpr int WAVES_PER_SIMD = im_waves<Prob>::value;
+    // each Philox block is drawn just before the step that first uses it when the class runs at two waves per SIMD
+    // (12 fewer normals live through the Newton iterations)
+    static constexpr bool PIPE_NOISE = WAVES_PER_SIMD == 1;
     using Lane = Prob;
     struct Cache { unsigned fails; };
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { L.load(rq, p); }

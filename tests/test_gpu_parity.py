@@ -284,19 +284,25 @@ def test_field_types_parity(ctx, name):
     compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10)
 
 
-@pytest.mark.parametrize("problem", ["Neel", "NeelSpherical", "BrownAndNeel"])
+@pytest.mark.parametrize("problem", ["Neel", "NeelSpherical", "NeelSpherical_uniaxial", "BrownAndNeel"])
 def test_implicit_midpoint_parity(ctx, problem):
+    """Implicit_Midpoint (Implicit_Midpoint.inl:38-188).  The NeelSpherical kernels form the Newton Jacobian by the
+    chain rule written out, the oracle by dual numbers: the same exact Jacobian, so the iterations stop on the same
+    particle-steps.  An FMA-contracted oracle build moves the NeelSpherical means by < 1e-14 (final states < 7e-13)
+    and stops on the same particle-steps (uniaxialcubic: test_uniaxialcubic_parity)."""
     if problem == "Neel":
         e = neel_ensemble(solver="Implicit_Midpoint", n=64, n_steps=500, field="sinusoidal")
     elif problem == "NeelSpherical":
         e = sph_ensemble(solver="Implicit_Midpoint", n=64, n_steps=500)
+    elif problem == "NeelSpherical_uniaxial":
+        e = sph_ensemble(solver="Implicit_Midpoint", n=64, n_steps=500, aniso="uniaxial")
     else:
         e = bn_ensemble(solver="Implicit_Midpoint", n=64, n_steps=300)
     g, o = ctx.run(e, keep_final=True), run_oracle(e)
     # near a chart switch the reference's residual is discontinuous and Newton can stall above 1e-10:
     # the GPU must stall on exactly the same particle-steps as the oracle
     assert g["newton_failures"] == o["newton_failures"]
-    compare(g, o, 1e-10, spherical=problem == "NeelSpherical")
+    compare(g, o, 1e-10, spherical=problem.startswith("NeelSpherical"))
 
 
 
