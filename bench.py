@@ -18,8 +18,9 @@ vector-ALU roofline of its kernel, a CPU baseline on host cores, and
     (1,048,576 particles x 5e5 steps), Heun_Strong and Millstein, FP64 vs mixed;
   * "serving": Poisson arrivals of spherical-cubic requests (Implicit_Midpoint
     for damping >= 0.3) at 200 req/s for 60 s through the continuous-batching
-    server, then at 50% and 80% of the measured capacity: p50/p99 end-to-end
-    latency and throughput;
+    server; the server's capacity from a run whose arrivals outpace it; then
+    50%, 80% and 95% of that capacity: p50/p99 end-to-end latency and
+    throughput;
   * "neel": the 4,096-particle Neel relaxation, FP64 and mixed, beside the CPU
     oracle on the whole ensemble;
   * "provider": the particles of a parameter file drawn on the device vs
@@ -284,22 +285,29 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
                               "mixed (Heun_Strong) or FP64 Implicit_Midpoint for damping >= 0.3")
 
 
-def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55):
-    """The serving configuration (Poisson arrivals at `rate` for `seconds`), then the same request distribution at
-    50% and 80% of the capacity that run measured (served particle-steps/s over the device's busy fraction), for
-    `low_seconds` each: p50/p99 latency against offered load."""
+def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55, sat_seconds=8.0):
+    """The serving configuration (Poisson arrivals at `rate` for `seconds`); the server's capacity, measured as the
+    served particle-steps/s of a run whose arrivals (3x `rate`, `sat_seconds`) outpace the device; then the same
+    request distribution at 50%, 80% and 95% of that capacity for `low_seconds` each: p50/p99 latency against
+    offered load."""
     main = serving_leg(ctx, configs, seconds, rate, seed)
-    capacity = main["throughput"] / max(main["device_busy_frac"], 1e-9)
+    sat = serving_leg(ctx, configs, sat_seconds, 3.0 * rate, seed + 10)
+    capacity = sat["throughput"]
     per_request = main["offered_particle_steps_per_s"] / rate
     main["offered_fraction_of_capacity"] = main["offered_particle_steps_per_s"] / capacity
     loads = []
-    for k, frac in enumerate((0.5, 0.8)):
+    for k, frac in enumerate((0.5, 0.8, 0.95)):
         r = serving_leg(ctx, configs, low_seconds, frac * capacity / per_request, seed + 1 + k)
         loads.append({key: r[key] for key in ("requests", "offered_req_per_s", "p50_ms", "p99_ms", "mean_ms", "throughput",
                                               "offered_particle_steps_per_s", "device_busy_frac", "fp64_fallbacks",
                                               "latency_by_solver")})
         loads[-1]["offered_fraction_of_capacity"] = r["offered_particle_steps_per_s"] / capacity
-    main.update(capacity_particle_steps_per_s=capacity, lower_loads=loads)
+    main.update(capacity_particle_steps_per_s=capacity,
+                capacity_run=dict(offered_req_per_s=3.0 * rate, seconds=sat_seconds, requests=sat["requests"],
+                                  p50_ms=sat["p50_ms"], p99_ms=sat["p99_ms"]),
+                other_loads=loads,
+                note="device_busy_frac sums the device time of the server's two lanes (explicit solvers, "
+                     "Implicit_Midpoint), whose iterations overlap")
     return main
 
 
@@ -541,7 +549,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-low-seconds", type=float, default=8.0,
-                    help="duration of each of the two lower-load serving runs (50%% and 80%% of capacity)")
+                    help="duration of each of the serving runs at 50%%, 80%% and 95%% of the measured capacity")
     ap.add_argument("--batch-sets", type=int, default=512, help="uneven sets in the mixed/batched leg")
     ap.add_argument("--no-tune", action="store_true", help="batched legs with the default tuning (no autotuner)")
     ap.add_argument("--reference-scheme", action="store_true",

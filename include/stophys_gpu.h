@@ -238,7 +238,8 @@ int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
 int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes);
 /* server (sp_submit): each kernel class's steps per scheduler iteration are cut, from its tuned chunk_steps, so
  * that its launch takes about `ms` by the class's measured cost per particle-step: requests of cheap classes then
- * finish without waiting on long launches of expensive ones.  0 keeps the tuned chunk; default 10. */
+ * finish without waiting on long launches of expensive ones (Implicit_Midpoint requests iterate on a lane of their
+ * own in any case).  0 keeps the tuned chunk; default 5. */
 int sp_ctx_set_iteration_budget(sp_ctx* ctx, double ms);
 
 /* statistics of the last sp_run on this context: [0] kernel launches, [1] kernel ms,

@@ -377,6 +377,74 @@ inline void copy_result(const HostRequest& hr, const Outputs& o, sp_result* r) {
 }
 
 // ------------------------------------------------------------------ context
+// Device scratch and streams of one sequence of iterations: the persistent launches of an iteration's kernel classes
+// run side by side (class k > 0 on side[(k - 1) % n_side], forked from and joined back into `stream` with events, all
+// streams at the default priority; with n_side = 0 they follow each other on `stream`); tables, partial sums and work
+// counters are reused from iteration to iteration.  The synchronous paths and the server's first lane use the
+// context's set (two side streams); the server's Implicit_Midpoint lane has its own stream and scratch, so that its
+// iterations overlap the first lane's.  A context creates four streams in all: the runtime maps streams onto four
+// hardware queues per process by default, and two streams sharing a queue would run their kernels one after the other.
+struct IterRes {
+    static constexpr int N_SIDE = 2;
+    static constexpr int N_CLASS_EV = 32;   // classes timed per iteration (further classes are not timed)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int n_side = 0;                         // side streams created (0..N_SIDE)
+    hipStream_t side[N_SIDE] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[N_SIDE] = {nullptr, nullptr};
+    double* partial = nullptr;
+    size_t partial_bytes = 0;
+    void* table = nullptr;                  // device copy of request / work / reduce tables
+    size_t table_bytes = 0;
+    void* pinned = nullptr;                 // pinned host staging of the tables
+    size_t pinned_bytes = 0;
+    uint32_t* counters = nullptr;           // per-launch work counters
+    size_t n_counters = 0;
+    hipEvent_t class_ev[2 * N_CLASS_EV] = {};   // per-class timing (the server's iterations)
+    bool class_timing = false;
+};
+// streams and events of r: `stream` is used as r's own when given, else one is created; n_side side streams
+inline int init_res(IterRes& r, hipStream_t stream, int n_side) {
+    if (stream) r.stream = stream;
+    else { SP_HIP(hipStreamCreateWithFlags(&r.stream, hipStreamNonBlocking)); r.own_stream = true; }
+    for (int i = 0; i < std::min(n_side, IterRes::N_SIDE); ++i) {
+        SP_HIP(hipStreamCreateWithFlags(&r.side[i], hipStreamNonBlocking));
+        SP_HIP(hipEventCreateWithFlags(&r.join_ev[i], hipEventDisableTiming));
+        r.n_side = i + 1;
+    }
+    SP_HIP(hipEventCreateWithFlags(&r.fork_ev, hipEventDisableTiming));
+    return SP_OK;
+}
+// the per-class timing events (created on first use by the server; without them the iteration budget is off)
+inline void init_class_timing(IterRes& r) {
+    if (r.class_timing) return;
+    r.class_timing = true;
+    for (int i = 0; i < 2 * IterRes::N_CLASS_EV && r.class_timing; ++i)
+        if (!r.class_ev[i] && hipEventCreate(&r.class_ev[i]) != hipSuccess) {
+            (void)hipGetLastError();
+            r.class_timing = false;
+        }
+}
+inline void destroy_res(IterRes& r) {
+    if (r.stream) hipStreamSynchronize(r.stream);
+    for (int i = 0; i < IterRes::N_SIDE; ++i)
+        if (r.side[i]) hipStreamSynchronize(r.side[i]);
+    if (r.partial) hipFree(r.partial);
+    if (r.table) hipFree(r.table);
+    if (r.pinned) hipHostFree(r.pinned);
+    if (r.counters) hipFree(r.counters);
+    for (int i = 0; i < IterRes::N_SIDE; ++i) {
+        if (r.side[i]) hipStreamDestroy(r.side[i]);
+        if (r.join_ev[i]) hipEventDestroy(r.join_ev[i]);
+    }
+    if (r.fork_ev) hipEventDestroy(r.fork_ev);
+    for (int i = 0; i < 2 * IterRes::N_CLASS_EV; ++i)
+        if (r.class_ev[i]) hipEventDestroy(r.class_ev[i]);
+    if (r.own_stream && r.stream) hipStreamDestroy(r.stream);
+    r = IterRes{};
+}
+
 struct Job {
     uint64_t id = 0;
     HostRequest hr;
@@ -412,12 +480,6 @@ struct sp_ctx_impl;
 struct sp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // kernel classes of one scheduler iteration run concurrently: class k > 0 goes to side[(k-1) % 3],
-    // forked from and joined back into `stream` with events (all streams at the default priority)
-    static constexpr int N_SIDE = 3;
-    hipStream_t side[N_SIDE] = {nullptr, nullptr, nullptr};
-    hipEvent_t fork_ev = nullptr;
-    hipEvent_t join_ev[N_SIDE] = {nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     std::mutex mu;                          // protects everything below
     std::map<uint64_t, sp_tuning> tuning;   // by (problem, solver, precision)
@@ -436,16 +498,10 @@ struct sp_ctx {
     // statistics
     double stat_active = 0, stat_completed = 0, stat_iterations = 0, stat_busy_ms = 0, stat_particle_steps = 0, stat_fallbacks = 0;
     sp::Clock::time_point t_start = sp::Clock::now();
-    // scratch owned by the worker (or by a synchronous call holding run_mu)
+    // scratch owned by the worker (or by a synchronous call holding run_mu); res.stream is `stream`
     std::mutex run_mu;                      // serialises device use between sp_run and the worker
-    double* partial = nullptr;
-    size_t partial_bytes = 0;
-    void* table = nullptr;                  // device copy of request / work / reduce tables
-    size_t table_bytes = 0;
-    void* pinned = nullptr;                 // pinned host staging of the tables
-    size_t pinned_bytes = 0;
-    uint32_t* counters = nullptr;           // per-launch work counters
-    size_t n_counters = 0;
+    std::atomic<int> run_waiters{0};        // synchronous calls waiting for run_mu (sp::RunLock)
+    sp::IterRes res;
     // plain path: largest partial-sum buffer for reducing a whole pass at once (larger passes are reduced
     // chunk by chunk)
     uint64_t whole_pass_limit = 4ull << 30;
@@ -453,15 +509,27 @@ struct sp_ctx {
     uint32_t inject_failures = 0;
     // server: each kernel class's chunk is sized so that its launch takes about this long (ms), at most its tuned
     // chunk; 0 keeps the tuned chunk.  The worker thread owns the measured costs and the timing events.
-    double iteration_budget_ms = 10.0;
+    double iteration_budget_ms = 5.0;
     std::map<uint32_t, double> class_ns_per_ps;   // kernel class -> smoothed device ns per particle-step
-    static constexpr int N_CLASS_EV = 32;         // classes timed per iteration (further classes are not timed)
-    hipEvent_t class_ev[2 * N_CLASS_EV] = {};
     // last synchronous run statistics
     double last_stats[8] = {0};
 };
 
 namespace sp {
+
+// run_mu held by a synchronous call.  The server's worker keeps run_mu while iterations are in flight; the count of
+// callers waiting for it makes the worker stop issuing iterations and let them in.
+struct RunLock {
+    sp_ctx* c;
+    explicit RunLock(sp_ctx* ctx) : c(ctx) {
+        c->run_waiters.fetch_add(1);
+        c->run_mu.lock();
+        c->run_waiters.fetch_sub(1);
+    }
+    ~RunLock() { c->run_mu.unlock(); }
+    RunLock(const RunLock&) = delete;
+    RunLock& operator=(const RunLock&) = delete;
+};
 
 inline uint64_t tuning_key(int problem, int solver, int precision) { return ((uint64_t)problem << 16) | ((uint64_t)solver << 8) | (uint64_t)precision; }
 
@@ -480,46 +548,49 @@ inline sp_tuning get_tuning(sp_ctx* ctx, const HostRequest& hr) {
     return it != ctx->tuning.end() ? it->second : default_tuning(ctx, hr);
 }
 
-inline int ensure_partial(sp_ctx* ctx, size_t bytes) {
-    if (bytes <= ctx->partial_bytes) return SP_OK;
-    if (ctx->partial) hipFree(ctx->partial);
-    ctx->partial = nullptr;
-    ctx->partial_bytes = 0;
-    hipError_t e = hipMalloc(&ctx->partial, bytes);
+inline int ensure_partial(IterRes& r, size_t bytes) {
+    if (bytes <= r.partial_bytes) return SP_OK;
+    if (r.partial) hipFree(r.partial);
+    r.partial = nullptr;
+    r.partial_bytes = 0;
+    hipError_t e = hipMalloc(&r.partial, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();      // a failed allocation must not be reported by a later launch check
-        ctx->partial = nullptr;
+        r.partial = nullptr;
         return set_error(SP_ERR_OUT_OF_MEMORY, std::string("partial buffer: ") + hipGetErrorString(e));
     }
-    ctx->partial_bytes = bytes;
+    r.partial_bytes = bytes;
     return SP_OK;
 }
-inline int ensure_table(sp_ctx* ctx, size_t bytes) {
-    if (bytes > ctx->table_bytes) {
-        if (ctx->table) hipFree(ctx->table);
-        ctx->table = nullptr;
-        ctx->table_bytes = 0;
-        SP_HIP(hipMalloc(&ctx->table, bytes));
-        ctx->table_bytes = bytes;
+inline int ensure_table(IterRes& r, size_t bytes) {
+    if (bytes > r.table_bytes) {
+        if (r.table) hipFree(r.table);
+        r.table = nullptr;
+        r.table_bytes = 0;
+        SP_HIP(hipMalloc(&r.table, bytes));
+        r.table_bytes = bytes;
     }
-    if (bytes > ctx->pinned_bytes) {
-        if (ctx->pinned) hipHostFree(ctx->pinned);
-        ctx->pinned = nullptr;
-        ctx->pinned_bytes = 0;
-        SP_HIP(hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault));
-        ctx->pinned_bytes = bytes;
+    if (bytes > r.pinned_bytes) {
+        if (r.pinned) hipHostFree(r.pinned);
+        r.pinned = nullptr;
+        r.pinned_bytes = 0;
+        SP_HIP(hipHostMalloc(&r.pinned, bytes, hipHostMallocDefault));
+        r.pinned_bytes = bytes;
     }
     return SP_OK;
 }
-inline int ensure_counters(sp_ctx* ctx, size_t n) {
-    if (n <= ctx->n_counters) return SP_OK;
-    if (ctx->counters) hipFree(ctx->counters);
-    ctx->counters = nullptr;
-    ctx->n_counters = 0;
-    SP_HIP(hipMalloc(&ctx->counters, n * sizeof(uint32_t)));
-    ctx->n_counters = n;
+inline int ensure_counters(IterRes& r, size_t n) {
+    if (n <= r.n_counters) return SP_OK;
+    if (r.counters) hipFree(r.counters);
+    r.counters = nullptr;
+    r.n_counters = 0;
+    SP_HIP(hipMalloc(&r.counters, n * sizeof(uint32_t)));
+    r.n_counters = n;
     return SP_OK;
 }
+inline int ensure_partial(sp_ctx* ctx, size_t bytes) { return ensure_partial(ctx->res, bytes); }
+inline int ensure_table(sp_ctx* ctx, size_t bytes) { return ensure_table(ctx->res, bytes); }
+inline int ensure_counters(sp_ctx* ctx, size_t n) { return ensure_counters(ctx->res, n); }
 
 // defined in sp_runtime.cpp
 struct Prepared {

@@ -42,7 +42,7 @@ int prepare_plain(sp_ctx* ctx, Prepared& pr) {
 }
 
 int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     HostRequest& hr = pr.hr;
     ReqDev& d = pr.d;
     hipStream_t s = ctx->stream;
@@ -68,7 +68,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     // (one kernel boundary per chunk instead of two); otherwise, or when that buffer cannot be allocated,
     // each chunk is reduced before the next one reuses a buffer of one chunk's rows
     bool whole = hr.n_out * row_bytes <= ctx->whole_pass_limit && hr.n_out * hr.dout < (1ull << 32);
-    if (whole && hr.n_out * row_bytes > ctx->partial_bytes) {
+    if (whole && hr.n_out * row_bytes > ctx->res.partial_bytes) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || hr.n_out * row_bytes > free_b / 4) whole = false;
     }
@@ -78,8 +78,8 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     }
     const uint64_t n_chunks = std::max<uint64_t>(1, (hr.total_steps + C - 1) / C);
     if (int e = ensure_table(ctx, n_chunks * sizeof(ReduceWork) + 256)) return e;
-    ReduceWork* rw_host = (ReduceWork*)ctx->pinned;
-    ReduceWork* rw_dev = (ReduceWork*)ctx->table;
+    ReduceWork* rw_host = (ReduceWork*)ctx->res.pinned;
+    ReduceWork* rw_dev = (ReduceWork*)ctx->res.table;
     const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, tun.block_size ? tun.block_size / 64 * 64 : MAX_BLOCK));
     const uint32_t waves_per_block = block / 64;
     const dim3 grid((hr.n_tiles + waves_per_block - 1) / waves_per_block);
@@ -96,7 +96,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     const std::vector<hipEvent_t>& ev = pr.events;
     // the reduction work goes to the device in one copy, ahead of the launches
     std::vector<uint32_t> chunk_rows(n_chunks);
-    std::vector<double*> chunk_partial(n_chunks, ctx->partial);
+    std::vector<double*> chunk_partial(n_chunks, ctx->res.partial);
     {
         uint64_t sb = 0;
         for (uint64_t k = 0; k < n_chunks; ++k) {
@@ -104,12 +104,12 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
             const uint64_t first_window = sb == 0 ? 0 : sb / OS + 1;
             const uint64_t rows = (se - sb) / OS + (sb == 0 ? 1 : 0);
             chunk_rows[k] = (uint32_t)(rows * hr.dout);
-            if (whole) chunk_partial[k] = ctx->partial + first_window * hr.dout * hr.n_tiles;
-            else rw_host[k] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, chunk_rows[k],
+            if (whole) chunk_partial[k] = ctx->res.partial + first_window * hr.dout * hr.n_tiles;
+            else rw_host[k] = ReduceWork{ctx->res.partial, hr.n_tiles, hr.n_check_tiles, chunk_rows[k],
                                          (uint32_t)(first_window * hr.dout), d.out, d.check};
             sb = se;
         }
-        if (whole) rw_host[0] = ReduceWork{ctx->partial, hr.n_tiles, hr.n_check_tiles, (uint32_t)(hr.n_out * hr.dout), 0u,
+        if (whole) rw_host[0] = ReduceWork{ctx->res.partial, hr.n_tiles, hr.n_check_tiles, (uint32_t)(hr.n_out * hr.dout), 0u,
                                            d.out, d.check};
     }
     SP_HIP(hipMemcpyAsync(rw_dev, rw_host, (whole ? 1 : n_chunks) * sizeof(ReduceWork), hipMemcpyHostToDevice, s));
@@ -266,11 +266,11 @@ int sp_ctx_create(int device, sp_ctx** out) {
     SP_HIP(hipSetDevice(device));
     SP_HIP(hipGetDeviceProperties(&c->prop, device));
     SP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int i = 0; i < sp_ctx::N_SIDE; ++i) {
-        SP_HIP(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
-        SP_HIP(hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming));
+    if (int e = init_res(c->res, c->stream, IterRes::N_SIDE)) {
+        destroy_res(c->res);
+        hipStreamDestroy(c->stream);
+        return e;
     }
-    SP_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
     size_t free_b = 0, total_b = 0;
     SP_HIP(hipMemGetInfo(&free_b, &total_b));
     c->mem_budget = (uint64_t)(0.5 * (double)free_b);
@@ -306,7 +306,7 @@ int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active) {
 
 int sp_ctx_set_reduction_limit(sp_ctx* ctx, uint64_t bytes) {
     if (!ctx) return set_error(SP_ERR_INVALID_ARG, "NULL context");
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     ctx->whole_pass_limit = bytes;
     return SP_OK;
 }
@@ -358,7 +358,7 @@ int sp_prepare(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp
     if (int e = prepare_request(req, parts, h->pr.hr)) return e;
     h->pr.keep_final = (req->flags & SP_FLAG_KEEP_FINAL_STATE) != 0;
     {
-        std::lock_guard<std::mutex> lk(ctx->run_mu);
+        RunLock lk(ctx);
         if (int e = prepare_plain(ctx, h->pr)) return e;
     }
     *out = h.release();
@@ -382,7 +382,7 @@ int sp_execute(sp_prepared* h, sp_result* res) {
 int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_state) {
     if (!h) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
     const HostRequest& hr = h->pr.hr;
-    std::lock_guard<std::mutex> lk(h->ctx->run_mu);
+    RunLock lk(h->ctx);
     SP_HIP(hipSetDevice(h->ctx->device));
     if (params) SP_HIP(hipMemcpy(params, h->pr.d.params, (size_t)NPARAM * hr.N * 8, hipMemcpyDeviceToHost));
     if (initial_state) {
@@ -397,7 +397,7 @@ int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_sta
 void sp_release(sp_prepared* h) {
     if (!h) return;
     {
-        std::lock_guard<std::mutex> lk(h->ctx->run_mu);
+        RunLock lk(h->ctx);
         hipSetDevice(h->ctx->device);
         free_prepared(h->pr);
     }
@@ -406,7 +406,7 @@ void sp_release(sp_prepared* h) {
 
 int sp_dev_philox(sp_ctx* ctx, const uint32_t ctr[4], const uint32_t key[2], uint64_t count, uint32_t* out) {
     if (!ctx || !ctr || !key || (!out && count)) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     SP_HIP(hipSetDevice(ctx->device));
     uint32_t* d = nullptr;
     SP_HIP(hipMalloc(&d, std::max<uint64_t>(1, count) * 16));
@@ -420,7 +420,7 @@ int sp_dev_philox(sp_ctx* ctx, const uint32_t ctr[4], const uint32_t key[2], uin
 
 int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first, uint64_t count, int precision, double* out) {
     if (!ctx || (!out && count)) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     SP_HIP(hipSetDevice(ctx->device));
     double* d = nullptr;
     SP_HIP(hipMalloc(&d, std::max<uint64_t>(1, count) * 8));
@@ -435,7 +435,7 @@ int sp_dev_normals(sp_ctx* ctx, uint64_t seed, uint64_t particle, uint64_t first
 int sp_dev_box_muller(sp_ctx* ctx, const uint32_t* words, uint64_t n_blocks, int precision, double* out) {
     if (!ctx || (n_blocks && (!words || !out))) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
     if (precision != SP_PREC_FP64 && precision != SP_PREC_MIXED) return set_error(SP_ERR_INVALID_ARG, "bad precision");
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     SP_HIP(hipSetDevice(ctx->device));
     const uint64_t nb = std::max<uint64_t>(1, n_blocks);
     uint32_t* dw = nullptr;
@@ -468,16 +468,7 @@ extern "C" void sp_ctx_destroy(sp_ctx* ctx) {
     if (!ctx) return;
     stop_worker(ctx);
     hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
-    if (ctx->partial) hipFree(ctx->partial);
-    if (ctx->table) hipFree(ctx->table);
-    if (ctx->pinned) hipHostFree(ctx->pinned);
-    if (ctx->counters) hipFree(ctx->counters);
-    for (int i = 0; i < sp_ctx::N_SIDE; ++i) {
-        if (ctx->side[i]) { hipStreamSynchronize(ctx->side[i]); hipStreamDestroy(ctx->side[i]); }
-        if (ctx->join_ev[i]) hipEventDestroy(ctx->join_ev[i]);
-    }
-    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
+    destroy_res(ctx->res);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

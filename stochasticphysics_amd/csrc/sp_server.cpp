@@ -117,35 +117,36 @@ static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning&
     return dim3(std::max<uint32_t>(1, (waves + wpb - 1) / wpb));
 }
 
-// Issue the launches of one iteration whose tables are already on the device.
-// timing: record a timing-event pair around each class's launch (ctx->class_ev; the server's iterations, not captured
+// Issue the launches of one iteration whose tables are already on the device, on R's streams (s is R.stream, or the
+// stream being captured).
+// timing: record a timing-event pair around each class's launch (R.class_ev; the server's iterations, not captured
 // in a graph)
-static int record_iteration(sp_ctx* ctx, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs, const SlotWork* d_work,
-                            const ReduceWork* d_reduce, uint32_t* d_counters, const std::vector<sp_tuning>& tun,
-                            bool timing = false) {
+static int record_iteration(sp_ctx* ctx, IterRes& R, hipStream_t s, const IterPlan& P, const DevRequest* d_reqs,
+                            const SlotWork* d_work, const ReduceWork* d_reduce, uint32_t* d_counters,
+                            const std::vector<sp_tuning>& tun, bool timing = false) {
     if (!P.kinds.empty()) launch_zero_u32(s, d_counters, (uint32_t)P.kinds.size());
     // fork: the classes of this iteration run side by side (the persistent launches of classes with few
     // waves, such as FP64 shadows, would otherwise serialise behind the others)
-    const int n_side = (int)std::min<size_t>(P.kinds.size() > 0 ? P.kinds.size() - 1 : 0, (size_t)sp_ctx::N_SIDE);
+    const int n_side = (int)std::min<size_t>(P.kinds.size() > 0 ? P.kinds.size() - 1 : 0, (size_t)R.n_side);
     if (n_side > 0) {
-        SP_HIP(hipEventRecord(ctx->fork_ev, s));
-        for (int i = 0; i < n_side; ++i) SP_HIP(hipStreamWaitEvent(ctx->side[i], ctx->fork_ev, 0));
+        SP_HIP(hipEventRecord(R.fork_ev, s));
+        for (int i = 0; i < n_side; ++i) SP_HIP(hipStreamWaitEvent(R.side[i], R.fork_ev, 0));
     }
     for (size_t k = 0; k < P.kinds.size(); ++k) {
         const sp_tuning& t = tun[k];
         const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, t.block_size / 64 * 64));
         const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
-        hipStream_t sk = k == 0 ? s : ctx->side[(k - 1) % sp_ctx::N_SIDE];
-        const bool timed = timing && k < (size_t)sp_ctx::N_CLASS_EV;
-        if (timed) SP_HIP(hipEventRecord(ctx->class_ev[2 * k], sk));
+        hipStream_t sk = (k == 0 || R.n_side == 0) ? s : R.side[(k - 1) % R.n_side];
+        const bool timed = timing && k < (size_t)IterRes::N_CLASS_EV;
+        if (timed) SP_HIP(hipEventRecord(R.class_ev[2 * k], sk));
         if (!launch_batch(P.kinds[k], grid, dim3(block), sk, d_reqs, d_work + P.kind_begin[k], P.kind_count[k], P.kind_tiles[k],
                           d_counters + k))
             return set_error(SP_ERR_UNSUPPORTED, "no batch kernel for a requested class");
-        if (timed) SP_HIP(hipEventRecord(ctx->class_ev[2 * k + 1], sk));
+        if (timed) SP_HIP(hipEventRecord(R.class_ev[2 * k + 1], sk));
     }
     for (int i = 0; i < n_side; ++i) {           // join
-        SP_HIP(hipEventRecord(ctx->join_ev[i], ctx->side[i]));
-        SP_HIP(hipStreamWaitEvent(s, ctx->join_ev[i], 0));
+        SP_HIP(hipEventRecord(R.join_ev[i], R.side[i]));
+        SP_HIP(hipStreamWaitEvent(s, R.join_ev[i], 0));
     }
     if (!P.reduce.empty()) launch_reduce(s, d_reduce, (uint32_t)P.reduce.size(), P.max_rows);
     return SP_OK;
@@ -311,12 +312,12 @@ static void assign_chunks(const sp_ctx* ctx, double budget_ms, const std::vector
     }
 }
 
-// smoothed device cost per particle-step of each class timed in the iteration just finished
-static void update_class_costs(sp_ctx* ctx, const IterPlan& P) {
-    const size_t n = std::min<size_t>(P.kinds.size(), (size_t)sp_ctx::N_CLASS_EV);
+// smoothed device cost per particle-step of each class timed in the iteration just finished (on R's events)
+static void update_class_costs(sp_ctx* ctx, const IterRes& R, const IterPlan& P) {
+    const size_t n = std::min<size_t>(P.kinds.size(), (size_t)IterRes::N_CLASS_EV);
     for (size_t k = 0; k < n; ++k) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, ctx->class_ev[2 * k], ctx->class_ev[2 * k + 1]) != hipSuccess) {
+        if (hipEventElapsedTime(&ms, R.class_ev[2 * k], R.class_ev[2 * k + 1]) != hipSuccess) {
             (void)hipGetLastError();
             continue;
         }
@@ -330,7 +331,7 @@ static void update_class_costs(sp_ctx* ctx, const IterPlan& P) {
 
 // ------------------------------------------------------------------ static batch (graph)
 static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::vector<Outputs>& outs, bool keep_final, bool use_graph) {
-    std::lock_guard<std::mutex> lk(ctx->run_mu);
+    RunLock lk(ctx);
     hipStream_t s = ctx->stream;
     SP_HIP(hipSetDevice(ctx->device));
     const size_t n = reqs.size();
@@ -384,19 +385,19 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
     size_t off = 0;
     std::vector<size_t> offs;
     for (size_t i = 0; i < plans.size(); ++i) {
-        fill_tables((char*)ctx->pinned + off, lays[i], dreqs, plans[i], ctx->partial);
+        fill_tables((char*)ctx->res.pinned + off, lays[i], dreqs, plans[i], ctx->res.partial);
         offs.push_back(off);
         off += lays[i].bytes;
     }
-    if (off) SP_HIP(hipMemcpyAsync(ctx->table, ctx->pinned, off, hipMemcpyHostToDevice, s));
+    if (off) SP_HIP(hipMemcpyAsync(ctx->res.table, ctx->res.pinned, off, hipMemcpyHostToDevice, s));
     std::vector<std::vector<sp_tuning>> tuns;
     for (auto& P : plans) tuns.push_back(tunings_for(ctx, P, order));
     auto record_all = [&](hipStream_t st) -> int {
         for (size_t i = 0; i < plans.size(); ++i) {
-            char* base = (char*)ctx->table + offs[i];
-            if (int e = record_iteration(ctx, st, plans[i], (const DevRequest*)(base + lays[i].req_off),
+            char* base = (char*)ctx->res.table + offs[i];
+            if (int e = record_iteration(ctx, ctx->res, st, plans[i], (const DevRequest*)(base + lays[i].req_off),
                                          (const SlotWork*)(base + lays[i].work_off), (const ReduceWork*)(base + lays[i].red_off),
-                                         ctx->counters, tuns[i]))
+                                         ctx->res.counters, tuns[i]))
                 return e;
         }
         return SP_OK;
@@ -461,20 +462,20 @@ static int run_batch_static(sp_ctx* ctx, std::vector<HostRequest>& reqs, std::ve
 }
 
 // ------------------------------------------------------------------ server worker
-static void finish_job(sp_ctx* ctx, Slot* sl) {
-    // called by the worker without ctx->mu; publishes under the lock.  A request that has already ended
-    // (failed, cancelled) is never published again.
+static void finish_job(sp_ctx* ctx, Slot* sl, hipStream_t s) {
+    // called by the worker without ctx->mu (s: the stream of the slot's lane); publishes under the lock.  A request
+    // that has already ended (failed, cancelled) is never published again.
     std::shared_ptr<Job> job = sl->job;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         if (job->status != SP_ERR_NOT_READY) return;
     }
     Outputs o;
-    int rc = download_outputs(*sl->hr, sl->dev, ctx->stream, job->keep_final, sl->shadow != nullptr, o);
+    int rc = download_outputs(*sl->hr, sl->dev, s, job->keep_final, sl->shadow != nullptr, o);
     bool requeue = false;
     if (rc == SP_OK && sl->shadow) {
         Outputs so;
-        rc = download_outputs(*sl->shadow->hr, sl->shadow->dev, ctx->stream, false, true, so);
+        rc = download_outputs(*sl->shadow->hr, sl->shadow->dev, s, false, true, so);
         if (rc == SP_OK) {
             const CheckResult cr = check_of(*sl->hr, o, so);
             o.check_error = cr.error;
@@ -520,41 +521,182 @@ static void finish_job(sp_ctx* ctx, Slot* sl) {
     ctx->done_cv.notify_all();
 }
 
+// The worker runs two lanes of iterations: Implicit_Midpoint requests (a Newton solve per step, several times the
+// device time of an explicit step) iterate on lane 1 with their own streams and scratch, every other request
+// (explicit solvers, mixed requests with their FP64 shadows) on lane 0.  The lanes' iterations overlap on the device
+// and each lane starts its next iteration as soon as its previous one ends, so requests of the cheap classes are not
+// held at the iteration boundaries of the expensive one.  A request's result does not depend on its lane or its
+// chunking.
+static int lane_of(const HostRequest& hr) { return hr.rq.solver == SP_SOLVER_IMPLICIT_MIDPOINT ? 1 : 0; }
+
+struct Lane {
+    IterRes* R = nullptr;
+    std::vector<std::unique_ptr<Slot>> active;
+    bool inflight = false;
+    IterPlan P;
+    std::vector<Slot*> slots;         // the in-flight iteration's slots (their DevRequest table positions)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool has_work() const {
+        for (auto& a : active) if (!a->done) return true;
+        return false;
+    }
+};
+
+// plan and issue one iteration of lane L (no wait); returns the error of the issue (no launch is then in flight)
+static int issue_iteration(sp_ctx* ctx, Lane& L) {
+    hipStream_t s = L.R->stream;
+    L.slots.clear();
+    for (auto& a : L.active) L.slots.push_back(a.get());
+    double budget_ms = 0.0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        budget_ms = ctx->iteration_budget_ms;
+    }
+    assign_chunks(ctx, L.R->class_timing ? budget_ms : 0.0, L.slots);
+    plan_iteration(L.slots, L.P);
+    if (L.P.work.empty()) return SP_OK;
+    std::vector<DevRequest> dreqs;
+    for (Slot* x : L.slots) dreqs.push_back(make_dev_request(*x->hr, x->dev));
+    const TableLayout T = layout_for(dreqs.size(), L.P.work.size(), L.P.reduce.size());
+    std::vector<sp_tuning> tun;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        tun = tunings_for(ctx, L.P, L.slots);
+        if (ctx->inject_failures > 0) {
+            --ctx->inject_failures;
+            return set_error(SP_ERR_DEVICE, "iteration failed (injected)");
+        }
+    }
+    if (int e = ensure_partial(*L.R, std::max<size_t>(8, L.P.partial_doubles * 8))) return e;
+    if (int e = ensure_table(*L.R, T.bytes)) return e;
+    if (int e = ensure_counters(*L.R, std::max<size_t>(1, L.P.kinds.size()))) return e;
+    fill_tables((char*)L.R->pinned, T, dreqs, L.P, L.R->partial);
+    SP_HIP(hipMemcpyAsync(L.R->table, L.R->pinned, T.bytes, hipMemcpyHostToDevice, s));
+    SP_HIP(hipEventRecord(L.e0, s));
+    char* base = (char*)L.R->table;
+    if (int e = record_iteration(ctx, *L.R, s, L.P, (const DevRequest*)(base + T.req_off), (const SlotWork*)(base + T.work_off),
+                                 (const ReduceWork*)(base + T.red_off), L.R->counters, tun, L.R->class_timing))
+        return e;
+    SP_HIP(hipEventRecord(L.e1, s));
+    L.inflight = true;
+    return SP_OK;
+}
+
+static void release_done(Lane& L);
+
+// bookkeeping after lane L's iteration ended with status rc (the device work of a successful one has completed)
+static void complete_iteration(sp_ctx* ctx, Lane& L, int rc) {
+    float ms = 0.f;
+    if (rc == SP_OK && L.inflight) {
+        if (hipEventElapsedTime(&ms, L.e0, L.e1) != hipSuccess) { (void)hipGetLastError(); ms = 0.f; }
+        if (L.R->class_timing) update_class_costs(ctx, *L.R, L.P);
+    }
+    L.inflight = false;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->stat_iterations += 1;
+        ctx->stat_busy_ms += ms;
+    }
+    for (Slot* x : L.slots) {     // device time share of this iteration by particle-steps
+        if (x->is_shadow || !x->job || L.P.particle_steps <= 0.0) continue;
+        x->job->device_ms_acc += ms * x->work_share / L.P.particle_steps;
+        x->work_share = 0.0;
+    }
+    if (rc == SP_OK) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (Slot* x : L.slots)
+            if (!x->is_shadow && x->job) x->job->steps_done = x->next_step;
+    }
+    if (rc != SP_OK) {
+        // every slot of the failed iteration ends, FP64 shadows included: a shadow left running would
+        // later publish its parent's partial sums
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (Slot* x : L.slots) {
+            if (x->done) continue;
+            if (x->job && x->job->status == SP_ERR_NOT_READY) x->job->status = rc;
+            x->done = true;
+        }
+        ctx->done_cv.notify_all();
+    } else {
+        const auto& F = L.P.finishing;
+        for (Slot* f : F) f->done = true;
+        for (Slot* f : F)
+            if (!f->is_shadow && f->job && (!f->shadow || f->shadow->done)) finish_job(ctx, f, L.R->stream);
+        for (Slot* f : F)
+            if (f->is_shadow && f->parent && f->parent->done && f->parent->job &&
+                std::find(F.begin(), F.end(), f->parent) == F.end())
+                finish_job(ctx, f->parent, L.R->stream);
+    }
+    L.slots.clear();
+    release_done(L);
+}
+
+// drain every stream of R (after an issue that failed part-way, launches may be queued on any of them)
+static void drain(IterRes& R) {
+    if (hipStreamSynchronize(R.stream) != hipSuccess) (void)hipGetLastError();
+    for (int i = 0; i < IterRes::N_SIDE; ++i)
+        if (R.side[i] && hipStreamSynchronize(R.side[i]) != hipSuccess) (void)hipGetLastError();
+}
+
+static void release_done(Lane& L) {
+    // release finished slots (parent and shadow together: a mixed request and its shadow share a lane)
+    std::vector<std::unique_ptr<Slot>> keep;
+    for (auto& a : L.active) {
+        const bool done_pair = a->done && (a->is_shadow ? (a->parent == nullptr || a->parent->done) : (!a->shadow || a->shadow->done));
+        if (done_pair) free_dev(a->dev);
+        else keep.push_back(std::move(a));
+    }
+    // parents point at shadows: clear dangling links of the survivors
+    for (auto& a : keep) {
+        bool has = false;
+        for (auto& b : keep) if (b.get() == a->shadow || b.get() == a->parent) has = true;
+        if (!has) { a->shadow = nullptr; a->parent = nullptr; }
+    }
+    L.active.swap(keep);
+}
+
 static void worker_main(sp_ctx* ctx) {
     hipSetDevice(ctx->device);
-    hipStream_t s = ctx->stream;
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    bool class_timing = true;                       // per-class launch timing for the iteration budget
-    for (int i = 0; i < 2 * sp_ctx::N_CLASS_EV && class_timing; ++i)
-        if (hipEventCreate(&ctx->class_ev[i]) != hipSuccess) {
-            (void)hipGetLastError();
-            class_timing = false;
-        }
-    std::vector<std::unique_ptr<Slot>> active;
+    IterRes own;                                    // lane 1's streams and scratch
+    const bool two_lanes = init_res(own, nullptr, 0) == SP_OK;
+    if (!two_lanes) { (void)hipGetLastError(); destroy_res(own); }
+    Lane lanes[2];
+    lanes[0].R = &ctx->res;
+    lanes[1].R = two_lanes ? &own : &ctx->res;
+    const int n_lanes = two_lanes ? 2 : 1;
+    for (int l = 0; l < n_lanes; ++l) {
+        init_class_timing(*lanes[l].R);             // per-class launch timing for the iteration budget
+        hipEventCreate(&lanes[l].e0);
+        hipEventCreate(&lanes[l].e1);
+    }
+    auto any_inflight = [&] { return lanes[0].inflight || lanes[1].inflight; };
+    size_t pending_left = 0;                        // requests left queued by the last admission pass
+    std::unique_lock<std::mutex> device_lock(ctx->run_mu, std::defer_lock);
     for (;;) {
         std::vector<std::shared_ptr<Job>> admitted;
         std::vector<size_t> admitted_need;
         {
             std::unique_lock<std::mutex> lk(ctx->mu);
-            ctx->cv.wait(lk, [&] { return ctx->stop || !ctx->pending.empty() || !active.empty(); });
+            if (!any_inflight())
+                ctx->cv.wait(lk, [&] { return ctx->stop || !ctx->pending.empty() || lanes[0].has_work() || lanes[1].has_work(); });
             if (ctx->stop) {
                 for (auto& j : ctx->pending) { j->status = SP_ERR_CANCELLED; }
                 ctx->pending.clear();
-                for (auto& a : active) if (!a->is_shadow && a->job) a->job->status = SP_ERR_CANCELLED;
+                for (auto& L : lanes)
+                    for (auto& a : L.active) if (!a->is_shadow && a->job) a->job->status = SP_ERR_CANCELLED;
                 ctx->done_cv.notify_all();
                 break;
             }
-            // drop cancelled active requests (the active list is the worker's own)
-            for (auto& a : active) {
-                if (!a->is_shadow && a->job && a->job->cancel && !a->done) {
-                    a->done = true;
-                    if (a->shadow) a->shadow->done = true;
-                    a->job->status = SP_ERR_CANCELLED;
-                    ctx->done_cv.notify_all();
+            // drop cancelled active requests (the active lists are the worker's own)
+            for (auto& L : lanes)
+                for (auto& a : L.active) {
+                    if (!a->is_shadow && a->job && a->job->cancel && !a->done) {
+                        a->done = true;
+                        if (a->shadow) a->shadow->done = true;
+                        a->job->status = SP_ERR_CANCELLED;
+                        ctx->done_cv.notify_all();
+                    }
                 }
-            }
             // admission by device-memory footprint (state + parameters + outputs, shadow included)
             while (!ctx->pending.empty()) {
                 auto& j = ctx->pending.front();
@@ -562,17 +704,21 @@ static void worker_main(sp_ctx* ctx) {
                 size_t need = request_bytes(j->hr) +
                               (wants_shadow(j->hr) ? request_bytes(j->hr) / std::max<uint64_t>(1, j->hr.n_tiles) * j->hr.n_check_tiles * 2 : 0);
                 size_t n_active = admitted.size();
-                for (auto& a : active) if (!a->is_shadow && !a->done) ++n_active;
+                for (auto& L : lanes)
+                    for (auto& a : L.active) if (!a->is_shadow && !a->done) ++n_active;
                 if (n_active > 0 && (ctx->mem_used + need > ctx->mem_budget || n_active >= ctx->max_active)) break;
                 admitted.push_back(j);
                 admitted_need.push_back(need);
                 ctx->mem_used += need;
                 ctx->pending.pop_front();
             }
+            pending_left = ctx->pending.size();
         }
-        std::unique_lock<std::mutex> device_lock(ctx->run_mu);   // the worker owns the stream and scratch from here
+        if (!device_lock.owns_lock()) device_lock.lock();   // the worker owns the streams and scratch from here
         for (size_t ai = 0; ai < admitted.size(); ++ai) {
             auto& j = admitted[ai];
+            Lane& L = lanes[two_lanes ? lane_of(j->hr) : 0];
+            hipStream_t s = L.R->stream;
             auto sl = std::make_unique<Slot>();
             sl->job = j;
             sl->hr = &j->hr;
@@ -592,7 +738,7 @@ static void worker_main(sp_ctx* ctx) {
                 sh->tuned_chunk = sh->chunk = sl->chunk;
                 rc = alloc_dev(sh->shadow_req, sh->dev, s);
                 sl->shadow = sh.get();
-                if (rc == SP_OK) active.push_back(std::move(sh));
+                if (rc == SP_OK) L.active.push_back(std::move(sh));
             }
             if (rc != SP_OK) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
@@ -602,117 +748,86 @@ static void worker_main(sp_ctx* ctx) {
                 free_dev(sl->dev);
                 continue;
             }
-            active.push_back(std::move(sl));
+            L.active.push_back(std::move(sl));
         }
-        // one iteration over all active slots
-        std::vector<Slot*> slots;
-        for (auto& a : active) slots.push_back(a.get());
-        double budget_ms = 0.0;
-        {
-            std::lock_guard<std::mutex> lk(ctx->mu);
-            budget_ms = ctx->iteration_budget_ms;
-        }
-        assign_chunks(ctx, class_timing ? budget_ms : 0.0, slots);
-        IterPlan P;
-        plan_iteration(slots, P);
-        if (!P.work.empty()) {
-            std::vector<DevRequest> dreqs;
-            for (Slot* x : slots) dreqs.push_back(make_dev_request(*x->hr, x->dev));
-            const TableLayout L = layout_for(dreqs.size(), P.work.size(), P.reduce.size());
-            std::vector<sp_tuning> tun;
-            {
-                std::lock_guard<std::mutex> lk(ctx->mu);
-                tun = tunings_for(ctx, P, slots);
-            }
-            int rc = SP_OK;
-            {
-                std::lock_guard<std::mutex> lk(ctx->mu);
-                if (ctx->inject_failures > 0) {
-                    --ctx->inject_failures;
-                    rc = set_error(SP_ERR_DEVICE, "iteration failed (injected)");
+        // a synchronous call waiting for the device gets it once the lanes in flight have ended
+        const bool yield_device = ctx->run_waiters.load() > 0;
+        if (!yield_device) {
+            for (int l = 0; l < n_lanes; ++l) {
+                Lane& L = lanes[l];
+                if (L.inflight || !L.has_work()) continue;
+                const int rc = issue_iteration(ctx, L);
+                if (rc != SP_OK) {
+                    drain(*L.R);                    // launches issued before the failure end first
+                    L.inflight = false;
+                    complete_iteration(ctx, L, rc);
                 }
             }
-            if (rc == SP_OK) rc = ensure_partial(ctx, std::max<size_t>(8, P.partial_doubles * 8));
-            if (rc == SP_OK) rc = ensure_table(ctx, L.bytes);
-            if (rc == SP_OK) rc = ensure_counters(ctx, std::max<size_t>(1, P.kinds.size()));
-            if (rc == SP_OK) {
-                fill_tables((char*)ctx->pinned, L, dreqs, P, ctx->partial);
-                hipMemcpyAsync(ctx->table, ctx->pinned, L.bytes, hipMemcpyHostToDevice, s);
-                hipEventRecord(e0, s);
-                char* base = (char*)ctx->table;
-                rc = record_iteration(ctx, s, P, (const DevRequest*)(base + L.req_off), (const SlotWork*)(base + L.work_off),
-                                      (const ReduceWork*)(base + L.red_off), ctx->counters, tun, class_timing);
-                hipEventRecord(e1, s);
-                if (hipStreamSynchronize(s) != hipSuccess && rc == SP_OK) rc = SP_ERR_DEVICE;
-                if (rc == SP_OK && class_timing) update_class_costs(ctx, P);
+        }
+        // cancelled or finished slots of the lanes not in flight
+        for (int l = 0; l < n_lanes; ++l)
+            if (!lanes[l].inflight) release_done(lanes[l]);
+        // wait until a lane's iteration ends, or a request that arrived since the admission above can start an idle lane
+        while (any_inflight()) {
+            bool ended = false;
+            for (int l = 0; l < n_lanes; ++l) {
+                Lane& L = lanes[l];
+                if (!L.inflight) continue;
+                const hipError_t q = hipEventQuery(L.e1);
+                if (q == hipErrorNotReady) continue;
+                int rc = SP_OK;
+                if (q != hipSuccess || hipStreamSynchronize(L.R->stream) != hipSuccess) {
+                    (void)hipGetLastError();
+                    drain(*L.R);
+                    rc = set_error(SP_ERR_DEVICE, "server iteration failed on the device");
+                }
+                complete_iteration(ctx, L, rc);
+                ended = true;
             }
-            float ms = 0.f;
-            if (rc == SP_OK) hipEventElapsedTime(&ms, e0, e1);
+            if (ended) break;
             {
                 std::lock_guard<std::mutex> lk(ctx->mu);
-                ctx->stat_iterations += 1;
-                ctx->stat_busy_ms += ms;
+                bool idle_lane = false;
+                for (int l = 0; l < n_lanes; ++l) idle_lane = idle_lane || !lanes[l].inflight;
+                if (ctx->stop || (ctx->pending.size() > pending_left && idle_lane)) break;
+                bool cancelled = false;
+                for (auto& L : lanes)
+                    for (auto& a : L.active) cancelled = cancelled || (!a->is_shadow && a->job && a->job->cancel && !a->done);
+                if (cancelled) break;
             }
-            for (Slot* x : slots) {     // device time share of this iteration by particle-steps
-                if (x->is_shadow || !x->job || P.particle_steps <= 0.0) continue;
-                x->job->device_ms_acc += ms * x->work_share / P.particle_steps;
-                x->work_share = 0.0;
-            }
-            if (rc == SP_OK) {
-                std::lock_guard<std::mutex> lk(ctx->mu);
-                for (Slot* x : slots)
-                    if (!x->is_shadow && x->job) x->job->steps_done = x->next_step;
-            }
-            if (rc != SP_OK) {
-                // every slot of the failed iteration ends, FP64 shadows included: a shadow left running would
-                // later publish its parent's partial sums
-                std::lock_guard<std::mutex> lk(ctx->mu);
-                for (Slot* x : slots) {
-                    if (x->done) continue;
-                    if (x->job && x->job->status == SP_ERR_NOT_READY) x->job->status = rc;
-                    x->done = true;
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        if (!any_inflight()) {
+            device_lock.unlock();
+            while (ctx->run_waiters.load() > 0) {       // let the waiting synchronous calls take the device first
+                {
+                    std::lock_guard<std::mutex> lk(ctx->mu);
+                    if (ctx->stop) break;
                 }
-                ctx->done_cv.notify_all();
-            } else {
-                for (Slot* f : P.finishing) f->done = true;
-                for (Slot* f : P.finishing)
-                    if (!f->is_shadow && f->job && (!f->shadow || f->shadow->done)) finish_job(ctx, f);
-                for (Slot* f : P.finishing)
-                    if (f->is_shadow && f->parent && f->parent->done && f->parent->job &&
-                        std::find(P.finishing.begin(), P.finishing.end(), f->parent) == P.finishing.end())
-                        finish_job(ctx, f->parent);
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
             }
         }
-        // release finished slots (parent and shadow together)
-        std::vector<std::unique_ptr<Slot>> keep;
-        size_t freed = 0;
-        for (auto& a : active) {
-            const bool done_pair = a->done && (a->is_shadow ? (a->parent == nullptr || a->parent->done) : (!a->shadow || a->shadow->done));
-            if (done_pair) { freed += a->dev.bytes; free_dev(a->dev); }
-            else keep.push_back(std::move(a));
-        }
-        // parents point at shadows: clear dangling links of the survivors
-        for (auto& a : keep) {
-            bool has = false;
-            for (auto& b : keep) if (b.get() == a->shadow || b.get() == a->parent) has = true;
-            if (!has) { a->shadow = nullptr; a->parent = nullptr; }
-        }
-        active.swap(keep);
-        device_lock.unlock();
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             size_t acc = 0, n_act = 0;
-            for (auto& a : active) { acc += a->dev.bytes; if (!a->is_shadow && !a->done) ++n_act; }
-            if (freed) ctx->mem_used = acc;
+            for (auto& L : lanes)
+                for (auto& a : L.active) { acc += a->dev.bytes; if (!a->is_shadow && !a->done) ++n_act; }
+            ctx->mem_used = acc;
             ctx->stat_active = (double)n_act;
         }
     }
-    for (auto& a : active) free_dev(a->dev);
-    active.clear();
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (int i = 0; i < 2 * sp_ctx::N_CLASS_EV; ++i)
-        if (ctx->class_ev[i]) { hipEventDestroy(ctx->class_ev[i]); ctx->class_ev[i] = nullptr; }
+    if (!device_lock.owns_lock()) device_lock.lock();
+    for (auto& L : lanes) {
+        if (L.inflight) hipStreamSynchronize(L.R->stream);
+        L.inflight = false;
+        for (auto& a : L.active) free_dev(a->dev);
+        L.active.clear();
+    }
+    for (int l = 0; l < n_lanes; ++l) {
+        if (lanes[l].e0) hipEventDestroy(lanes[l].e0);
+        if (lanes[l].e1) hipEventDestroy(lanes[l].e1);
+    }
+    if (two_lanes) destroy_res(own);
 }
 
 static void ensure_worker(sp_ctx* ctx) {

@@ -118,6 +118,23 @@ def test_server_alone_vs_mix(ctx):
     assert st["completed"] >= 8 and st["iterations"] >= 1
 
 
+def test_synchronous_run_while_serving(ctx):
+    """The server's worker keeps the device while iterations of its two lanes (explicit solvers; Implicit_Midpoint)
+    are in flight; a synchronous run issued meanwhile waits for the lanes to drain, takes the device and returns
+    the plain result, and the served requests then finish with theirs."""
+    bag = [sph_ensemble(n=256, n_steps=4000, solver="Implicit_Midpoint", seed=90 + i) for i in range(3)] + \
+          [sph_ensemble(n=512, n_steps=6000, solver="Heun_Strong", seed=95 + i) for i in range(3)]
+    probe = neel_ensemble(n=128, n_steps=2000, solver="Heun_Strong", seed=99)
+    plain_probe = ctx.run(probe)
+    plain = [ctx.run(e) for e in bag]
+    ids = [ctx.submit(e) for e in bag]
+    during = ctx.run(probe)
+    same(plain_probe, during)
+    for rid, e, p in zip(ids, bag, plain):
+        assert ctx.wait(rid, 120_000) == SP_OK
+        same(p, ctx.result(rid, e))
+
+
 @pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_chunk_boundaries_do_not_change_results(ctx, solver, precision):
@@ -154,7 +171,7 @@ def test_server_iteration_budget_keeps_results(ctx):
                     assert np.array_equal(g["mean"], p["mean"]), (budget, rnd, i, e.problem, e.solver,
                                                                   float(np.abs(g["mean"] - p["mean"]).max()))
     finally:
-        ctx.set_iteration_budget(10.0)
+        ctx.set_iteration_budget(5.0)
     with pytest.raises(SpError):
         ctx.set_iteration_budget(-1.0)
 
