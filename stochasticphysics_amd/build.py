@@ -30,9 +30,9 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC"
 # (source, extra flags, object name); a source compiled several times gets one object per flag set
 SOURCES = [
     "sp_kernels_neel.hip", "sp_kernels_uc.hip",
-    # BrownAndNeel: FP64 and ImplicitMidpoint; mixed precision without machine LICM (see sp_kernels_bn.hip)
+    # BrownAndNeel: FP64 and ImplicitMidpoint; mixed precision (see sp_kernels_bn.hip)
     ("sp_kernels_bn.hip", ["-DSP_BN_PART=1"], "sp_kernels_bn64.o"),
-    ("sp_kernels_bn.hip", ["-DSP_BN_PART=2", "-mllvm", "-disable-machine-licm"], "sp_kernels_bn.o"),
+    ("sp_kernels_bn.hip", ["-DSP_BN_PART=2"], "sp_kernels_bn.o"),
     # the FP64 explicit NeelSpherical classes without machine LICM (see the head of sp_kernels_sph.hip)
     ("sp_kernels_sph.hip", ["-DSP_SPH_PART=1", "-mllvm", "-disable-machine-licm"], "sp_kernels_sph64.o"),
     ("sp_kernels_sph.hip", ["-DSP_SPH_PART=2"], "sp_kernels_sph.o"),

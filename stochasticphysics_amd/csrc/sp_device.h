@@ -253,6 +253,8 @@ __device__ __forceinline__ V3<R> neel_s(const NeelLane<ANISO, R>& L, V3<R> m, V3
 
 template <int ANISO, int SOLVER, class R> struct NeelCfg {
     static constexpr int DY = 3, DW = 3, DOUT = 3, NORM = 1;   // NORM: 3-blocks that normalize() rescales
+    // FP64: the compiler's register count (208-235, two waves per SIMD); FP32 physics: four waves (128 registers)
+    static constexpr int WAVES_PER_SIMD = sizeof(R) == 8 ? 1 : 4;
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = NeelLane<ANISO, R>;
     struct Cache {};
@@ -346,7 +348,7 @@ __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
     static constexpr int DY = 6, DW = 6, DOUT = 6, NORM = 2;
-    static constexpr int WAVES_PER_SIMD = sizeof(R) == 8 ? 2 : 4;   // FP64: two; FP32 physics: four
+    static constexpr int WAVES_PER_SIMD = 2;     // FP64 (with a one-wave build for small launches) and FP32 physics
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNLane<ANISO, R>;
     struct Cache {};
