@@ -280,7 +280,9 @@ int sp_run_batch(sp_ctx* ctx, uint32_t n_req, const sp_request* reqs,
                  const sp_particles* parts, sp_result* res);
 
 /* ---- serving: continuous batching request loop.  sp_submit copies the inputs;
- *      sp_wait blocks; sp_result copies the outputs and releases the request ---- */
+ *      sp_wait blocks; sp_result copies the outputs and releases the request.  A worker thread runs the
+ *      requests in two lanes of iterations that overlap on the device: Implicit_Midpoint requests in one,
+ *      every other request in the other ---- */
 int sp_submit(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, uint64_t* id);
 int sp_wait(sp_ctx* ctx, uint64_t id, int timeout_ms);
 int sp_result_get(sp_ctx* ctx, uint64_t id, sp_result* res);
@@ -289,8 +291,10 @@ int sp_cancel(sp_ctx* ctx, uint64_t id);   /* replaces SimulationManager::abort 
  * the per-finished-simulation count of SimulationManager::finishSimulationTask (SimulationManager.h:177-214), which a
  * GPU ensemble that advances all its particles together cannot give */
 int sp_progress(sp_ctx* ctx, uint64_t id, double* fraction);
-int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending, iterations,
-                                                 busy_ms, wall_ms, particle_steps, fallbacks */);
+int sp_server_stats(sp_ctx* ctx, double* out /* [8]: completed, active, pending, iterations (both lanes),
+                                                 busy_ms (device time of the iterations, summed over the
+                                                 lanes, so above wall_ms while they overlap), wall_ms,
+                                                 particle_steps, fallbacks */);
 
 /* ---- parameter files: the reference's .ini schema (Settings/SimulationManagerSettings.h:34-186,
  *      Provider/ParticleProvider.h:230-295) read into one request; CPU only ---- */
