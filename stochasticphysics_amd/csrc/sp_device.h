@@ -1017,13 +1017,72 @@ __device__ __forceinline__ void normalize_blocks(double* y) {
 
 template <bool B> struct BoolTag { static constexpr bool value = B; };
 
+// ================================================================ split noise
+// A tile of a small launch (one that leaves most SIMDs idle) runs as two waves of one work-group
+// (k_plain_split): a noise wave computes the Philox + Box-Muller normals of the tile's 64 particles into an LDS
+// ring, the integrator wave reads them and runs the physics, so the noise work moves to an otherwise idle SIMD.
+// The ring holds two buffers of SPLIT_NG step groups (12 normals per lane and group, packed 16 bytes per lane,
+// consecutive lanes at consecutive addresses); iteration i of both waves ends at a barrier, the noise wave fills
+// buffer i & 1 with batch i and the integrator consumes batch i - 1 from the other buffer, so a buffer is written
+// only after the barrier that follows its last read.  Both waves run the same number of iterations, decided from
+// the same request fields.
+template <class R> struct alignas(16) NoisePack { R v[16 / sizeof(R)]; };
+template <class R> constexpr int noise_packs() { return 12 / (int)(16 / sizeof(R)); }
+constexpr int SPLIT_NG = 2;       // step groups per ring buffer and noise wave
+// ring of a split tile with NW noise waves: two buffers of NW * SPLIT_NG step groups
+template <class R, int NW> constexpr int split_ring_packs() { return 2 * NW * SPLIT_NG * noise_packs<R>() * 64; }
+
+// whether a tile takes the fast path of run_tile (Philox noise, windows ending on group boundaries); uniform
+template <class Cfg>
+__device__ __forceinline__ bool fast_path(const DevRequest& rq) {
+    const uint32_t flags = __builtin_amdgcn_readfirstlane(rq.flags);
+    const uint32_t OS = __builtin_amdgcn_readfirstlane(rq.oversampling);
+    return OS % Cfg::G == 0 && (flags & RQ_REPLAY) == 0;
+}
+
+// noise wave k of the NW of a split tile: groups [k SPLIT_NG, (k + 1) SPLIT_NG) of every batch of NW SPLIT_NG
+// groups of the steps [tw.step_begin, tw.step_end)
+template <class Cfg, class R, int NW>
+__device__ __forceinline__ void produce_noise(const DevRequest& rq, const TileWork& tw, NoisePack<R>* ring, int k) {
+    constexpr int G = Cfg::G, NZ = Cfg::NZ, PK = 16 / (int)sizeof(R), NP = noise_packs<R>(), NB = NW * SPLIT_NG;
+    const int lane = threadIdx.x & 63;
+    const uint64_t N = rq.n_particles;
+    const uint64_t pr = (uint64_t)tw.tile * 64 + lane;
+    const uint64_t p = pr < N ? pr : N - 1;
+    const uint64_t seed = rq.seed;
+    const uint64_t ng = (tw.step_end - tw.step_begin) / G;
+    const uint64_t nb = (ng + NB - 1) / NB;
+    for (uint64_t i = 0; i <= nb; ++i) {
+        if (i < nb) {
+            NoisePack<R>* buf = ring + (i & 1) * (NB * NP * 64);
+#pragma unroll
+            for (int g = k * SPLIT_NG; g < (k + 1) * SPLIT_NG; ++g) {
+                const uint64_t gi = i * NB + g;
+                if (gi < ng) {
+                    R z[12];
+                    NormalsT<R>::philox(seed, p, ((tw.step_begin + gi * G) * NZ) >> 2, z);
+#pragma unroll
+                    for (int k = 0; k < NP; ++k) {
+                        NoisePack<R> pk;
+#pragma unroll
+                        for (int e = 0; e < PK; ++e) pk.v[e] = z[k * PK + e];
+                        buf[(g * NP + k) * 64 + lane] = pk;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ================================================================ tile driver
 // Runs steps [tw.step_begin, tw.step_end) of one 64-particle tile
 // (SingleParticleSimulator::startSimulation, :129-159, for 64 particles at
 // once) and writes one moment-weighted partial sum per window and component
 // (MeanSimulationResult::operator+=, :87-119, for the wave's 64 particles).
-template <class Cfg, class R, bool KAHAN, bool MIDPOINT>
-__device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& tw) {
+// NW > 0: the integrator wave of a split tile, whose fast path reads its normals from the ring of its NW noise waves.
+template <class Cfg, class R, bool KAHAN, bool MIDPOINT, int NW = 0>
+__device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& tw, const NoisePack<R>* ring = nullptr) {
     constexpr int DY = Cfg::DY, NZ = Cfg::NZ, DOUT = Cfg::DOUT, G = Cfg::G;   // NZ normals per step, 12 per group
     const int lane = threadIdx.x & 63;
     const uint64_t N = rq.n_particles;
@@ -1129,39 +1188,75 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
             if (l == OS) { close_window(); l = 0; }
         }
     };
+    // one step group from its 12 normals; the G outputs of the group are added first and enter the window's
+    // compensated sum as one term (the reference compensates every step, SingleParticleSimulator.h:148-153; the
+    // two differ by the rounding of G - 1 plain additions)
+    auto group = [&](uint64_t n0, const R* zc, auto renorm_tag) {
+        constexpr bool RN = decltype(renorm_tag)::value;
+        double gsum[DOUT];
+#pragma unroll
+        for (int s = 0; s < G; ++s) {
+            Cfg::step(rq, L, y, cache, zc + s * NZ, fc.at(n0 + s));
+            if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
+            double o[DOUT];
+            Cfg::output(L, y, cache, o);
+#pragma unroll
+            for (int c = 0; c < DOUT; ++c) gsum[c] = s == 0 ? o[c] : gsum[c] + o[c];
+        }
+        accumulate(gsum);
+        l += G;
+        if (l == OS) { close_window(); l = 0; }
+    };
     auto fast = [&](auto renorm_tag) {
         if constexpr (!pipe_noise<Cfg>::value) {
             fast_unpiped(renorm_tag);
             return;
         }
-        constexpr bool RN = decltype(renorm_tag)::value;
         const uint64_t seed = rq.seed;
         R zc[12], zn[12];
         if (tw.step_begin < tw.step_end) NormalsT<R>::philox(seed, p, (tw.step_begin * NZ) >> 2, zc);
         for (uint64_t n0 = tw.step_begin; n0 < tw.step_end; n0 += G) {
             fc.ensure<MIDPOINT>(rq, n0, G);
             NormalsT<R>::philox(seed, p, ((n0 + G) * NZ) >> 2, zn);
-            // the G outputs of the group are added first and enter the window's compensated sum as one term
-            // (the reference compensates every step, SingleParticleSimulator.h:148-153; the two differ by the
-            // rounding of G - 1 plain additions)
-            double gsum[DOUT];
-#pragma unroll
-            for (int s = 0; s < G; ++s) {
-                Cfg::step(rq, L, y, cache, zc + s * NZ, fc.at(n0 + s));
-                if constexpr (RN) normalize_blocks<Cfg::NORM>(y);
-                double o[DOUT];
-                Cfg::output(L, y, cache, o);
-#pragma unroll
-                for (int c = 0; c < DOUT; ++c) gsum[c] = s == 0 ? o[c] : gsum[c] + o[c];
-            }
-            accumulate(gsum);
-            l += G;
-            if (l == OS) { close_window(); l = 0; }
+            group(n0, zc, renorm_tag);
 #pragma unroll
             for (int j = 0; j < 12; ++j) zc[j] = zn[j];
         }
     };
-    if (OS % G == 0 && !replay) {
+    // the integrator of a split tile: batch i - 1 from the ring, then the barrier of iteration i (produce_noise)
+    auto fast_split = [&](auto renorm_tag) {
+        constexpr int PK = 16 / (int)sizeof(R), NP = noise_packs<R>(), NB = (NW > 0 ? NW : 1) * SPLIT_NG;
+        const uint64_t ng = (tw.step_end - tw.step_begin) / G;
+        const uint64_t nb = (ng + NB - 1) / NB;
+        for (uint64_t i = 0; i <= nb; ++i) {
+            if (i > 0) {
+                const NoisePack<R>* buf = ring + ((i - 1) & 1) * (NB * NP * 64);
+#pragma unroll
+                for (int g = 0; g < NB; ++g) {
+                    const uint64_t gi = (i - 1) * NB + g;
+                    if (gi < ng) {
+                        const uint64_t n0 = tw.step_begin + gi * G;
+                        R zc[12];
+#pragma unroll
+                        for (int k = 0; k < NP; ++k) {
+                            const NoisePack<R> pk = buf[(g * NP + k) * 64 + lane];
+#pragma unroll
+                            for (int e = 0; e < PK; ++e) zc[k * PK + e] = pk.v[e];
+                        }
+                        fc.ensure<MIDPOINT>(rq, n0, G);
+                        group(n0, zc, renorm_tag);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    };
+    if (NW > 0 && fast_path<Cfg>(rq)) {
+        if constexpr (NW > 0) {
+            if (renorm) fast_split(BoolTag<true>{});
+            else fast_split(BoolTag<false>{});
+        }
+    } else if (OS % G == 0 && !replay) {
         if (renorm) fast(BoolTag<true>{});
         else fast(BoolTag<false>{});
     } else {

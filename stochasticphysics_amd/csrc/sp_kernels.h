@@ -22,6 +22,10 @@ bool launch_batch(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const Dev
 bool kernel_info(uint32_t kind, KernelInfo* info);
 // SIMDs of the current device (four per compute unit; queried once per device)
 uint32_t device_simds();
+// noise waves per tile of a plain launch of `tiles` tiles that runs split (k_plain_split: noise waves beside each
+// integrator wave): 2 when the launch leaves two thirds of the device's SIMDs idle, 1 when it leaves half, else 0
+// (not split); SP_PLAIN_SPLIT=0 in the environment turns the split kernel off
+int split_noise_waves(uint32_t tiles);
 
 // per-problem registries (one translation unit each)
 bool launch_plain_neel(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);

@@ -59,7 +59,7 @@ bool info_bn64(uint32_t kind, KernelInfo* info) {
 bool launch_plain_bn(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                      uint64_t se, double* partial, uint32_t pstride) {
     switch (kind) {
-        SP_CLASSES(SP_CASE_PLAIN)
+        SP_CLASSES(SP_CASE_PLAIN_SPLIT)
         default: return launch_plain_bn64(kind, grid, block, s, rq, sb, se, partial, pstride);
     }
 }

@@ -25,6 +25,18 @@ uint32_t device_simds() {
     return v;
 }
 
+int split_noise_waves(uint32_t tiles) {
+    static const bool enabled = [] {
+        const char* e = getenv("SP_PLAIN_SPLIT");
+        return !(e && e[0] == '0');
+    }();
+    if (!enabled || tiles == 0) return 0;
+    const uint64_t simds = device_simds();
+    if (3ull * tiles <= simds) return 2;
+    if (2ull * tiles <= simds) return 1;
+    return 0;
+}
+
 bool launch_plain(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* rq, uint64_t sb,
                   uint64_t se, double* partial, uint32_t pstride) {
     if (kind_is_uniaxialcubic(kind) && kind / 128 <= 2) return launch_plain_uc(kind, grid, block, s, rq, sb, se, partial, pstride);

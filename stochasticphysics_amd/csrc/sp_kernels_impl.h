@@ -28,6 +28,39 @@ __global__ __launch_bounds__(MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(waves
     run_tile<Cfg, R, KAHAN, MID>(rq, tw);
 }
 
+// One 64-particle tile per work-group of 1 + NW waves: waves 1..NW compute the tile's normals into an LDS ring,
+// wave 0 integrates (run_tile's split mode).  For launches that leave most SIMDs idle: the noise work moves to SIMDs
+// of its own.  Tiles that do not take the fast path (replayed noise, windows not ending on group boundaries) run in
+// wave 0 alone.  All waves of a block take every decision below from the same uniform values, so they reach the
+// same barriers.
+template <class Cfg, class R, bool KAHAN, bool MID, int NW>
+__global__ __launch_bounds__(64 * (1 + NW)) void k_plain_split(const DevRequest* __restrict__ rqp, uint64_t step_begin,
+                                                               uint64_t step_end, double* partial, uint32_t pstride) {
+    __shared__ NoisePack<R> ring[split_ring_packs<R, NW>()];
+    const DevRequest& rq = *rqp;
+    bm_lds_fill();       // every thread, before any wave leaves
+    const uint32_t tile = blockIdx.x;
+    if (tile >= __builtin_amdgcn_readfirstlane(rq.n_tiles)) return;    // uniform over the block
+    const TileWork tw{0u, tile, step_begin, step_end, partial, pstride, tile};
+    const int w = (int)(threadIdx.x >> 6);
+    if (w > 0) {
+        if (fast_path<Cfg>(rq)) produce_noise<Cfg, R, NW>(rq, tw, ring, w - 1);
+        return;
+    }
+    run_tile<Cfg, R, KAHAN, MID, NW>(rq, tw, ring);
+}
+// a plain launch of `tiles` tiles, split when it leaves at least half of the SIMDs idle (two noise waves per tile
+// when it leaves two thirds)
+template <class Cfg, class R, bool KAHAN, bool MID>
+bool launch_split(uint32_t tiles, hipStream_t s, const DevRequest* rq, uint64_t sb, uint64_t se, double* partial,
+                  uint32_t pstride) {
+    const int nw = split_noise_waves(tiles);
+    if (nw == 2) k_plain_split<Cfg, R, KAHAN, MID, 2><<<dim3(tiles), dim3(192), 0, s>>>(rq, sb, se, partial, pstride);
+    else if (nw == 1) k_plain_split<Cfg, R, KAHAN, MID, 1><<<dim3(tiles), dim3(128), 0, s>>>(rq, sb, se, partial, pstride);
+    else return false;
+    return true;
+}
+
 __device__ __forceinline__ uint32_t next_work(uint32_t* counter) {
     uint32_t idx = 0;
     if ((threadIdx.x & 63) == 0) idx = atomicAdd(counter, 1u);
@@ -99,9 +132,17 @@ bool fill_info(KernelInfo* info, const char* name) {
 // the plain launch of a class that also has its one-wave build (OneWave)
 #define SP_CASE_PLAIN_FIT(KIND, CFG, R, KAHAN, MID)                                                    \
     case KIND:                                                                                           \
-        if ((uint64_t)grid.x * block.x <= 64ull * device_simds())                                        \
+        if (launch_split<CFG, R, KAHAN, MID>(grid.x * (block.x / 64), s, rq, sb, se, partial, pstride))  \
+            ;                                                                                            \
+        else if ((uint64_t)grid.x * block.x <= 64ull * device_simds())                                   \
             k_plain<OneWave<CFG>, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);  \
         else                                                                                             \
+            k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);            \
+        return true;
+// the plain launch of a class with the split-noise kernel for small launches
+#define SP_CASE_PLAIN_SPLIT(KIND, CFG, R, KAHAN, MID)                                                  \
+    case KIND:                                                                                           \
+        if (!launch_split<CFG, R, KAHAN, MID>(grid.x * (block.x / 64), s, rq, sb, se, partial, pstride)) \
             k_plain<CFG, R, KAHAN, MID><<<grid, block, 0, s>>>(rq, sb, se, partial, pstride);            \
         return true;
 #define SP_CASE_PLAIN(KIND, CFG, R, KAHAN, MID)                                                        \
