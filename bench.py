@@ -375,7 +375,8 @@ def neel_relaxation(ctx, configs):
     return dict(workload="Neel uniaxial EulerMaruyama, 4096 particles x 199,900 steps, dt 1e-12 s, zero field",
                 fp64=ps / t64, mixed=ps / tm, unit="particle-steps/s", cpu_baseline=cpu, fp64_vs_cpu=ps / t64 / cpu["value"],
                 err_max=float(np.abs(d).max()), err_rms=float(np.sqrt(np.mean(d ** 2))),
-                note="the GPU is latency bound here: 4,096 particles are 64 waves on 1,024 SIMDs")
+                note="the GPU is latency bound here: 4,096 particles are 64 tiles on 1,024 SIMDs, each run as an integrator "
+                     "wave and two noise waves (split noise)")
 
 
 def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
