@@ -602,7 +602,11 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
     static constexpr int WAVES_PER_SIMD = ANISO == 4 ? 2 : (sizeof(R) == 8 ? 3 : 4);
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = SphLane<ANISO, R>;
-    using Cache = Trig<R>;          // trig of the current (unrotated) state
+    // the trig of the state in its chart; rot: the state is in the rotated chart.  A state is kept in the rotated
+    // chart from step to step while the reference would rotate it at every step (prepareCalculations maps it in,
+    // finishCalculations maps it back, NeelRelaxationSpherical.h:440-471, 568-602); those two maps cancel up to
+    // rounding, so they are applied only where the chart changes and at the end of every output window.
+    struct Cache { Trig<R> T; bool rot; };
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) {
         L.a = (R)ld(rq, P_A, p); L.b = (R)ld(rq, P_B, p);
         L.dr = (R)(-0.5 * ld(rq, P_D, p));
@@ -611,49 +615,59 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
         load_aniso<ANISO, R>(rq, p, L.A);
         L.w = ld(rq, P_MU, p);
     }
-    __device__ static void init_cache(const Lane&, const double* y, Cache& T) { trig_of(y[0], y[1], T); }
+    __device__ static void init_cache(const Lane&, const double* y, Cache& c) { trig_of(y[0], y[1], c.T); c.rot = false; }
     __device__ static void weights(const Lane& L, double w[3]) { w[0] = w[1] = w[2] = L.w; }
-    __device__ static void output(const Lane&, const double*, const Cache& T, double o[3]) {
-        o[0] = (double)(T.st * T.cp); o[1] = (double)(T.st * T.sp); o[2] = (double)T.ct;
+    // m in the lab frame from either chart (NeelRelaxationSpherical.h:499-516 after the map back)
+    __device__ static void output(const Lane&, const double*, const Cache& c, double o[3]) {
+        const Trig<R>& T = c.T;
+        if (!c.rot) { o[0] = (double)(T.st * T.cp); o[1] = (double)(T.st * T.sp); o[2] = (double)T.ct; }
+        else { o[0] = (double)(-T.ct); o[1] = (double)(T.st * T.sp); o[2] = (double)(T.st * T.cp); }
     }
-    // prepareCalculations: rotate 90 degrees about y when theta leaves [tmin, tmax] (:568-580)
-    __device__ __forceinline__ static bool prepare(const DevRequest& rq, const double* y, double Y[2], Trig<R>& T) {
-        const bool rot = (y[0] < rq.tmin) || (y[0] > rq.tmax);
-        Y[0] = y[0]; Y[1] = y[1];
-        if (rot) {
-            double st, ct, sp, cp;
-            if constexpr (sizeof(R) == 8) { st = T.st; ct = T.ct; sp = T.sp; cp = T.cp; }   // the cached trig of y
-            else { sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp); }
-            Trig<double> Td;
-            chart_map<sizeof(R) == 4>(-cp * st, sp * st, ct, Y, Td);       // Y = (acos(-cp st), atan2(sp st, ct)) and its trig
-            T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
-        }
-        return rot;
+    // the FP64 sines and cosines of the state (the cached ones in FP64, evaluated again from y in the mixed mode)
+    __device__ __forceinline__ static void trig64(const double* y, const Cache& c, double& st, double& ct, double& sp, double& cp) {
+        if constexpr (sizeof(R) == 8) { st = c.T.st; ct = c.T.ct; sp = c.T.sp; cp = c.T.cp; }
+        else { sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp); }
     }
-    // finishCalculations (:440-471), then the trig of the new state.  After a rotated step the trig is evaluated
-    // from y itself, as prepareCalculations evaluates it from the state (:145-146) and as a launch starting from
-    // y does (init_cache): a run then gives the same bits however its steps are split into launches.
-    __device__ __forceinline__ static void finish(const double r[2], bool rot, double* y, Trig<R>& T) {
-        if (rot) {
-            double st, ct, sp, cp;
-            sincos_fast(r[0], &st, &ct); sincos_fast(r[1], &sp, &cp);
-            Trig<double> Td;
-            chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);       // y = (acos(cp st), atan2(sp st, -ct))
-            trig_of(y[0], y[1], T);
+    // the chart of this step, as prepareCalculations chooses it (:568-580): rotated (90 degrees about y) when the
+    // polar angle of the state leaves [tmin, tmax].  A state in the rotated chart has the polar angle
+    // acos(cos phi' sin theta') of the map back (:440-471), compared here through its cosine.
+    __device__ __forceinline__ static void to_chart(const DevRequest& rq, double* y, Cache& c) {
+        bool need;
+        if (!c.rot) {
+            need = (y[0] < rq.tmin) || (y[0] > rq.tmax);
         } else {
-            y[0] = fmod_2pi(r[0]);
-            y[1] = fmod_2pi(r[1]);
-            trig_of(y[0], y[1], T);
+            const double cth = (double)c.T.cp * (double)c.T.st;
+            need = cth > rq.ctmin || cth < rq.ctmax;
         }
+        if (need == c.rot) return;
+        double st, ct, sp, cp;
+        trig64(y, c, st, ct, sp, cp);
+        Trig<double> Td;
+        if (need) chart_map<sizeof(R) == 4>(-cp * st, sp * st, ct, y, Td);    // (acos(-cp st), atan2(sp st, ct))
+        else chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);         // (acos(cp st), atan2(sp st, -ct))
+        c.T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp};
+        c.rot = need;
     }
-    __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& T, const R* z, V3<double> hd) {
+    // end of an output window: a state in the rotated chart is mapped back (finishCalculations, :440-471) and the
+    // trig of the standard state is evaluated from it, as a launch that starts at this window does (init_cache), so
+    // a run gives the same bits however its steps are split into launches (launches start and end on window ends)
+    __device__ __forceinline__ static void window_end(const Lane&, double* y, Cache& c) {
+        if (!c.rot) return;
+        double st, ct, sp, cp;
+        trig64(y, c, st, ct, sp, cp);
+        Trig<double> Td;
+        chart_map<sizeof(R) == 4>(cp * st, sp * st, -ct, y, Td);
+        trig_of(y[0], y[1], c.T);
+        c.rot = false;
+    }
+    __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& c, const R* z, V3<double> hd) {
         const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
         const V3<R> zz = {z[0], z[1], z[2]};
-        double Y[2];
-        Trig<R> T0 = T;
-        const bool rot = prepare(rq, y, Y, T0);
+        to_chart(rq, y, c);
+        const bool rot = c.rot;
+        const double Y[2] = {y[0], y[1]};
         Chart<R> c0; V3<R> det0;
-        make_chart<ANISO>(L, T0, rot, h, c0, det0);
+        make_chart<ANISO>(L, c.T, rot, h, c0, det0);
         const R s0 = L.sg * dot(c0.P0, zz), s1 = L.sg * dot(c0.P1, zz);
         double r[2];
         if (SOLVER == S_EM) {                         // EulerMaruyama.inl:41-72
@@ -682,7 +696,11 @@ template <int ANISO, int SOLVER, class R> struct SphCfg {
             r[0] = Y[0] + (double)(det0.x * L.dt + s0 + c0.drift0 * dWdW);
             r[1] = Y[1] + (double)(det0.y * L.dt + s1);
         }
-        finish(r, rot, y, T);
+        // the new state in the chart of the step (the rotated one in FP64 trig, as the step's stages)
+        y[0] = fmod_2pi(r[0]);
+        y[1] = fmod_2pi(r[1]);
+        if (rot) { Trig<double> Td; trig_of(y[0], y[1], Td); c.T = {(R)Td.st, (R)Td.ct, (R)Td.sp, (R)Td.cp}; }
+        else trig_of(y[0], y[1], c.T);
     }
 };
 
@@ -1016,6 +1034,9 @@ __device__ __forceinline__ void normalize_blocks(double* y) {
 }
 
 template <bool B> struct BoolTag { static constexpr bool value = B; };
+// whether a kernel class has work to do at the end of every output window (Cfg::window_end)
+template <class Cfg, class = void> struct has_window_end : std::false_type {};
+template <class Cfg> struct has_window_end<Cfg, std::void_t<decltype(&Cfg::window_end)>> : std::true_type {};
 
 // ================================================================ split noise
 // A tile of a small launch (one that leaves most SIMDs idle) runs as two waves of one work-group
@@ -1158,6 +1179,7 @@ __device__ __forceinline__ void run_tile(const DevRequest& rq, const TileWork& t
 #pragma unroll
         for (int c = 0; c < DOUT; ++c) { mean[c] = acc[c] / (double)OS; acc[c] = 0.0; comp[c] = 0.0; }
         flush(mean);
+        if constexpr (has_window_end<Cfg>::value) Cfg::window_end(L, y, cache);
     };
     // Fast path (Philox noise, windows ending on group boundaries): each group is straight-line code, the
     // next group's Philox + Box-Muller is issued alongside this group's physics, and the per-step

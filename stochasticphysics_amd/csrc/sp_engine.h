@@ -270,6 +270,8 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     const bool rot = (r.flags & SP_FLAG_COORD_TRANSFORM) != 0;
     dr.tmin = rot ? r.min_angle : -1e300;   // no chart rotation: theta may leave [0, pi] freely
     dr.tmax = rot ? (3.14159265358979323846 - r.min_angle) : 1e300;
+    dr.ctmin = rot ? std::cos(dr.tmin) : 2.0;     // with no rotation no state is ever in the rotated chart
+    dr.ctmax = rot ? std::cos(dr.tmax) : -2.0;
     const bool erot = (r.flags & SP_FLAG_EULER_TRANSFORM) != 0;     // BrownAndNeelEulerSpherical Brown chart
     dr.bmin = erot ? r.brown_min_angle : -1e300;
     dr.bmax = erot ? (3.14159265358979323846 - r.brown_min_angle) : 1e300;

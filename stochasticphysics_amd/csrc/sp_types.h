@@ -33,6 +33,7 @@ enum ParamIndex : int {
 struct DevRequest {
     double dt, sqdt;
     double tmin, tmax;             // NeelSpherical chart rotation window (rotation when theta outside)
+    double ctmin, ctmax;           // cos(tmin), cos(tmax) (a state in the rotated chart is tested through cos theta)
     double bmin, bmax;             // BrownAndNeelEulerSpherical: Euler chart window of the body angle theta
     double newton_tol;
     FieldParams field;             // applied field (sp_field.h)
