@@ -240,7 +240,9 @@ __device__ __forceinline__ void bnes_noise(const BNESLane<ANISO, R>& L, const BC
 }
 
 // EulertoEulerRotated (:859-877), EulerRotatedtoEuler (:880-898), the 90 degree chart maps of the Neel angles
-// (:823-834, :844-855); all in double, with the table-driven atan2 and acos
+// (:823-834, :844-855); all in double, with the table-driven atan2 and acos.  The rotated charts are those of the
+// body and of the moment turned by 90 degrees about y: Euler angles (phi', theta', psi') of the rotated chart give
+// the rotation matrix Ry(90) A(phi, theta, psi), Neel angles of the rotated chart the moment Ry(90) m.
 __device__ __forceinline__ void euler_to_rotated(double S0, double C0, double S1, double C1, double S2, double C2, double* r) {
     r[0] = atan2_fast(-C2 * C0 + C1 * S0 * S2, S0 * C2 + C1 * C0 * S2);
     r[1] = acos_fast(-S1 * S2);
@@ -251,12 +253,15 @@ __device__ __forceinline__ void euler_to_rotated(const double* y, double* r) {
     sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
     euler_to_rotated(S0, C0, S1, C1, S2, C2, r);
 }
-__device__ __forceinline__ void rotated_to_euler(const double* y, double* r) {
-    double S0, C0, S1, C1, S2, C2;
-    sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
+__device__ __forceinline__ void rotated_to_euler(double S0, double C0, double S1, double C1, double S2, double C2, double* r) {
     r[0] = atan2_fast(C0 * C2 - C1 * S0 * S2, -C2 * S0 - C1 * C0 * S2);
     r[1] = acos_fast(S1 * S2);
     r[2] = atan2_fast(-C1, C2 * S1);
+}
+__device__ __forceinline__ void rotated_to_euler(const double* y, double* r) {
+    double S0, C0, S1, C1, S2, C2;
+    sincos_fast(y[0], &S0, &C0); sincos_fast(y[1], &S1, &C1); sincos_fast(y[2], &S2, &C2);
+    rotated_to_euler(S0, C0, S1, C1, S2, C2, r);
 }
 __device__ __forceinline__ void neel_to_rotated(double st, double ct, double sp, double cp, double* r) {
     r[0] = acos_fast(-cp * st);
@@ -267,66 +272,138 @@ __device__ __forceinline__ void neel_to_rotated(const double* y, double* r) {
     sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
     neel_to_rotated(st, ct, sp, cp, r);
 }
+__device__ __forceinline__ void rotated_to_neel(double st, double ct, double sp, double cp, double* r) {
+    r[0] = acos_fast(cp * st);
+    r[1] = atan2_fast(sp * st, -ct);
+}
 __device__ __forceinline__ void rotated_to_neel(const double* y, double* r) {
     double st, ct, sp, cp;
     sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
-    r[0] = acos_fast(cp * st);
-    r[1] = atan2_fast(sp * st, -ct);
+    rotated_to_neel(st, ct, sp, cp, r);
+}
+// the sines and cosines of the angles [I0, I1) as the step computes them in the standard charts: the table-driven
+// FP64 sin/cos, or in the mixed mode the hardware FP32 ones after a reduction to one turn in double (as trig5)
+template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<double>& T) { trig_exact<I0, I1>(y, T); }
+template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<float>& T) {
+#pragma unroll
+    for (int i = I0; i < I1; ++i) {
+        double r = y[i] * INV_TWO_PI;
+        r -= floor(r);
+        const float f = (float)r;
+        T.s[i] = __builtin_amdgcn_sinf(f);
+        T.c[i] = __builtin_amdgcn_cosf(f);
+    }
 }
 
 template <int ANISO, int SOLVER, class R> struct BNESCfg {
     static constexpr int DY = 5, DW = 6, DOUT = 9, NORM = 0;
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNESLane<ANISO, R>;
-    using Cache = Trig5<R>;      // trig of the current state (both charts unrotated)
+    // the trig of the state in its charts; brot, nrot: the Euler angles, the Neel angles are in their rotated charts.
+    // A block is kept in its rotated chart from step to step while the reference would rotate it at every step
+    // (prepareCalculations maps it in, finishCalculations maps it back, :142-169, :635-665); those two maps cancel
+    // up to rounding, so they are applied only where a chart changes and at the end of every output window.
+    struct Cache { Trig5<R> T; bool brot, nrot; };
     __device__ static void load(const DevRequest& rq, uint64_t p, Lane& L) { bnes_load<ANISO, R>(rq, p, L); }
-    __device__ static void init_cache(const Lane&, const double* y, Cache& T) { trig5(y, T); }
+    __device__ static void init_cache(const Lane&, const double* y, Cache& c) { trig5(y, c.T); c.brot = c.nrot = false; }
     __device__ static void weights(const Lane& L, double w[9]) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) w[i] = L.w;                     // getWeighting (:755-759)
     }
-    // calculateOutputResult (:674-735): body x axis, body y axis, magnetisation direction
-    __device__ static void output(const Lane&, const double*, const Cache& T, double o[9]) {
+    // calculateOutputResult (:674-735): body x axis, body y axis, magnetisation direction, in the lab frame from
+    // either chart (a vector v' of the rotated chart is Ry(90) v: v = (-v'_z, v'_y, v'_x))
+    __device__ static void output(const Lane&, const double*, const Cache& c, double o[9]) {
+        const Trig5<R>& T = c.T;
         const R cphi = T.c[0], ctheta = T.c[1], cpsi = T.c[2], sphi = T.s[0], stheta = T.s[1], spsi = T.s[2];
         const R cphicpsi = cphi * cpsi, sphicpsi = sphi * cpsi, cphispsi = cphi * spsi, sphispsi = sphi * spsi;
-        o[0] = (double)(cphicpsi - ctheta * sphispsi); o[1] = (double)(ctheta * sphicpsi + cphispsi); o[2] = (double)(stheta * sphi);
-        o[3] = (double)(-sphicpsi - ctheta * cphispsi); o[4] = (double)(ctheta * cphicpsi - sphispsi); o[5] = (double)(stheta * cphi);
-        o[6] = (double)(T.s[3] * T.c[4]); o[7] = (double)(T.s[3] * T.s[4]); o[8] = (double)T.c[3];
-    }
-    // prepareCalculations (:142-169): |theta| of each block against its window (:770-814)
-    __device__ __forceinline__ static void prepare(const DevRequest& rq, const double* y, double Y[5], Trig5<R>& T,
-                                                   bool& brot, bool& nrot) {
-        brot = fabs(y[1]) < rq.bmin || fabs(y[1]) > rq.bmax;
-        nrot = fabs(y[3]) < rq.tmin || fabs(y[3]) > rq.tmax;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) Y[i] = y[i];
-        // in FP64 the cache holds exactly the sines and cosines of y the chart maps need
-        if (brot) {
-            if constexpr (sizeof(R) == 8) euler_to_rotated(T.s[0], T.c[0], T.s[1], T.c[1], T.s[2], T.c[2], Y);
-            else euler_to_rotated(y, Y);
-            trig_exact<0, 3>(Y, T);
+        const R ex[3] = {cphicpsi - ctheta * sphispsi, ctheta * sphicpsi + cphispsi, stheta * sphi};
+        const R ey[3] = {-sphicpsi - ctheta * cphispsi, ctheta * cphicpsi - sphispsi, stheta * cphi};
+        const R m[3] = {T.s[3] * T.c[4], T.s[3] * T.s[4], T.c[3]};
+        if (!c.brot) {
+            o[0] = (double)ex[0]; o[1] = (double)ex[1]; o[2] = (double)ex[2];
+            o[3] = (double)ey[0]; o[4] = (double)ey[1]; o[5] = (double)ey[2];
+        } else {
+            o[0] = (double)(-ex[2]); o[1] = (double)ex[1]; o[2] = (double)ex[0];
+            o[3] = (double)(-ey[2]); o[4] = (double)ey[1]; o[5] = (double)ey[0];
         }
-        if (nrot) {
-            if constexpr (sizeof(R) == 8) neel_to_rotated(T.s[3], T.c[3], T.s[4], T.c[4], Y + 3);
-            else neel_to_rotated(y + 3, Y + 3);
-            trig_exact<3, 5>(Y, T);
+        if (!c.nrot) { o[6] = (double)m[0]; o[7] = (double)m[1]; o[8] = (double)m[2]; }
+        else { o[6] = (double)(-m[2]); o[7] = (double)m[1]; o[8] = (double)m[0]; }
+    }
+    // the charts of this step, as prepareCalculations chooses them (:142-169, windows :770-814): a block is rotated
+    // when |theta| of its standard chart leaves its window.  A block in its rotated chart has the standard theta of
+    // the map back, acos(sin theta' sin psi') for the body (:880-898) and acos(cos phi' sin theta') for the moment
+    // (:844-855), compared here through its cosine.  In FP64 the cache holds exactly the sines and cosines of y
+    // the chart maps need; the mixed mode evaluates them again in FP64.
+    __device__ __forceinline__ static void to_chart(const DevRequest& rq, double* y, Cache& c) {
+        bool bneed, nneed;
+        if (!c.brot) bneed = fabs(y[1]) < rq.bmin || fabs(y[1]) > rq.bmax;
+        else {
+            const double x = (double)c.T.s[1] * (double)c.T.s[2];
+            bneed = x > rq.bcmin || x < rq.bcmax;
+        }
+        if (!c.nrot) nneed = fabs(y[3]) < rq.tmin || fabs(y[3]) > rq.tmax;
+        else {
+            const double x = (double)c.T.c[4] * (double)c.T.s[3];
+            nneed = x > rq.ctmin || x < rq.ctmax;
+        }
+        if (bneed != c.brot) {
+            double Y[3];
+            if constexpr (sizeof(R) == 8) {
+                if (bneed) euler_to_rotated(c.T.s[0], c.T.c[0], c.T.s[1], c.T.c[1], c.T.s[2], c.T.c[2], Y);
+                else rotated_to_euler(c.T.s[0], c.T.c[0], c.T.s[1], c.T.c[1], c.T.s[2], c.T.c[2], Y);
+            } else {
+                if (bneed) euler_to_rotated(y, Y);
+                else rotated_to_euler(y, Y);
+            }
+            y[0] = Y[0]; y[1] = Y[1]; y[2] = Y[2];
+            if (bneed) trig_exact<0, 3>(y, c.T);
+            else trig_std<0, 3>(y, c.T);
+            c.brot = bneed;
+        }
+        if (nneed != c.nrot) {
+            double Y[2];
+            if constexpr (sizeof(R) == 8) {
+                if (nneed) neel_to_rotated(c.T.s[3], c.T.c[3], c.T.s[4], c.T.c[4], Y);
+                else rotated_to_neel(c.T.s[3], c.T.c[3], c.T.s[4], c.T.c[4], Y);
+            } else {
+                if (nneed) neel_to_rotated(y + 3, Y);
+                else rotated_to_neel(y + 3, Y);
+            }
+            y[3] = Y[0]; y[4] = Y[1];
+            if (nneed) trig_exact<3, 5>(y, c.T);
+            else trig_std<3, 5>(y, c.T);
+            c.nrot = nneed;
         }
     }
-    // finishCalculations (:635-665), then the trig of the new state
-    __device__ __forceinline__ static void finish(const double r[5], bool brot, bool nrot, double* y, Trig5<R>& T) {
-        if (brot) rotated_to_euler(r, y);
-        else { y[0] = fmod_2pi(r[0]); y[1] = fmod_2pi(r[1]); y[2] = fmod_2pi(r[2]); }
-        if (nrot) rotated_to_neel(r + 3, y + 3);
-        else { y[3] = fmod_2pi(r[3]); y[4] = fmod_2pi(r[4]); }
-        trig5(y, T);
+    // end of an output window: a block in its rotated chart is mapped back (finishCalculations, :635-665) and the
+    // trig of the standard state is evaluated from it, as a launch that starts at this window does (init_cache), so
+    // a run gives the same bits however its steps are split into launches (launches start and end on window ends)
+    __device__ __forceinline__ static void window_end(const Lane&, double* y, Cache& c) {
+        if (!c.brot && !c.nrot) return;
+        if (c.brot) {
+            double Y[3];
+            if constexpr (sizeof(R) == 8) rotated_to_euler(c.T.s[0], c.T.c[0], c.T.s[1], c.T.c[1], c.T.s[2], c.T.c[2], Y);
+            else rotated_to_euler(y, Y);
+            y[0] = Y[0]; y[1] = Y[1]; y[2] = Y[2];
+        }
+        if (c.nrot) {
+            double Y[2];
+            if constexpr (sizeof(R) == 8) rotated_to_neel(c.T.s[3], c.T.c[3], c.T.s[4], c.T.c[4], Y);
+            else rotated_to_neel(y + 3, Y);
+            y[3] = Y[0]; y[4] = Y[1];
+        }
+        trig5(y, c.T);
+        c.brot = c.nrot = false;
     }
-    __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& T, const R* z, V3<double> hd) {
+    __device__ __forceinline__ static void step(const DevRequest& rq, const Lane& L, double* y, Cache& c, const R* z, V3<double> hd) {
         const V3<R> h = {(R)hd.x, (R)hd.y, (R)hd.z};
         const V3<R> zF = {z[0], z[1], z[2]}, zH = {z[3], z[4], z[5]};
+        to_chart(rq, y, c);
+        const bool brot = c.brot, nrot = c.nrot;
         double Y[5];
-        Trig5<R> T0 = T;
-        bool brot, nrot;
-        prepare(rq, y, Y, T0, brot, nrot);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) Y[i] = y[i];
+        const Trig5<R>& T0 = c.T;
         BChart<R> c0;
         bnes_chart(T0, brot, nrot, c0);
         R f0[5], s0[5];
@@ -368,7 +445,13 @@ template <int ANISO, int SOLVER, class R> struct BNESCfg {
             for (int i = 0; i < 4; ++i) r[i] = Y[i] + (double)(f0[i] * L.dt + s0[i] + dr[i] * dWdW);
             r[4] = Y[4] + (double)(f0[4] * L.dt + s0[4]);
         }
-        finish(r, brot, nrot, y, T);
+        // the new state in the charts of the step (a rotated block's trig in FP64, as the step's stages take it)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) y[i] = fmod_2pi(r[i]);
+        if (brot) trig_exact<0, 3>(y, c.T);
+        else trig_std<0, 3>(y, c.T);
+        if (nrot) trig_exact<3, 5>(y, c.T);
+        else trig_std<3, 5>(y, c.T);
     }
 };
 

@@ -275,6 +275,8 @@ inline DevRequest make_dev_request(const HostRequest& hr, const ReqDev& d) {
     const bool erot = (r.flags & SP_FLAG_EULER_TRANSFORM) != 0;     // BrownAndNeelEulerSpherical Brown chart
     dr.bmin = erot ? r.brown_min_angle : -1e300;
     dr.bmax = erot ? (3.14159265358979323846 - r.brown_min_angle) : 1e300;
+    dr.bcmin = erot ? std::cos(dr.bmin) : 2.0;    // with no rotation no body is ever in the rotated chart
+    dr.bcmax = erot ? std::cos(dr.bmax) : -2.0;
     dr.newton_tol = r.newton_tol;
     dr.field = field_params(r);
     dr.n_particles = hr.N;

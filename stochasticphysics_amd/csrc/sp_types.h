@@ -35,6 +35,7 @@ struct DevRequest {
     double tmin, tmax;             // NeelSpherical chart rotation window (rotation when theta outside)
     double ctmin, ctmax;           // cos(tmin), cos(tmax) (a state in the rotated chart is tested through cos theta)
     double bmin, bmax;             // BrownAndNeelEulerSpherical: Euler chart window of the body angle theta
+    double bcmin, bcmax;           // cos(bmin), cos(bmax) (a body in the rotated chart is tested through cos theta)
     double newton_tol;
     FieldParams field;             // applied field (sp_field.h)
     uint64_t n_particles;

@@ -148,12 +148,19 @@ def test_synchronous_run_while_serving(ctx):
         same(p, ctx.result(rid, e))
 
 
+@pytest.mark.parametrize("problem", ["NeelSpherical", "BrownAndNeelEulerSpherical"])
 @pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_chunk_boundaries_do_not_change_results(ctx, solver, precision):
-    """A launch restarts a tile from its stored state: NeelSpherical with the chart rotation at pi/4 (a third of
-    the steps rotated) run in launches of 100 steps and of 4,000 steps gives the same bits."""
-    e = sph_ensemble(n=256, n_steps=4000, solver=solver, precision=precision, seed=81)
+def test_chunk_boundaries_do_not_change_results(ctx, problem, solver, precision):
+    """A launch restarts a tile from its stored state, and a state kept in a rotated chart within an output window
+    is mapped back at every window end: NeelSpherical with the chart rotation at pi/4 (a third of the steps
+    rotated), BrownAndNeelEulerSpherical with both charts at pi/4, run in launches of 100 steps and of 4,000 steps
+    give the same bits."""
+    if problem == "NeelSpherical":
+        e = sph_ensemble(n=256, n_steps=4000, solver=solver, precision=precision, seed=81)
+    else:
+        e = bnes_ensemble(n=256, n_steps=4000, solver=solver, precision=precision, seed=81)
+        e.brown_min_angle = math.pi / 4
     e.min_angle = math.pi / 4
     e.skip_check = True
     out = []

@@ -217,6 +217,16 @@ def test_spherical_parity(ctx, solver, aniso):
     compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10, spherical=True)
 
 
+@pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein"])
+def test_spherical_parity_wide_window(ctx, solver):
+    """Chart rotation whenever theta is within pi/4 of a pole (about a third of the steps): the kernels keep a
+    rotated state in its chart for the rest of its output window, where the reference maps it back and in again
+    at every step (NeelRelaxationSpherical.h:440-471, 568-602); the two agree up to rounding."""
+    e = sph_ensemble(solver=solver)
+    e.min_angle = math.pi / 4
+    compare(ctx.run(e, keep_final=True), run_oracle(e), 1e-10, spherical=True)
+
+
 def test_spherical_no_rotation(ctx):
     """Without chart rotation theta is never folded back: particles pass through the poles and theta leaves
     [0, pi] (here one reaches theta < -1).  Near a pole the reference's 1/sin(theta) terms amplify rounding
@@ -402,6 +412,16 @@ def test_bnes_parity(ctx, solver, aniso):
     well: the kernels' table-driven sin/cos (which equals the C library's on 98% of arguments) put into the oracle
     in place of the C library's moves the cubic Heun_Strong case by 7e-11 on the CPU, hence 3e-10 here."""
     e = bnes_ensemble(solver=solver, aniso=aniso)
+    compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 3e-10)
+
+
+@pytest.mark.parametrize("solver", ["Heun_Strong", "Millstein"])
+def test_bnes_parity_wide_windows(ctx, solver):
+    """Both charts rotate whenever their theta is within pi/4 of a pole: the kernels keep a rotated block in its
+    chart for the rest of its output window, where the reference maps it back and in again at every step
+    (BrownAndNeelRelaxationEulerSpherical.h:142-169, 635-665); the two agree up to rounding."""
+    e = bnes_ensemble(solver=solver)
+    e.min_angle = e.brown_min_angle = math.pi / 4
     compare_bnes(ctx.run(e, keep_final=True), run_oracle(e), 3e-10)
 
 
