@@ -287,11 +287,11 @@ def serving_leg(ctx, configs, seconds, rate, seed=55):
 
 def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55, sat_seconds=8.0):
     """The serving configuration (Poisson arrivals at `rate` for `seconds`); the server's capacity, measured as the
-    served particle-steps/s of a run whose arrivals (3x `rate`, `sat_seconds`) outpace the device; then the same
+    served particle-steps/s of a run whose arrivals (8x `rate`, `sat_seconds`) outpace the device; then the same
     request distribution at 50%, 80% and 95% of that capacity for `low_seconds` each: p50/p99 latency against
     offered load."""
     main = serving_leg(ctx, configs, seconds, rate, seed)
-    sat = serving_leg(ctx, configs, sat_seconds, 3.0 * rate, seed + 10)
+    sat = serving_leg(ctx, configs, sat_seconds, 8.0 * rate, seed + 10)
     capacity = sat["throughput"]
     per_request = main["offered_particle_steps_per_s"] / rate
     main["offered_fraction_of_capacity"] = main["offered_particle_steps_per_s"] / capacity
@@ -303,7 +303,7 @@ def serving_study(ctx, configs, seconds, rate, low_seconds, seed=55, sat_seconds
                                               "latency_by_solver")})
         loads[-1]["offered_fraction_of_capacity"] = r["offered_particle_steps_per_s"] / capacity
     main.update(capacity_particle_steps_per_s=capacity,
-                capacity_run=dict(offered_req_per_s=3.0 * rate, seconds=sat_seconds, requests=sat["requests"],
+                capacity_run=dict(offered_req_per_s=8.0 * rate, seconds=sat_seconds, requests=sat["requests"],
                                   p50_ms=sat["p50_ms"], p99_ms=sat["p99_ms"]),
                 other_loads=loads,
                 note="device_busy_frac sums the device time of the server's two lanes (explicit solvers, "

@@ -227,7 +227,8 @@ int sp_ctx_create(int device, sp_ctx** out);
 void sp_ctx_destroy(sp_ctx* ctx);
 int sp_ctx_set_tuning(sp_ctx* ctx, int problem, int solver, int precision, const sp_tuning* t);
 int sp_ctx_get_tuning(sp_ctx* ctx, int problem, int solver, int precision, sp_tuning* t);
-/* device memory budget for admitted requests (state + params + outputs) */
+/* device memory budget for admitted requests (state + params + outputs; default half of the free device memory)
+ * and the most requests admitted at once (default 512) */
 int sp_ctx_set_memory_budget(sp_ctx* ctx, uint64_t bytes, uint32_t max_active);
 
 /* plain path (sp_run, sp_execute): the largest partial-sum buffer used to reduce a whole pass at once

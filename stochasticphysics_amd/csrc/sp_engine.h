@@ -192,6 +192,7 @@ inline int prepare_provided(const sp_request* rq, std::shared_ptr<const ProvSpec
 }
 
 // ------------------------------------------------------------------ device side of one request
+// All buffers of a request are carved from one device block (256-byte aligned pieces).
 struct ReqDev {
     double* params = nullptr;
     double* state = nullptr;
@@ -201,7 +202,39 @@ struct ReqDev {
     unsigned long long* newton_fail = nullptr;
     DevRequest* dreq = nullptr;   // device copy of the request descriptor (plain path)
     double* traj = nullptr;       // [n_out][dout][n_saved] single-simulation outputs
+    size_t bytes = 0;             // the request's footprint (admission accounting)
+    void* block = nullptr;        // the device block holding all of the above
+    size_t block_bytes = 0;
+};
+
+// Device blocks of finished served requests, kept for the requests that follow: hipFree waits for the work of every
+// stream of the device (an iteration in flight on the server's other lane included) and each hipMalloc costs tens
+// of microseconds, so the server worker takes blocks from here and gives them back.  A block is given back only
+// when no work on any stream uses it.  At most CAP bytes are kept; used by one thread (the worker).
+struct BlockCache {
+    static constexpr size_t CAP = size_t(1) << 30;
+    std::multimap<size_t, void*> blocks;   // by size
     size_t bytes = 0;
+    // a cached block of at least `need` bytes and at most twice that, or null
+    void* take(size_t need, size_t& got) {
+        auto it = blocks.lower_bound(need);
+        if (it == blocks.end() || it->first > 2 * need) return nullptr;
+        void* p = it->second;
+        got = it->first;
+        bytes -= it->first;
+        blocks.erase(it);
+        return p;
+    }
+    void give(void* p, size_t n) {
+        if (bytes + n > CAP) { (void)hipFree(p); return; }
+        blocks.emplace(n, p);
+        bytes += n;
+    }
+    void clear() {
+        for (auto& b : blocks) (void)hipFree(b.second);
+        blocks.clear();
+        bytes = 0;
+    }
 };
 
 inline size_t traj_bytes(const HostRequest& hr) { return (size_t)hr.n_out * hr.dout * hr.n_saved * 8; }
@@ -216,15 +249,12 @@ inline size_t check_bytes(const HostRequest& hr) {
     return (size_t)hr.n_out * hr.dout * std::max<uint32_t>(1, hr.n_check_tiles) * 8;
 }
 
-inline void free_dev(ReqDev& d) {
-    if (d.params) hipFree(d.params);
-    if (d.state) hipFree(d.state);
-    if (d.replay) hipFree(d.replay);
-    if (d.out) hipFree(d.out);
-    if (d.check) hipFree(d.check);
-    if (d.newton_fail) hipFree(d.newton_fail);
-    if (d.dreq) hipFree(d.dreq);
-    if (d.traj) hipFree(d.traj);
+// the request's block back to `cache`, or freed (which waits for the device) without one
+inline void free_dev(ReqDev& d, BlockCache* cache = nullptr) {
+    if (d.block) {
+        if (cache) cache->give(d.block, d.block_bytes);
+        else (void)hipFree(d.block);
+    }
     d = ReqDev{};
 }
 
@@ -232,22 +262,42 @@ int run_provider(HostRequest& hr, ReqDev& d, hipStream_t s);   // sp_runtime.cpp
 
 // device buffers of a request; its parameters and initial state are uploaded, or drawn on the device by
 // the provider kernel (which also fills the weight and property sums of hr)
-inline int alloc_dev(HostRequest& hr, ReqDev& d, hipStream_t s) {
-    auto fail = [&](const char* what, hipError_t e) {
-        (void)hipGetLastError();      // clear the failed allocation's error
-        free_dev(d);
-        return set_error(e == hipErrorOutOfMemory ? SP_ERR_OUT_OF_MEMORY : SP_ERR_DEVICE,
-                         std::string("allocating ") + what + ": " + hipGetErrorString(e));
-    };
-    hipError_t e;
-    if ((e = hipMalloc(&d.params, (size_t)NPARAM * hr.N * 8)) != hipSuccess) return fail("params", e);
-    if ((e = hipMalloc(&d.state, (size_t)hr.dy * hr.N * 8)) != hipSuccess) return fail("state", e);
-    if (!hr.replay.empty() && (e = hipMalloc(&d.replay, hr.replay.size() * 8)) != hipSuccess) return fail("replay", e);
-    if ((e = hipMalloc(&d.out, hr.n_out * hr.dout * 8)) != hipSuccess) return fail("out", e);
-    if ((e = hipMalloc(&d.check, check_bytes(hr))) != hipSuccess) return fail("check", e);
-    if ((e = hipMalloc(&d.newton_fail, 8)) != hipSuccess) return fail("counter", e);
-    if ((e = hipMalloc(&d.dreq, sizeof(DevRequest))) != hipSuccess) return fail("descriptor", e);
-    if (traj_bytes(hr) && (e = hipMalloc(&d.traj, traj_bytes(hr))) != hipSuccess) return fail("trajectories", e);
+// (from `cache` when it holds a block of about the size; a failed allocation empties the cache and tries again)
+inline int alloc_dev(HostRequest& hr, ReqDev& d, hipStream_t s, BlockCache* cache = nullptr) {
+    size_t off = 0;
+    auto piece = [&](size_t n) { const size_t o = off; off += (n + 255) / 256 * 256; return o; };
+    const size_t o_params = piece((size_t)NPARAM * hr.N * 8), o_state = piece((size_t)hr.dy * hr.N * 8);
+    const size_t o_replay = piece(hr.replay.size() * 8), o_out = piece(hr.n_out * hr.dout * 8);
+    const size_t o_check = piece(check_bytes(hr)), o_nf = piece(8), o_dreq = piece(sizeof(DevRequest));
+    const size_t o_traj = piece(traj_bytes(hr));
+    size_t got = 0;
+    void* blk = cache ? cache->take(off, got) : nullptr;
+    if (!blk) {
+        hipError_t e = hipMalloc(&blk, off);
+        if (e != hipSuccess && cache && !cache->blocks.empty()) {
+            (void)hipGetLastError();
+            cache->clear();
+            e = hipMalloc(&blk, off);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();      // clear the failed allocation's error
+            d = ReqDev{};
+            return set_error(e == hipErrorOutOfMemory ? SP_ERR_OUT_OF_MEMORY : SP_ERR_DEVICE,
+                             std::string("allocating request buffers: ") + hipGetErrorString(e));
+        }
+        got = off;
+    }
+    char* b = (char*)blk;
+    d.block = blk;
+    d.block_bytes = got;
+    d.params = (double*)(b + o_params);
+    d.state = (double*)(b + o_state);
+    d.replay = hr.replay.empty() ? nullptr : (double*)(b + o_replay);
+    d.out = (double*)(b + o_out);
+    d.check = (double*)(b + o_check);
+    d.newton_fail = (unsigned long long*)(b + o_nf);
+    d.dreq = (DevRequest*)(b + o_dreq);
+    d.traj = traj_bytes(hr) ? (double*)(b + o_traj) : nullptr;
     d.bytes = request_bytes(hr);
     if (hr.prov) {
         if (int e2 = run_provider(hr, d, s)) { free_dev(d); return e2; }
@@ -488,7 +538,7 @@ struct sp_ctx {
     std::mutex mu;                          // protects everything below
     std::map<uint64_t, sp_tuning> tuning;   // by (problem, solver, precision)
     uint64_t mem_budget = 0;
-    uint32_t max_active = 64;
+    uint32_t max_active = 512;              // admitted requests at once (512 small requests fill the device)
     // server
     std::thread worker;
     bool worker_started = false;
@@ -506,6 +556,7 @@ struct sp_ctx {
     std::mutex run_mu;                      // serialises device use between sp_run and the worker
     std::atomic<int> run_waiters{0};        // synchronous calls waiting for run_mu (sp::RunLock)
     sp::IterRes res;
+    sp::BlockCache blocks;                  // request blocks for reuse (the server worker's own)
     // plain path: largest partial-sum buffer for reducing a whole pass at once (larger passes are reduced
     // chunk by chunk)
     uint64_t whole_pass_limit = 4ull << 30;

@@ -536,6 +536,7 @@ struct Lane {
     IterPlan P;
     std::vector<Slot*> slots;         // the in-flight iteration's slots (their DevRequest table positions)
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    BlockCache* cache = nullptr;      // where finished slots' blocks go
     bool has_work() const {
         for (auto& a : active) if (!a->done) return true;
         return false;
@@ -643,7 +644,7 @@ static void release_done(Lane& L) {
     std::vector<std::unique_ptr<Slot>> keep;
     for (auto& a : L.active) {
         const bool done_pair = a->done && (a->is_shadow ? (a->parent == nullptr || a->parent->done) : (!a->shadow || a->shadow->done));
-        if (done_pair) free_dev(a->dev);
+        if (done_pair) free_dev(a->dev, L.cache);      // no work in flight uses a finished slot
         else keep.push_back(std::move(a));
     }
     // parents point at shadows: clear dangling links of the survivors
@@ -663,6 +664,7 @@ static void worker_main(sp_ctx* ctx) {
     Lane lanes[2];
     lanes[0].R = &ctx->res;
     lanes[1].R = two_lanes ? &own : &ctx->res;
+    lanes[0].cache = lanes[1].cache = &ctx->blocks;
     const int n_lanes = two_lanes ? 2 : 1;
     for (int l = 0; l < n_lanes; ++l) {
         init_class_timing(*lanes[l].R);             // per-class launch timing for the iteration budget
@@ -722,7 +724,7 @@ static void worker_main(sp_ctx* ctx) {
             auto sl = std::make_unique<Slot>();
             sl->job = j;
             sl->hr = &j->hr;
-            int rc = alloc_dev(j->hr, sl->dev, s);
+            int rc = alloc_dev(j->hr, sl->dev, s, &ctx->blocks);
             sp_tuning t;
             {
                 std::lock_guard<std::mutex> lk(ctx->mu);
@@ -736,7 +738,7 @@ static void worker_main(sp_ctx* ctx) {
                 sh->is_shadow = true;
                 sh->parent = sl.get();
                 sh->tuned_chunk = sh->chunk = sl->chunk;
-                rc = alloc_dev(sh->shadow_req, sh->dev, s);
+                rc = alloc_dev(sh->shadow_req, sh->dev, s, &ctx->blocks);
                 sl->shadow = sh.get();
                 if (rc == SP_OK) L.active.push_back(std::move(sh));
             }
@@ -823,6 +825,7 @@ static void worker_main(sp_ctx* ctx) {
         for (auto& a : L.active) free_dev(a->dev);
         L.active.clear();
     }
+    ctx->blocks.clear();
     for (int l = 0; l < n_lanes; ++l) {
         if (lanes[l].e0) hipEventDestroy(lanes[l].e0);
         if (lanes[l].e1) hipEventDestroy(lanes[l].e1);

@@ -426,7 +426,7 @@ class Context:
         keys = ("completed", "active", "pending", "iterations", "busy_ms", "wall_ms", "particle_steps", "fallbacks")
         return dict(zip(keys, s.tolist()))
 
-    def set_memory_budget(self, nbytes, max_active=64):
+    def set_memory_budget(self, nbytes, max_active=512):
         check(lib().sp_ctx_set_memory_budget(self._ctx, int(nbytes), int(max_active)))
 
     def set_reduction_limit(self, nbytes):
