@@ -348,7 +348,8 @@ __device__ __forceinline__ void bn_s(const BNLane<ANISO, R>& L, V3<R> n, V3<R> m
 
 template <int ANISO, int SOLVER, class R> struct BNCfg {
     static constexpr int DY = 6, DW = 6, DOUT = 6, NORM = 2;
-    static constexpr int WAVES_PER_SIMD = 2;     // FP64 (with a one-wave build for small launches) and FP32 physics
+    // FP64 at two waves per SIMD (with a one-wave build for small launches), FP32 physics at three (168 registers)
+    static constexpr int WAVES_PER_SIMD = sizeof(R) == 8 ? 2 : 3;
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
     using Lane = BNLane<ANISO, R>;
     struct Cache {};

@@ -4,9 +4,9 @@
 //   SP_BN_PART == 1: FP64 and ImplicitMidpoint.  The explicit FP64 classes run at two waves per SIMD
 //     (BNCfg::WAVES_PER_SIMD); a plain launch with no more waves than the device has SIMDs takes their one-wave
 //     build instead (OneWave).  Registry functions *_bn64.
-//   SP_BN_PART == 2: mixed precision, also at two waves per SIMD (measured on the 256 BrownAndNeel sets of the
-//     512-set batching distribution: 9.19e10 particle-steps/s against 8.80e10 at four waves with 56-119 registers
-//     spilled); its registry hands the kinds it does not hold to part 1.
+//   SP_BN_PART == 2: mixed precision, at three waves per SIMD (168 registers; measured on the 256 BrownAndNeel
+//     sets of the 512-set batching distribution, autotuned: 9.49e10 particle-steps/s against 9.04e10 at two waves
+//     and 8.80e10 at four with 56-119 registers spilled); its registry hands the kinds it does not hold to part 1.
 #include "sp_kernels_impl.h"
 
 #ifndef SP_BN_PART
