@@ -36,6 +36,7 @@ SOURCES = [
     # the FP64 explicit NeelSpherical classes without machine LICM (see the head of sp_kernels_sph.hip)
     ("sp_kernels_sph.hip", ["-DSP_SPH_PART=1", "-mllvm", "-disable-machine-licm"], "sp_kernels_sph64.o"),
     ("sp_kernels_sph.hip", ["-DSP_SPH_PART=2"], "sp_kernels_sph.o"),
+    ("sp_kernels_sph.hip", ["-DSP_SPH_PART=3"], "sp_kernels_sph1w.o"),
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=1"], "sp_kernels_bnes_a1.o"),
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=2"], "sp_kernels_bnes_a2.o"),
     ("sp_kernels_bnes.hip", ["-DSP_BNES_ANISO=4"], "sp_kernels_bnes_a4.o"),

@@ -50,13 +50,15 @@ def test_batch_equals_plain(ctx):
 
 
 @pytest.mark.parametrize("n", [4096, 25_600, 70_000])
-@pytest.mark.parametrize("case", ["neel_em_fp64", "neel_heun_mixed", "bn_heun_fp64", "bn_heun_mixed"])
+@pytest.mark.parametrize("case", ["neel_em_fp64", "neel_heun_mixed", "bn_heun_fp64", "bn_heun_mixed", "sph_heun_fp64",
+                                  "sph_heun_mixed"])
 def test_plain_variants_equal_batch(ctx, case, n):
     """A plain launch of 64 tiles runs split with two noise waves per tile, of 400 tiles split with one, of 1,094
-    tiles in the usual one-wave-per-tile kernels; the persistent batch kernel never splits.  All give the same bits."""
+    tiles in the usual one-wave-per-tile kernels; NeelSpherical launches of up to one wave per SIMD take the classes'
+    one-wave builds; the persistent batch kernel never splits and takes the regular builds.  All give the same bits."""
     solver = "EulerMaruyama" if case.startswith("neel_em") else "Heun_Strong"
     precision = "mixed" if case.endswith("mixed") else "fp64"
-    mk = neel_ensemble if case.startswith("neel") else bn_ensemble
+    mk = {"neel": neel_ensemble, "bn": bn_ensemble, "sph": sph_ensemble}[case.split("_")[0]]
     e = mk(n=n, n_steps=400, solver=solver, precision=precision, seed=7)
     e.skip_check = True
     same(ctx.run(e), ctx.run_batch([e])[0])
