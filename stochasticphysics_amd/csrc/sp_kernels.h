@@ -42,6 +42,7 @@ bool launch_batch_sph64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, co
 bool info_sph64(uint32_t, KernelInfo*);
 // one-wave builds of the explicit NeelSpherical classes (false for a kind it does not hold)
 bool launch_plain_sph1w(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
+bool launch_batch_sph1w(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);
 // the FP64 and ImplicitMidpoint BrownAndNeel classes, a translation unit of their own (launch_plain_bn hands them on)
 bool launch_plain_bn64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, uint64_t, uint64_t, double*, uint32_t);
 bool launch_batch_bn64(uint32_t, dim3, dim3, hipStream_t, const DevRequest*, const SlotWork*, uint32_t, uint32_t, uint32_t*);

@@ -7,10 +7,10 @@
 //     rematerialised where they are used.  Registry functions *_sph64.
 //   SP_SPH_PART == 2: mixed precision (four waves per SIMD) and ImplicitMidpoint; its registry hands the kinds it
 //     does not hold to part 1.
-//   SP_SPH_PART == 3: one-wave builds (OneWave) of the explicit classes, FP64 and mixed, with machine LICM: a plain
-//     launch with no more waves than the device has SIMDs runs one wave per SIMD anyway, and there the hoisted
-//     constants fit (65,536 particles, Heun_Strong FP64: 3.29e10 particle-steps/s against 3.06e10).  The registry
-//     of part 2 (the problem's entry) hands such launches to launch_plain_sph1w first.
+//   SP_SPH_PART == 3: one-wave builds (OneWave) of the explicit classes, FP64 and mixed, with machine LICM: a launch
+//     of no more tiles than the device has SIMDs runs one wave per SIMD anyway, and there the hoisted constants fit
+//     (65,536 particles, Heun_Strong FP64: 3.28e10 particle-steps/s against 3.06e10).  The registry of part 2 (the
+//     problem's entry) hands such plain and batch launches (served iterations of a few requests) to it first.
 #include "sp_kernels_impl.h"
 
 #ifndef SP_SPH_PART
@@ -63,6 +63,10 @@ bool launch_plain_sph1w(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, con
                         uint64_t se, double* partial, uint32_t pstride) {
     switch (kind) { SP_CLASSES(SP_CASE_PLAIN) default: return false; }
 }
+bool launch_batch_sph1w(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
+                        const SlotWork* slots, uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
+    switch (kind) { SP_CLASSES(SP_CASE_BATCH) default: return false; }
+}
 #else
 #define SP_CLASSES(X)                                                                                       \
     X(kernel_kind(2, S_EM, 1, 1), SphCfg<1 SP_COMMA S_EM SP_COMMA float>, float, false, false)        \
@@ -86,6 +90,9 @@ bool launch_plain_sph(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const
 }
 bool launch_batch_sph(uint32_t kind, dim3 grid, dim3 block, hipStream_t s, const DevRequest* reqs,
                      const SlotWork* slots, uint32_t n_slots, uint32_t n_tiles, uint32_t* counter) {
+    if (n_tiles <= device_simds() &&
+        launch_batch_sph1w(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter))
+        return true;
     switch (kind) {
         SP_CLASSES(SP_CASE_BATCH)
         default: return launch_batch_sph64(kind, grid, block, s, reqs, slots, n_slots, n_tiles, counter);

@@ -54,14 +54,20 @@ def test_batch_equals_plain(ctx):
                                   "sph_heun_mixed"])
 def test_plain_variants_equal_batch(ctx, case, n):
     """A plain launch of 64 tiles runs split with two noise waves per tile, of 400 tiles split with one, of 1,094
-    tiles in the usual one-wave-per-tile kernels; NeelSpherical launches of up to one wave per SIMD take the classes'
-    one-wave builds; the persistent batch kernel never splits and takes the regular builds.  All give the same bits."""
+    tiles in the usual one-wave-per-tile kernels; NeelSpherical launches of at most one tile per SIMD, plain or
+    batched, take the classes' one-wave builds.  The persistent batch kernel never splits; batched beside a
+    70,000-particle ensemble of the same class (more tiles than SIMDs) it takes the regular builds.  All give the
+    same bits."""
     solver = "EulerMaruyama" if case.startswith("neel_em") else "Heun_Strong"
     precision = "mixed" if case.endswith("mixed") else "fp64"
     mk = {"neel": neel_ensemble, "bn": bn_ensemble, "sph": sph_ensemble}[case.split("_")[0]]
     e = mk(n=n, n_steps=400, solver=solver, precision=precision, seed=7)
     e.skip_check = True
-    same(ctx.run(e), ctx.run_batch([e])[0])
+    big = mk(n=70_000, n_steps=400, solver=solver, precision=precision, seed=8)
+    big.skip_check = True
+    plain = ctx.run(e)
+    same(plain, ctx.run_batch([e])[0])
+    same(plain, ctx.run_batch([e, big])[0])
 
 
 def test_batch_graph_and_direct_launch_agree(ctx):
