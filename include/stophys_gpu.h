@@ -194,7 +194,8 @@ typedef struct sp_result {
 
 /* execution knobs chosen by the autotuner (or by hand) */
 typedef struct sp_tuning {
-    uint32_t block_size;       /* work-group size: 64, 128, 192 or 256 */
+    uint32_t block_size;       /* work-group size: 64, 128, 192 or 256 (a launch of fewer 64-particle tiles than the
+                                  device has SIMDs uses 64, one wave per work-group) */
     uint32_t chunk_steps;      /* steps advanced per request per scheduler iteration / launch */
     uint32_t use_graph;        /* capture chunk launches in a hipGraph (static batches) */
     uint32_t persistent_waves_per_simd; /* persistent grid = CUs * 4 * this (1..8) */

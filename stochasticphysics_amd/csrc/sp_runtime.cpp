@@ -80,7 +80,7 @@ int execute_plain(sp_ctx* ctx, Prepared& pr, Outputs& o, const sp_tuning& tun) {
     if (int e = ensure_table(ctx, n_chunks * sizeof(ReduceWork) + 256)) return e;
     ReduceWork* rw_host = (ReduceWork*)ctx->res.pinned;
     ReduceWork* rw_dev = (ReduceWork*)ctx->res.table;
-    const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, tun.block_size ? tun.block_size / 64 * 64 : MAX_BLOCK));
+    const uint32_t block = launch_block(tun.block_size, hr.n_tiles);
     const uint32_t waves_per_block = block / 64;
     const dim3 grid((hr.n_tiles + waves_per_block - 1) / waves_per_block);
     // timing events, kept with the prepared request so that repeated executions do not create them again

@@ -108,9 +108,9 @@ static void plan_iteration(const std::vector<Slot*>& slots, IterPlan& P) {
     P.partial_doubles = off;
 }
 
-// persistent grid for one kind
-static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning& t) {
-    const uint32_t wpb = std::max<uint32_t>(1, std::min<uint32_t>(MAX_BLOCK, t.block_size) / 64);
+// persistent grid for one kind, in work-groups of `block` threads
+static dim3 persistent_grid(const sp_ctx* ctx, uint32_t n_work, const sp_tuning& t, uint32_t block) {
+    const uint32_t wpb = std::max<uint32_t>(1, block / 64);
     const uint32_t per_simd = std::max<uint32_t>(1, t.persistent_waves_per_simd);
     const uint32_t resident = (uint32_t)ctx->prop.multiProcessorCount * 4u * per_simd;   // waves
     const uint32_t waves = std::min(n_work, resident);
@@ -134,8 +134,8 @@ static int record_iteration(sp_ctx* ctx, IterRes& R, hipStream_t s, const IterPl
     }
     for (size_t k = 0; k < P.kinds.size(); ++k) {
         const sp_tuning& t = tun[k];
-        const uint32_t block = std::max<uint32_t>(64, std::min<uint32_t>(MAX_BLOCK, t.block_size / 64 * 64));
-        const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t);
+        const uint32_t block = launch_block(t.block_size, P.kind_tiles[k]);
+        const dim3 grid = persistent_grid(ctx, P.kind_tiles[k], t, block);
         hipStream_t sk = (k == 0 || R.n_side == 0) ? s : R.side[(k - 1) % R.n_side];
         const bool timed = timing && k < (size_t)IterRes::N_CLASS_EV;
         if (timed) SP_HIP(hipEventRecord(R.class_ev[2 * k], sk));
