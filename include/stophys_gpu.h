@@ -334,7 +334,9 @@ int sp_settings_submit(sp_ctx* ctx, const sp_settings* s, uint64_t* id);
  * device provider drew */
 int sp_prepared_inputs(const sp_prepared* h, double* params, double* initial_state);
 
-/* ---- autotuner: times candidate tunings on a short synthetic run ---- */
+/* ---- autotuner: times candidate tunings (work-group size, persistent waves per SIMD, chunk length) on a shortened
+ * copy of the request, the four fastest again on a copy up to four times longer, and stores the fastest for the
+ * request's class; best_rate = its device rate in particle-steps/s (launches only) ---- */
 int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts,
                 sp_tuning* best, double* best_rate);
 

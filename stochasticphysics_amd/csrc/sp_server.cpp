@@ -995,8 +995,10 @@ int sp_server_stats(sp_ctx* ctx, double* out) {
 }
 
 // Autotuner: times the batch path on a shortened copy of the request for each
-// candidate (work-group size x persistent waves per SIMD x chunk length) and
-// stores the fastest for the request's (problem, solver, precision) class.
+// candidate (work-group size x persistent waves per SIMD x chunk length), times
+// the four fastest again on a longer copy, and stores the fastest for the
+// request's (problem, solver, precision) class.  Rates are device rates (the
+// launches between two events).
 int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, sp_tuning* best, double* best_rate) {
     if (!ctx || !req || !parts || !best) return set_error(SP_ERR_INVALID_ARG, "NULL argument");
     HostRequest base;
@@ -1024,35 +1026,54 @@ int sp_autotune(sp_ctx* ctx, const sp_request* req, const sp_particles* parts, s
         auto it = ctx->tuning.find(key);
         if (it != ctx->tuning.end()) { had_prev = true; prev = it->second; }
     }
-    auto trial = [&](const sp_tuning& t) -> double {
+    // the device rate of one run of request q (the event-timed launches of run_batch_static: allocation, uploads,
+    // graph instantiation and downloads excluded, which a trial of tens of milliseconds would otherwise be noisy with)
+    auto trial = [&](const sp_tuning& t, const sp_request& q, double ps) -> double {
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             ctx->tuning[tuning_key(r.problem, r.solver, prec)] = t;
         }
         std::vector<HostRequest> one(1);
-        if ((err = prepare_request(&r, parts, one[0]))) return -1.0;
+        if ((err = prepare_request(&q, parts, one[0]))) return -1.0;
         std::vector<Outputs> outs;
-        const auto t0 = Clock::now();
         if ((err = run_batch_static(ctx, one, outs, false, t.use_graph != 0))) return -1.0;
-        const double ms = ms_since(t0, Clock::now());
-        return (double)probe.N * (double)probe.total_steps / (ms * 1e-3);
+        const double ms = ctx->last_stats[1];
+        return ms > 0.0 ? ps / (ms * 1e-3) : -1.0;
     };
+    auto fail = [&]() {                   // a failed trial ends the search with its error
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (had_prev) ctx->tuning[key] = prev;     // the caller's tuning stays as it was
+        else ctx->tuning.erase(key);
+        return err;
+    };
+    const double probe_ps = (double)probe.N * (double)probe.total_steps;
+    std::vector<std::pair<double, sp_tuning>> cand;
     for (uint32_t c : chunks)
         for (uint32_t b : blocks)
             for (uint32_t w : waves) {
                 sp_tuning t{b, c, 1, w};
-                trial(t);                        // warm (allocations, code load)
-                // the better of two timed runs: a single short run is disturbed now and then by the host
-                double rate = err == SP_OK ? trial(t) : -1.0;
-                if (err == SP_OK) rate = std::max(rate, trial(t));
-                if (err != SP_OK) {               // a failed trial ends the search with its error
-                    std::lock_guard<std::mutex> lk(ctx->mu);
-                    if (had_prev) ctx->tuning[key] = prev;     // the caller's tuning stays as it was
-                    else ctx->tuning.erase(key);
-                    return err;
-                }
-                if (rate > best_r) { best_r = rate; bt = t; }
+                trial(t, r, probe_ps);           // warm (allocations, code load)
+                // the better of two timed runs
+                double rate = err == SP_OK ? trial(t, r, probe_ps) : -1.0;
+                if (err == SP_OK) rate = std::max(rate, trial(t, r, probe_ps));
+                if (err != SP_OK) return fail();
+                cand.emplace_back(rate, t);
             }
+    // the four fastest again, on a run up to four times longer: the short trials rank them only roughly
+    std::sort(cand.begin(), cand.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    sp_request r2 = *req;
+    r2.flags |= SP_FLAG_SKIP_CHECK;
+    if (r2.n_steps > 4 * max_steps) r2.n_steps = 4 * max_steps;
+    HostRequest probe2;
+    if (int e = prepare_request(&r2, parts, probe2)) return e;
+    const double probe2_ps = (double)probe2.N * (double)probe2.total_steps;
+    for (size_t i = 0; i < std::min<size_t>(4, cand.size()); ++i) {
+        const sp_tuning& t = cand[i].second;
+        double rate = trial(t, r2, probe2_ps);
+        if (err == SP_OK) rate = std::max(rate, trial(t, r2, probe2_ps));
+        if (err != SP_OK) return fail();
+        if (rate > best_r) { best_r = rate; bt = t; }
+    }
     {
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->tuning[tuning_key(r.problem, r.solver, prec)] = bt;
