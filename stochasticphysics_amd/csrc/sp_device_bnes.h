@@ -9,29 +9,7 @@ namespace sp {
 
 template <class R> struct Trig5 { R s[5], c[5]; };
 
-__device__ __forceinline__ void trig5(const double* y, Trig5<double>& T) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i) sincos_fast(y[i], &T.s[i], &T.c[i]);
-}
-// mixed precision: reduction to one turn in double, hardware sin/cos (they take revolutions)
-__device__ __forceinline__ void trig5(const double* y, Trig5<float>& T) {
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double r = y[i] * INV_TWO_PI;
-        r -= floor(r);
-        const float f = (float)r;
-        T.s[i] = __builtin_amdgcn_sinf(f);
-        T.c[i] = __builtin_amdgcn_cosf(f);
-    }
-}
-// near a chart switch the trig is taken in double whatever the precision of the bulk
-template <class R> __device__ __forceinline__ void trig5_exact(const double* y, Trig5<R>& T) {
-    Trig5<double> Td;
-    trig5(y, Td);
-#pragma unroll
-    for (int i = 0; i < 5; ++i) { T.s[i] = (R)Td.s[i]; T.c[i] = (R)Td.c[i]; }
-}
-// the same for the angles [I0, I1) only
+// the sines and cosines of the angles [I0, I1) in double whatever the precision of the bulk (near a chart switch)
 template <int I0, int I1, class R> __device__ __forceinline__ void trig_exact(const double* y, Trig5<R>& T) {
 #pragma unroll
     for (int i = I0; i < I1; ++i) {
@@ -40,6 +18,21 @@ template <int I0, int I1, class R> __device__ __forceinline__ void trig_exact(co
         T.s[i] = (R)sd; T.c[i] = (R)cd;
     }
 }
+// the same as the step takes them in the standard charts: the table-driven FP64 sin/cos, or in the mixed mode the
+// hardware FP32 ones after a reduction to one turn in double (they take revolutions)
+template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<double>& T) { trig_exact<I0, I1>(y, T); }
+template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<float>& T) {
+#pragma unroll
+    for (int i = I0; i < I1; ++i) {
+        double r = y[i] * INV_TWO_PI;
+        r -= floor(r);
+        const float f = (float)r;
+        T.s[i] = __builtin_amdgcn_sinf(f);
+        T.c[i] = __builtin_amdgcn_cosf(f);
+    }
+}
+template <class R> __device__ __forceinline__ void trig5(const double* y, Trig5<R>& T) { trig_std<0, 5>(y, T); }
+template <class R> __device__ __forceinline__ void trig5_exact(const double* y, Trig5<R>& T) { trig_exact<0, 5>(y, T); }
 
 template <int ANISO, class R> struct BNESLane {
     R a, bpd, c, d;              // NeelFactor1, b + d, BrownPrefactor, d = c mu
@@ -281,20 +274,6 @@ __device__ __forceinline__ void rotated_to_neel(const double* y, double* r) {
     sincos_fast(y[0], &st, &ct); sincos_fast(y[1], &sp, &cp);
     rotated_to_neel(st, ct, sp, cp, r);
 }
-// the sines and cosines of the angles [I0, I1) as the step computes them in the standard charts: the table-driven
-// FP64 sin/cos, or in the mixed mode the hardware FP32 ones after a reduction to one turn in double (as trig5)
-template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<double>& T) { trig_exact<I0, I1>(y, T); }
-template <int I0, int I1> __device__ __forceinline__ void trig_std(const double* y, Trig5<float>& T) {
-#pragma unroll
-    for (int i = I0; i < I1; ++i) {
-        double r = y[i] * INV_TWO_PI;
-        r -= floor(r);
-        const float f = (float)r;
-        T.s[i] = __builtin_amdgcn_sinf(f);
-        T.c[i] = __builtin_amdgcn_cosf(f);
-    }
-}
-
 template <int ANISO, int SOLVER, class R> struct BNESCfg {
     static constexpr int DY = 5, DW = 6, DOUT = 9, NORM = 0;
     static constexpr int NZ = noise_per_step<SOLVER, DW>(), G = 12 / NZ;
