@@ -380,7 +380,7 @@ def neel_relaxation(ctx, configs):
 
 
 def problem_rates(ctx, configs, n=65_536, n_steps=20_000):
-    """FP64 and mixed throughput of every problem class on one 65,536-particle ensemble each (20,000 steps, inputs
+    """FP64 and mixed throughput of every problem class on one n-particle ensemble each (20,000 steps, inputs
     resident): Neel EulerMaruyama, NeelSpherical Heun_Strong (cubic), BrownAndNeel Heun_Strong, BrownAndNeelEulerSpherical
     Heun_Strong (both charts on), Neel and NeelSpherical with uniaxialcubic anisotropy; each rate is the best of three
     timed runs after an untimed one."""
@@ -570,6 +570,7 @@ def main():
                          "of the headline one)")
     ap.add_argument("--no-provider", action="store_true", help="headline run without the particle-provider leg")
     ap.add_argument("--particles", type=int, default=1_048_576, help="precision study: particles")
+    ap.add_argument("--problem-particles", type=int, default=65_536, help="per-problem leg: particles per ensemble")
     ap.add_argument("--study-steps", type=int, default=500_000, help="precision study: steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (the reference kernel that calibrates the "
@@ -597,7 +598,7 @@ def main():
         if args.workload == "precision":
             res = precision_study(ctx, configs, args.particles, args.study_steps)
         elif args.workload == "problems":
-            res = problem_rates(ctx, configs)
+            res = problem_rates(ctx, configs, n=args.problem_particles)
         elif args.workload == "provider":
             res = provider_leg(ctx)
         elif args.workload == "implicit":
